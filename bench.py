@@ -1,0 +1,180 @@
+"""Benchmark of the MI355X particle mover: FP64 particle-steps/s of push + tet walk + scatter.
+
+Workload (one GPU): a 60x60x60 six-tet box of side 1 m (1,296,000 tets, 226,981 nodes),
+20,000,000 FP64 particles uniform in the box, Maxwellian velocities (sigma 2000 m/s) plus a
+5000 m/s +x drift, frozen field from phi = 10 V sin(pi x) sin(pi y) sin(pi z), O+ ions
+(16 amu, +1 e), dt = 2.75e-7 s (about 30% of particles cross a face per step), absorbing
+walls, cell sort every 10 steps.  Particles stay resident in HBM between steps.
+
+Prints ONE JSON line.  value = particle-steps/s over the timed steps (whole step: push,
+walk, scatter, normalisation, periodic sort).  roofline = the push kernel's algorithmic
+HBM bytes per launch (104 B per active particle + 192 B per tet + 8 B per node) over its
+average launch time from HIP events on the context stream.  cpu_baseline = the oracle's
+OpenMP port of the reference kernel body on the host cores, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mini-fem-pic-main_amd")
+for p in (PKG, ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import minipic_amd as mp  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+N_HEX, SIDE, N_PART = 60, 1.0, 20_000_000
+SIGMA, DRIFT, AMP = 2000.0, 5000.0, 10.0
+DT = 2.75e-7
+SPWT = 200.0
+QM = mp.QE / (16 * mp.AMU)
+SEED = 20240601
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
+    return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 8.0 * n_nodes)
+
+
+def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval):
+    import torch
+
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=sort_interval)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for _ in range(warmup):
+            mv.step(QM, DT, SPWT, mode)
+        mv.sync()
+        mv.reset_counters()
+        mv.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            mv.step(QM, DT, SPWT, mode)
+        mv.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        push_ms, launches = mv.timing()
+        mv.set_timing(False)
+        c = mv.counters()
+    wall = t1 - t0
+    return {"wall_s": wall, "particle_steps": c["particle_steps"], "rate": c["particle_steps"] / wall,
+            "push_ms": push_ms, "launches": launches, "counters": c}
+
+
+def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30):
+    import oracle
+
+    tets = mesh.tets
+    p, v, c = pos.copy(), vel.copy(), cell.copy()
+    den = np.zeros(mesh.n_nodes)
+    done, steps, t0 = 0, 0, time.perf_counter()
+    threads = 0
+    while steps < max_steps:
+        den[:] = 0
+        _, n, threads = oracle.move_omp(tets, ef, p, v, c, QM, DT, mesh.n_nodes, den=den)
+        done += n
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    wall = time.perf_counter() - t0
+    return {"value": done / wall, "steps": steps, "threads": threads, "wall_s": wall}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", choices=["atomic", "deterministic"], default="atomic")
+    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--det-steps", type=int, default=20, help="timed steps of the deterministic-mode line (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--particles", type=int, default=N_PART)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
+              "MI355X; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    mesh = mp.Mesh.box_mesh(N_HEX, SIDE)
+    ef = mesh.field_sinusoid(AMP)
+    n = args.particles
+    pos, vel = mp.synth_uniform(SEED, n, [0, 0, 0], [SIDE] * 3, SIGMA, [DRIFT, 0, 0])
+    cell = mesh.locate_box(pos)
+    log(f"setup {time.perf_counter() - t:.1f}s: {mesh.n_tets} tets, {mesh.n_nodes} nodes, {n} particles")
+
+    mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
+    r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval)
+    log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
+    avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
+    bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
+    achieved = bytes_per_launch / avg_push_s / 1e9
+    extra = {}
+    if args.det_steps > 0 and mode == mp.SCATTER_ATOMIC:
+        d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval)
+        extra["deterministic"] = {"value": d["rate"], "ms_per_step": 1e3 * d["wall_s"] / args.det_steps,
+                                  "steps": args.det_steps}
+        log(f"gpu deterministic: {d['rate'] / 1e9:.3f} G particle-steps/s")
+    cpu = None
+    if args.cpu_seconds > 0:
+        cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)
+        cpu = {"value": cb["value"], "unit": "particle-steps/s", "cores": cb["threads"], "kind": "port",
+               "sample": f"the same 20M particles and mesh, {cb['steps']} steps from the same initial state "
+                         f"({cb['wall_s']:.1f} s), OpenMP port of the reference kernel body with atomic "
+                         f"density adds on {cb['threads']} threads of {cpu_model()}"}
+        log(f"cpu baseline: {cb['value'] / 1e6:.2f} M particle-steps/s on {cb['threads']} threads")
+    c = r["counters"]
+    out = {
+        "metric": "FP64 particle-steps/s (push+tet-walk+scatter)",
+        "value": r["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["wall_s"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 + Box-Muller on the host)",
+        "config": {"workload": "push-only scale test: 60^3 six-tet box, side 1 m (1,296,000 tets, 226,981 nodes), "
+                               "20M FP64 particles, frozen sinusoidal field, absorbing walls",
+                   "particles": n, "tets": mesh.n_tets, "nodes": mesh.n_nodes, "dt": DT,
+                   "scatter": args.mode, "sort_interval": args.sort_interval,
+                   "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
+                   "lost": c["lost"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push_soa", "avg_launch_ms": avg_push_s * 1e3,
+                     "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * mp_abi.h - C ABI of the MI355X particle mover (push + tet walk + nodal scatter).
+ *
+ * This is the drop-in boundary for mini-pic's MoveParticles hot path.  Every entry
+ * point takes plain pointers and sizes, returns an int status (MP_OK == 0) and never
+ * throws.  A context owns its device buffers and one HIP stream; all work is
+ * stream-ordered on that stream.  One context per host thread, or serialise calls.
+ *
+ * Reference interfaces replaced (AD1641/mini-fem-pic-main):
+ *   src/mini-pic.cpp:39-53    struct DeviceData (ctor uploads Tetra table, owns buffers)
+ *   src/mini-pic.cpp:268-320  MoveParticles (copy in, kernel, copy out, erase_if, normalise)
+ *   src/mini-pic.cpp:287-308  the device kernel: push, XtoLtet walk, FP64 atomic scatter
+ *   src/mini-pic.cpp:348-355  DeviceData::reserveParticles (grow to 2n)
+ *   src/meshes.h:75-100       XtoLtet barycentric walk
+ *   src/particles.h:20-25     struct Particle (88 B layout kept: mp_particle)
+ *   src/meshes.h:33-43        struct Tetra    (168 B layout kept: mp_tetra)
+ */
+#ifndef MP_ABI_H
+#define MP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define MP_OK            0
+#define MP_ERR_ARG       1   /* bad argument (null pointer, size out of range, ...)  */
+#define MP_ERR_HIP       2   /* HIP runtime error                                     */
+#define MP_ERR_ALLOC     3   /* device or host allocation failed                      */
+#define MP_ERR_STATE     4   /* call out of order (no field uploaded, no particles)   */
+#define MP_ERR_NODEV     5   /* no usable gfx950 device                               */
+
+/* ---- modes -------------------------------------------------------------------------- */
+#define MP_SCATTER_ATOMIC        0  /* FP64 atomics, order nondeterministic (reference)  */
+#define MP_SCATTER_DETERMINISTIC 1  /* sort (node,pid) then serial fold per node          */
+
+#define MP_WALL_ABSORB   0  /* walk exits through a boundary face: particle is lost (reference) */
+#define MP_WALL_REFLECT  1  /* specular reflection about the boundary face plane (extension)    */
+
+/* ---- records with the reference byte layout ----------------------------------------- */
+/* Tetra, src/meshes.h:33-43: con@0 volume@16 alpha@24 beta@56 gamma@88 delta@120 cell_con@152 */
+typedef struct mp_tetra {
+    int32_t con[4];       /* node ids                                       */
+    double  volume;       /* tet volume (positive)                          */
+    double  alpha[4];     /* precomputed 3x3 determinants for barycentrics  */
+    double  beta[4];
+    double  gamma[4];
+    double  delta[4];
+    int32_t cell_con[4];  /* neighbour across the face opposite node i, -1 = boundary */
+} mp_tetra;
+
+/* Particle, src/particles.h:20-25: pos@0 vel@24 lc@48 cell_index@80, size 88 */
+typedef struct mp_particle {
+    double  pos[3];
+    double  vel[3];
+    double  lc[4];        /* barycentric weights, written by the walk       */
+    int32_t cell_index;   /* containing tet, -1 = left the domain           */
+    int32_t _pad;
+} mp_particle;
+
+/* Tuning knobs of the device-resident path (mp_step). */
+typedef struct mp_options {
+    int32_t sort_interval;         /* cell-sort + compact every k steps (0 = never)    */
+    int32_t wall_mode;             /* MP_WALL_ABSORB / MP_WALL_REFLECT                 */
+    int32_t block_size;            /* threads per workgroup of the push kernel         */
+    int32_t threads_per_particle;  /* 1 or 4 (4: one lane per barycentric coordinate)  */
+    int32_t particles_per_thread;  /* 1, 2 or 4 (TPP=1 only)                           */
+    int32_t walk_cap;              /* max face moves per particle per step             */
+} mp_options;
+
+/* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
+typedef struct mp_counters {
+    int64_t particle_steps;   /* sum over steps of particles active at step start   */
+    int64_t lost;             /* particles that left the domain                     */
+    int64_t crossed;          /* particle-steps that moved to at least one other tet */
+    int64_t moves;            /* total face moves (+ reflections)                   */
+    int64_t cap_hits;         /* particles stopped by walk_cap (treated as lost)    */
+    int64_t reflections;      /* wall reflections (MP_WALL_REFLECT)                 */
+} mp_counters;
+
+typedef struct mp_ctx mp_ctx;
+
+/* ---- lifetime ---------------------------------------------------------------------- */
+/* DeviceData::DeviceData (src/mini-pic.cpp:322-338): checks FP64 atomics, uploads the
+ * Tetra table once.  node_volume[n] is Node::volume (src/meshes.h:29). */
+int  mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume,
+               int32_t n_nodes, mp_ctx** out);
+/* DeviceData::~DeviceData (src/mini-pic.cpp:340-345). */
+void mp_destroy(mp_ctx* ctx);
+/* Message of the last failed call on ctx (ctx == NULL: last failed mp_create). */
+const char* mp_last_error(const mp_ctx* ctx);
+
+int  mp_get_options(const mp_ctx* ctx, mp_options* opt);
+int  mp_set_options(mp_ctx* ctx, const mp_options* opt);
+
+/* FESolver::ef (src/FESolver.h:49): per-element E, ef[n_tets][3], copied to the device.
+ * MoveParticles copies it every step (src/mini-pic.cpp:276). */
+int  mp_upload_field(mp_ctx* ctx, const double* ef, int32_t n_tets);
+
+/* ---- drop-in step over the reference's particle array -------------------------------- */
+/* MoveParticles (src/mini-pic.cpp:268-320) on an AoS Particle array in host memory:
+ * uploads n particles, pushes, walks and scatters on the device, downloads, removes the
+ * particles with cell_index < 0 keeping the order of the others (std::erase_if), and
+ * writes den_out[n_nodes] = raw * (spwt / node_volume).  *n_inout becomes the number of
+ * particles kept; parts[0..*n_inout) holds them. */
+int  mp_step_aos(mp_ctx* ctx, mp_particle* parts, size_t* n_inout, double q_over_m,
+                 double dt, double spwt, double* den_out, int32_t mode);
+
+/* ---- device-resident path (structure of arrays in HBM) ------------------------------- */
+/* Replace the resident particle set.  pos, vel: [n][3]; cell: [n].  Particle ids (pid)
+ * are 0..n-1 in array order; pid order is the order a serial MoveParticles sees. */
+int  mp_set_particles(mp_ctx* ctx, const double* pos, const double* vel,
+                      const int32_t* cell, int64_t n);
+/* Append particles (injection); they get the next pids. */
+int  mp_add_particles(mp_ctx* ctx, const double* pos, const double* vel,
+                      const int32_t* cell, int64_t n);
+/* One MoveParticles step on the resident set: push + walk + scatter, then the nodal
+ * density normalised by spwt / node_volume.  Stream-ordered, does not block the host. */
+int  mp_step(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode);
+/* Block until all queued work on the context's stream is done. */
+int  mp_sync(mp_ctx* ctx);
+/* Number of resident particles still in the domain (blocks). */
+int  mp_num_active(mp_ctx* ctx, int64_t* n_active);
+/* Download the particles still in the domain in ascending pid order (blocks).
+ * Any output pointer may be NULL.  *n_inout: capacity in, count out. */
+int  mp_get_particles(mp_ctx* ctx, double* pos, double* vel, int32_t* cell, int64_t* pid,
+                      int64_t* n_inout);
+/* Normalised nodal density of the last step, den[n_nodes] (blocks). */
+int  mp_get_density(mp_ctx* ctx, double* den);
+int  mp_get_counters(mp_ctx* ctx, mp_counters* out);
+int  mp_reset_counters(mp_ctx* ctx);
+/* Sort the resident particles by cell and drop the lost ones now. */
+int  mp_sort_now(mp_ctx* ctx);
+/* The context's hipStream_t, for callers that capture or time its work. */
+void* mp_stream(mp_ctx* ctx);
+
+/* Push-kernel timing with HIP events recorded on the context's stream around every push
+ * launch while enabled.  mp_set_timing(ctx, 1) clears and starts recording, 0 stops.
+ * mp_get_timing blocks and returns the summed push-kernel time (ms) and launch count. */
+int  mp_set_timing(mp_ctx* ctx, int32_t enable);
+int  mp_get_timing(mp_ctx* ctx, double* push_ms, int64_t* push_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MP_ABI_H */

@@ -1,0 +1,647 @@
+// C ABI of the particle mover (declared in include/mp_abi.h): context, device buffers,
+// per-step orchestration of the kernels in mp_kernels.hip.
+//
+// Reference behaviour mirrored:
+//   DeviceData (src/mini-pic.cpp:39-53, 322-355): one device context owning the Tetra table,
+//     field, density and particle buffers; particle capacity grows to 2n on overflow.
+//   MoveParticles (src/mini-pic.cpp:268-320): mp_step_aos does copy-in, kernel, copy-out,
+//     stable erase of cell_index < 0 and den *= spwt / node volume, blocking like q.wait().
+// The device-resident path (mp_step) keeps particles in HBM as structure of arrays with a
+// particle id (pid) per particle; pid order is the order a serial MoveParticles would see.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mp_abi.h"
+#include "mp_launch.h"
+
+using mpd::DevTet;
+using mpd::Soa;
+
+struct mp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t n_tets = 0, n_nodes = 0;
+    DevTet* tets = nullptr;
+    double* node_vol = nullptr;
+    double* den_raw = nullptr;
+    double* den_out = nullptr;
+    unsigned long long* ctr = nullptr;
+    bool have_field = false;
+    // resident particles, double-buffered for the cell sort
+    void* soa_mem[2] = {nullptr, nullptr};
+    Soa soa[2] = {};
+    int cur = 0;
+    int64_t cap = 0;
+    int64_t n_slots = 0;
+    int64_t next_pid = 0;
+    int64_t since_sort = 0;
+    // counting-sort and scan scratch
+    int32_t* cell_cnt = nullptr;
+    int32_t* cell_cur = nullptr;
+    int32_t* scan_tmp = nullptr;
+    int64_t scan_tmp_n = 0;
+    // deterministic-scatter scratch
+    uint64_t* keys[2] = {nullptr, nullptr};
+    double* vals[2] = {nullptr, nullptr};
+    int64_t rec_cap = 0;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int32_t* seg_start = nullptr;
+    // drop-in AoS scratch
+    mp_particle* aos = nullptr;
+    mp_particle* aos_out = nullptr;
+    int32_t* flags = nullptr;
+    int32_t* offs = nullptr;
+    int64_t aos_cap = 0;
+    int32_t* pinned_i32 = nullptr;
+    // push-kernel timing (events recorded around each push launch while enabled)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    mp_options opt{};
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int set_err(mp_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e__ = (expr);                                                            \
+        if (e__ != hipSuccess)                                                              \
+            return set_err((ctx), MP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e__)); \
+    } while (0)
+
+// Records an event on the context stream from the pool (grows on demand).
+hipError_t record_event(mp_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        c->ev_pool.push_back(e);
+    }
+    return hipEventRecord(c->ev_pool[c->ev_used++], c->stream);
+}
+
+int bit_width(uint64_t v) {
+    int b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)n);
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+mp_options default_options() {
+    mp_options o;
+    o.sort_interval = 10;
+    o.wall_mode = MP_WALL_ABSORB;
+    o.block_size = 256;
+    o.threads_per_particle = 1;
+    o.particles_per_thread = 1;
+    o.walk_cap = 4096;
+    return o;
+}
+
+// Lays out one SoA buffer of capacity n in a single allocation.
+hipError_t alloc_soa(void** mem, Soa* s, int64_t n) {
+    const size_t a = 256;
+    auto up = [&](size_t b) { return (b + a - 1) / a * a; };
+    const size_t d = up(sizeof(double) * (size_t)n), i = up(sizeof(int32_t) * (size_t)n);
+    hipError_t e = hipMalloc(mem, 6 * d + 2 * i);
+    if (e != hipSuccess) {
+        *mem = nullptr;
+        return e;
+    }
+    char* b = (char*)*mem;
+    s->x = (double*)(b + 0 * d);
+    s->y = (double*)(b + 1 * d);
+    s->z = (double*)(b + 2 * d);
+    s->vx = (double*)(b + 3 * d);
+    s->vy = (double*)(b + 4 * d);
+    s->vz = (double*)(b + 5 * d);
+    s->cell = (int32_t*)(b + 6 * d);
+    s->pid = (int32_t*)(b + 6 * d + i);
+    return hipSuccess;
+}
+
+int reserve_soa(mp_ctx* c, int64_t n, bool keep) {
+    if (n <= c->cap) return MP_OK;
+    const int64_t nc = std::max<int64_t>(2 * n, 1024);
+    void* mem[2] = {nullptr, nullptr};
+    Soa s[2];
+    for (int k = 0; k < 2; k++) {
+        if (alloc_soa(&mem[k], &s[k], nc) != hipSuccess) {
+            dfree(mem[0]);
+            (void)hipGetLastError();
+            return set_err(c, MP_ERR_ALLOC, "Failed to allocate " + std::to_string(nc) + " particles on the device");
+        }
+    }
+    if (keep && c->n_slots > 0) {
+        const Soa& o = c->soa[c->cur];
+        const size_t nd = sizeof(double) * (size_t)c->n_slots, ni = sizeof(int32_t) * (size_t)c->n_slots;
+        HIP_TRY(c, hipMemcpyAsync(s[0].x, o.x, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].y, o.y, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].z, o.z, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].vx, o.vx, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].vy, o.vy, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].vz, o.vz, nd, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].cell, o.cell, ni, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(s[0].pid, o.pid, ni, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    dfree(c->soa_mem[0]);
+    dfree(c->soa_mem[1]);
+    c->soa_mem[0] = mem[0];
+    c->soa_mem[1] = mem[1];
+    c->soa[0] = s[0];
+    c->soa[1] = s[1];
+    c->cur = 0;
+    c->cap = nc;
+    return MP_OK;
+}
+
+int reserve_scan(mp_ctx* c, int64_t n) {
+    const int64_t need = mpd::scan_scratch_ints(n);
+    if (need <= c->scan_tmp_n) return MP_OK;
+    dfree(c->scan_tmp);
+    c->scan_tmp = nullptr;
+    c->scan_tmp_n = 0;
+    if (dalloc(&c->scan_tmp, need) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "scan scratch allocation failed");
+    c->scan_tmp_n = need;
+    return MP_OK;
+}
+
+int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
+    if (m > c->rec_cap) {
+        for (int k = 0; k < 2; k++) {
+            dfree(c->keys[k]);
+            dfree(c->vals[k]);
+            c->keys[k] = nullptr;
+            c->vals[k] = nullptr;
+        }
+        c->rec_cap = 0;
+        const int64_t nc = std::max<int64_t>(m + m / 4, 4096);
+        for (int k = 0; k < 2; k++)
+            if (dalloc(&c->keys[k], nc) != hipSuccess || dalloc(&c->vals[k], nc) != hipSuccess)
+                return set_err(c, MP_ERR_ALLOC, "deterministic scatter buffers: allocation failed");
+        c->rec_cap = nc;
+    }
+    const size_t need = mpd::radix_sort_temp_bytes(c->rec_cap, end_bit);
+    if (need > c->sort_tmp_bytes) {
+        dfree(c->sort_tmp);
+        c->sort_tmp = nullptr;
+        c->sort_tmp_bytes = 0;
+        if (hipMalloc(&c->sort_tmp, need) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "radix sort scratch allocation failed");
+        c->sort_tmp_bytes = need;
+    }
+    if (!c->seg_start && dalloc(&c->seg_start, c->n_nodes) != hipSuccess)
+        return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
+    return MP_OK;
+}
+
+// Deterministic scatter tail: sort the m records on their used bits, fold per node.
+int det_finish(mp_ctx* c, int64_t m, int pid_bits, double spwt) {
+    const int end_bit = pid_bits + bit_width((uint64_t)c->n_nodes);
+    HIP_TRY(c, mpd::radix_sort_pairs(c->sort_tmp, c->sort_tmp_bytes, c->keys[0], c->keys[1], c->vals[0],
+                                     c->vals[1], m, end_bit, c->stream));
+    HIP_TRY(c, mpd::launch_det_fold(c->keys[1], c->vals[1], m, pid_bits, c->seg_start, c->node_vol,
+                                    c->n_nodes, spwt, c->den_out, c->stream));
+    return MP_OK;
+}
+
+int sort_now(mp_ctx* c) {
+    c->since_sort = 0;
+    if (c->n_slots == 0) return MP_OK;
+    const int64_t nt1 = (int64_t)c->n_tets + 1;
+    int rc = reserve_scan(c, nt1);
+    if (rc) return rc;
+    HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nt1, 0, c->stream));
+    const Soa& src = c->soa[c->cur];
+    const Soa& dst = c->soa[1 - c->cur];
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, c->n_slots, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nt1, c->cell_cur, c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + c->n_tets, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, c->cell_cur, dst, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cur = 1 - c->cur;
+    c->n_slots = *c->pinned_i32;
+    return MP_OK;
+}
+
+bool good_options(const mp_options& o) {
+    return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
+           (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
+           o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, int32_t n_nodes, mp_ctx** out) {
+    if (!out) return set_err(nullptr, MP_ERR_ARG, "mp_create: out is null");
+    *out = nullptr;
+    if (!tets || !node_volume || n_tets <= 0 || n_nodes <= 0)
+        return set_err(nullptr, MP_ERR_ARG, "mp_create: empty mesh or null table");
+    for (int32_t e = 0; e < n_tets; e++)
+        for (int v = 0; v < 4; v++) {
+            if (tets[e].con[v] < 0 || tets[e].con[v] >= n_nodes)
+                return set_err(nullptr, MP_ERR_ARG, "mp_create: tet " + std::to_string(e) + " has a node id out of range");
+            if (tets[e].cell_con[v] < -1 || tets[e].cell_con[v] >= n_tets)
+                return set_err(nullptr, MP_ERR_ARG, "mp_create: tet " + std::to_string(e) + " has a neighbour id out of range");
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return set_err(nullptr, MP_ERR_NODEV, "mp_create: no HIP device");
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return set_err(nullptr, MP_ERR_NODEV, "mp_create: cannot query the device");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(nullptr, MP_ERR_NODEV, std::string("mp_create: device ") + prop.gcnArchName +
+                                                  " is not gfx950 (this library is built for MI355X only)");
+    mp_ctx* c = new (std::nothrow) mp_ctx;
+    if (!c) return set_err(nullptr, MP_ERR_ALLOC, "mp_create: out of host memory");
+    c->device = dev;
+    c->n_tets = n_tets;
+    c->n_nodes = n_nodes;
+    c->opt = default_options();
+    std::vector<DevTet> h((size_t)n_tets);
+    for (int32_t e = 0; e < n_tets; e++) {
+        DevTet& d = h[e];
+        std::memset(&d, 0, sizeof(d));
+        for (int v = 0; v < 4; v++) {
+            d.c[v][0] = tets[e].alpha[v];
+            d.c[v][1] = tets[e].beta[v];
+            d.c[v][2] = tets[e].gamma[v];
+            d.c[v][3] = tets[e].delta[v];
+            d.nbr[v] = tets[e].cell_con[v];
+            d.con[v] = tets[e].con[v];
+        }
+        d.vol = tets[e].volume;
+    }
+    auto bail = [&](const std::string& m) {
+        mp_destroy(c);
+        return set_err(nullptr, MP_ERR_ALLOC, m);
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail("mp_create: stream creation failed");
+    if (dalloc(&c->tets, n_tets) != hipSuccess || dalloc(&c->node_vol, n_nodes) != hipSuccess ||
+        dalloc(&c->den_raw, n_nodes) != hipSuccess || dalloc(&c->den_out, n_nodes) != hipSuccess ||
+        dalloc(&c->ctr, mpd::kNumCounters) != hipSuccess || dalloc(&c->cell_cnt, (int64_t)n_tets + 1) != hipSuccess ||
+        dalloc(&c->cell_cur, (int64_t)n_tets + 1) != hipSuccess)
+        return bail("Failed to allocate the mesh on the device");
+    if (hipHostMalloc((void**)&c->pinned_i32, 16 * sizeof(int32_t), 0) != hipSuccess)
+        return bail("mp_create: pinned allocation failed");
+    if (hipMemcpy(c->tets, h.data(), sizeof(DevTet) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->node_vol, node_volume, sizeof(double) * (size_t)n_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->den_raw, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||
+        hipMemset(c->den_out, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||
+        hipMemset(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters) != hipSuccess)
+        return bail("mp_create: upload failed");
+    *out = c;
+    return MP_OK;
+}
+
+void mp_destroy(mp_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
+                    c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
+                    c->sort_tmp, c->seg_start, c->aos, c->aos_out, c->flags, c->offs};
+    for (void* p : bufs) dfree(p);
+    if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* mp_last_error(const mp_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int mp_get_options(const mp_ctx* c, mp_options* o) {
+    if (!c || !o) return MP_ERR_ARG;
+    *o = c->opt;
+    return MP_OK;
+}
+
+int mp_set_options(mp_ctx* c, const mp_options* o) {
+    if (!c || !o) return MP_ERR_ARG;
+    if (!good_options(*o)) return set_err(c, MP_ERR_ARG, "mp_set_options: unsupported option values");
+    c->opt = *o;
+    return MP_OK;
+}
+
+int mp_upload_field(mp_ctx* c, const double* ef, int32_t n_tets) {
+    if (!c) return MP_ERR_ARG;
+    if (!ef || n_tets != c->n_tets) return set_err(c, MP_ERR_ARG, "mp_upload_field: ef must be [n_tets][3]");
+    HIP_TRY(c, hipSetDevice(c->device));
+    double* tmp = nullptr;
+    HIP_TRY(c, dalloc(&tmp, 3 * (int64_t)n_tets));
+    hipError_t e = hipMemcpyAsync(tmp, ef, sizeof(double) * 3 * (size_t)n_tets, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = mpd::launch_set_field(c->tets, tmp, n_tets, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(tmp);
+    if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_upload_field: ") + hipGetErrorString(e));
+    c->have_field = true;
+    return MP_OK;
+}
+
+int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m, double dt, double spwt,
+                double* den_out, int32_t mode) {
+    if (!c) return MP_ERR_ARG;
+    if (!n_inout || !den_out || (*n_inout > 0 && !parts)) return set_err(c, MP_ERR_ARG, "mp_step_aos: null argument");
+    if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
+        return set_err(c, MP_ERR_ARG, "mp_step_aos: unknown scatter mode");
+    if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_step_aos: no field uploaded");
+    const int64_t n = (int64_t)*n_inout;
+    if (n > INT32_MAX / 4) return set_err(c, MP_ERR_ARG, "mp_step_aos: too many particles");
+    for (int64_t i = 0; i < n; i++)
+        if (parts[i].cell_index < 0 || parts[i].cell_index >= c->n_tets)
+            return set_err(c, MP_ERR_ARG, "mp_step_aos: particle " + std::to_string(i) + " has cell_index out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (n == 0) {  // nothing to move: the density is all zero
+        for (int32_t i = 0; i < c->n_nodes; i++) den_out[i] = 0.0;
+        return MP_OK;
+    }
+    if (n > c->aos_cap) {  // DeviceData::reserveParticles: grow to 2n
+        dfree(c->aos); dfree(c->aos_out); dfree(c->flags); dfree(c->offs);
+        c->aos = c->aos_out = nullptr;
+        c->flags = c->offs = nullptr;
+        c->aos_cap = 0;
+        const int64_t nc = 2 * n;
+        if (dalloc(&c->aos, nc) != hipSuccess || dalloc(&c->aos_out, nc) != hipSuccess ||
+            dalloc(&c->flags, nc) != hipSuccess || dalloc(&c->offs, nc + 1) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "Failed to allocate " + std::to_string(nc) + " particles on the device");
+        c->aos_cap = nc;
+    }
+    const bool det = mode == MP_SCATTER_DETERMINISTIC;
+    const int pid_bits = std::max(1, bit_width((uint64_t)std::max<int64_t>(n - 1, 1)));
+    if (det) {
+        int rc = reserve_records(c, 4 * n, pid_bits + bit_width((uint64_t)c->n_nodes));
+        if (rc) return rc;
+    }
+    int rc = reserve_scan(c, n + 1);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->aos, parts, sizeof(mp_particle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, mpd::launch_push_aos(c->tets, c->aos, n, q_over_m, dt, c->den_raw, det ? c->keys[0] : nullptr,
+                                    det ? c->vals[0] : nullptr, pid_bits, c->ctr, c->opt.walk_cap, det, c->stream));
+    if (det) {
+        rc = det_finish(c, 4 * n, pid_bits, spwt);
+        if (rc) return rc;
+    } else {
+        HIP_TRY(c, mpd::launch_normalise(c->den_raw, c->den_out, c->node_vol, c->n_nodes, spwt, c->stream));
+    }
+    // stable erase of cell_index < 0: flags -> exclusive scan -> compaction
+    HIP_TRY(c, mpd::launch_flag_aos(c->aos, n, c->flags, c->stream));
+    HIP_TRY(c, mpd::launch_fill_i32(c->flags + n, 1, 0, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(c->flags, n + 1, c->offs, c->scan_tmp, c->stream));
+    HIP_TRY(c, mpd::launch_compact_aos(c->aos, n, c->offs, c->aos_out, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->offs + n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(den_out, c->den_out, sizeof(double) * (size_t)c->n_nodes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int64_t kept = *c->pinned_i32;
+    if (kept > 0)
+        HIP_TRY(c, hipMemcpy(parts, c->aos_out, sizeof(mp_particle) * (size_t)kept, hipMemcpyDeviceToHost));
+    *n_inout = (size_t)kept;
+    return MP_OK;
+}
+
+int mp_set_particles(mp_ctx* c, const double* pos, const double* vel, const int32_t* cell, int64_t n) {
+    if (!c) return MP_ERR_ARG;
+    if (n < 0 || (n > 0 && (!pos || !vel || !cell))) return set_err(c, MP_ERR_ARG, "mp_set_particles: bad arrays");
+    if (n > INT32_MAX / 4) return set_err(c, MP_ERR_ARG, "mp_set_particles: too many particles");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->n_slots = 0;
+    c->next_pid = 0;
+    HIP_TRY(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters, c->stream));
+    return mp_add_particles(c, pos, vel, cell, n);
+}
+
+int mp_add_particles(mp_ctx* c, const double* pos, const double* vel, const int32_t* cell, int64_t n) {
+    if (!c) return MP_ERR_ARG;
+    if (n < 0 || (n > 0 && (!pos || !vel || !cell))) return set_err(c, MP_ERR_ARG, "mp_add_particles: bad arrays");
+    if (c->next_pid + n > INT32_MAX / 4) return set_err(c, MP_ERR_ARG, "mp_add_particles: too many particles");
+    for (int64_t i = 0; i < n; i++)
+        if (cell[i] < -1 || cell[i] >= c->n_tets)
+            return set_err(c, MP_ERR_ARG, "mp_add_particles: particle " + std::to_string(i) + " has cell out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = reserve_soa(c, c->n_slots + n, true);
+    if (rc) return rc;
+    if (n > 0) {
+        char* tmp = nullptr;
+        const size_t bp = sizeof(double) * 3 * (size_t)n, bc = sizeof(int32_t) * (size_t)n;
+        HIP_TRY(c, hipMalloc((void**)&tmp, 2 * bp + bc));
+        hipError_t e = hipMemcpyAsync(tmp, pos, bp, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tmp + bp, vel, bp, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tmp + 2 * bp, cell, bc, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = mpd::launch_unpack((double*)tmp, (double*)(tmp + bp), (int32_t*)(tmp + 2 * bp), n, c->next_pid,
+                                   c->soa[c->cur], c->n_slots, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(tmp);
+        if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_add_particles: ") + hipGetErrorString(e));
+    }
+    c->n_slots += n;
+    c->next_pid += n;
+    c->since_sort = c->opt.sort_interval;  // sort before the next step
+    return MP_OK;
+}
+
+int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
+    if (!c) return MP_ERR_ARG;
+    if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
+        return set_err(c, MP_ERR_ARG, "mp_step: unknown scatter mode");
+    if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_step: no field uploaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->opt.sort_interval > 0 && c->since_sort >= c->opt.sort_interval) {
+        int rc = sort_now(c);
+        if (rc) return rc;
+    }
+    const bool det = mode == MP_SCATTER_DETERMINISTIC;
+    const int pid_bits = std::max(1, bit_width((uint64_t)std::max<int64_t>(c->next_pid - 1, 1)));
+    if (det) {
+        int rc = reserve_records(c, 4 * c->n_slots, pid_bits + bit_width((uint64_t)c->n_nodes));
+        if (rc) return rc;
+    }
+    mpd::PushArgs a;
+    a.tets = c->tets;
+    a.soa = c->soa[c->cur];
+    a.n = c->n_slots;
+    a.qm = q_over_m;
+    a.dt = dt;
+    a.den = c->den_raw;
+    a.keys = det ? c->keys[0] : nullptr;
+    a.vals = det ? c->vals[0] : nullptr;
+    a.pid_bits = pid_bits;
+    a.ctr = c->ctr;
+    a.cap = c->opt.walk_cap;
+    a.wall = c->opt.wall_mode;
+    a.det = det;
+    a.block = c->opt.block_size;
+    if (c->timing) HIP_TRY(c, record_event(c));
+    HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
+    if (c->timing) HIP_TRY(c, record_event(c));
+    if (det) {
+        int rc = det_finish(c, 4 * c->n_slots, pid_bits, spwt);
+        if (rc) return rc;
+    } else {
+        HIP_TRY(c, mpd::launch_normalise(c->den_raw, c->den_out, c->node_vol, c->n_nodes, spwt, c->stream));
+    }
+    c->since_sort++;
+    return MP_OK;
+}
+
+int mp_sync(mp_ctx* c) {
+    if (!c) return MP_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_sort_now(mp_ctx* c) {
+    if (!c) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    return sort_now(c);
+}
+
+int mp_num_active(mp_ctx* c, int64_t* n_active) {
+    if (!c || !n_active) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<int32_t> cell((size_t)c->n_slots);
+    if (c->n_slots > 0)
+        HIP_TRY(c, hipMemcpyAsync(cell.data(), c->soa[c->cur].cell, sizeof(int32_t) * cell.size(),
+                                  hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int64_t k = 0;
+    for (int32_t v : cell) k += v >= 0;
+    *n_active = k;
+    return MP_OK;
+}
+
+int mp_get_particles(mp_ctx* c, double* pos, double* vel, int32_t* cell, int64_t* pid, int64_t* n_inout) {
+    if (!c || !n_inout) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t n = c->n_slots;
+    std::vector<double> hp(3 * (size_t)n), hv(3 * (size_t)n);
+    std::vector<int32_t> hc((size_t)n), hid((size_t)n);
+    if (n > 0) {
+        double* tmp = nullptr;
+        HIP_TRY(c, dalloc(&tmp, 6 * n));
+        hipError_t e = mpd::launch_pack(c->soa[c->cur], n, tmp, tmp + 3 * n, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hv.data(), tmp + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), c->soa[c->cur].cell, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hid.data(), c->soa[c->cur].pid, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(tmp);
+        if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_get_particles: ") + hipGetErrorString(e));
+    }
+    // ascending pid: place slots by pid, then walk the pid range
+    std::vector<int64_t> slot((size_t)c->next_pid, -1);
+    int64_t live = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (hc[i] >= 0) {
+            slot[(size_t)hid[i]] = i;
+            live++;
+        }
+    if (live > *n_inout) {
+        *n_inout = live;
+        return set_err(c, MP_ERR_ARG, "mp_get_particles: output capacity too small");
+    }
+    int64_t k = 0;
+    for (int64_t p = 0; p < c->next_pid; p++) {
+        const int64_t i = slot[(size_t)p];
+        if (i < 0) continue;
+        if (pos) for (int d = 0; d < 3; d++) pos[3 * k + d] = hp[3 * i + d];
+        if (vel) for (int d = 0; d < 3; d++) vel[3 * k + d] = hv[3 * i + d];
+        if (cell) cell[k] = hc[i];
+        if (pid) pid[k] = p;
+        k++;
+    }
+    *n_inout = k;
+    return MP_OK;
+}
+
+int mp_get_density(mp_ctx* c, double* den) {
+    if (!c || !den) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(den, c->den_out, sizeof(double) * (size_t)c->n_nodes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_get_counters(mp_ctx* c, mp_counters* out) {
+    if (!c || !out) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    unsigned long long h[mpd::kNumCounters];
+    HIP_TRY(c, hipMemcpyAsync(h, c->ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    out->particle_steps = (int64_t)h[mpd::kCtrParticleSteps];
+    out->lost = (int64_t)h[mpd::kCtrLost];
+    out->crossed = (int64_t)h[mpd::kCtrCrossed];
+    out->moves = (int64_t)h[mpd::kCtrMoves];
+    out->cap_hits = (int64_t)h[mpd::kCtrCapHits];
+    out->reflections = (int64_t)h[mpd::kCtrReflections];
+    return MP_OK;
+}
+
+int mp_reset_counters(mp_ctx* c) {
+    if (!c) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+void* mp_stream(mp_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int mp_set_timing(mp_ctx* c, int32_t enable) {
+    if (!c) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timing = enable != 0;
+    c->ev_used = 0;
+    return MP_OK;
+}
+
+int mp_get_timing(mp_ctx* c, double* push_ms, int64_t* push_launches) {
+    if (!c || !push_ms || !push_launches) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double total = 0.0;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev_pool[k], c->ev_pool[k + 1]));
+        total += ms;
+    }
+    *push_ms = total;
+    *push_launches = (int64_t)(c->ev_used / 2);
+    return MP_OK;
+}
+
+}  // extern "C"

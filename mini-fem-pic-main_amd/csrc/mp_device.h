@@ -1,0 +1,100 @@
+// Device-side data layout and the per-particle arithmetic shared by every kernel.
+//
+// HBM layout of a tet (192 B, 64 B aligned, one record per tet, gathered by cell index):
+//   c[i][0..3] = alpha_i, beta_i, gamma_i, delta_i of vertex i   (128 B)
+//   vol, nbr[4] (= Tetra::cell_con), con[4], ef[3]              ( 64 B)
+// Everything the walk, the push (start-cell E) and the scatter (node ids) need for one
+// tet is in one record, i.e. one or two 128 B lines per gather.
+//
+// Arithmetic follows the reference kernel with no contraction (built -ffp-contract=off):
+//   push   src/mini-pic.cpp:291-295   v = v + (qm*E)*dt ; x = x + v*dt
+//   walk   src/meshes.h:75-100        lc_i = ((1/6)*(((a_i - x b_i) + y g_i) - z d_i)) / V
+//                                     inside <=> no lc_i < 0 and no lc_i > 1
+//                                     move across the face of the first minimum lc
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpd {
+
+struct alignas(64) DevTet {
+    double c[4][4];  // per vertex: alpha, beta, gamma, delta
+    double vol;
+    int32_t nbr[4];
+    int32_t con[4];
+    double ef[3];
+};
+static_assert(sizeof(DevTet) == 192, "DevTet must stay 192 B");
+
+// Structure-of-arrays particle state in HBM.
+struct Soa {
+    double* x;
+    double* y;
+    double* z;
+    double* vx;
+    double* vy;
+    double* vz;
+    int32_t* cell;
+    int32_t* pid;
+};
+
+enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };
+
+__device__ __forceinline__ double bary(double a, double b, double g, double d, double x, double y,
+                                       double z, double vol) {
+    return ((1.0 / 6.0) * (((a - x * b) + y * g) - z * d)) / vol;
+}
+
+struct WalkStats {
+    int moves;
+    int refl;
+};
+
+// XtoLtet (src/meshes.h:75-100) from cell c.  On return c is the last tet visited and
+// l0..l3 its barycentric weights.  WALL == 1 reflects at boundary faces instead of
+// reporting the particle lost; a reflection counts as one move toward cap.
+template <int WALL>
+__device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c, double& x,
+                                    double& y, double& z, double& vx, double& vy, double& vz,
+                                    double& l0, double& l1, double& l2, double& l3, int cap,
+                                    WalkStats& st) {
+    while (true) {
+        const DevTet* t = tets + c;
+        const double vol = t->vol;
+        l0 = bary(t->c[0][0], t->c[0][1], t->c[0][2], t->c[0][3], x, y, z, vol);
+        l1 = bary(t->c[1][0], t->c[1][1], t->c[1][2], t->c[1][3], x, y, z, vol);
+        l2 = bary(t->c[2][0], t->c[2][1], t->c[2][2], t->c[2][3], x, y, z, vol);
+        l3 = bary(t->c[3][0], t->c[3][1], t->c[3][2], t->c[3][3], x, y, z, vol);
+        const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
+                         (l3 < 0 || l3 > 1.0);
+        if (!out) return kInside;
+        int mi = 0;
+        double ml = l0;
+        if (l1 < ml) { ml = l1; mi = 1; }
+        if (l2 < ml) { ml = l2; mi = 2; }
+        if (l3 < ml) { ml = l3; mi = 3; }
+        const int32_t nb = t->nbr[mi];
+        if (st.moves >= cap) return kCapped;
+        st.moves++;
+        if (nb >= 0) {
+            c = nb;
+            continue;
+        }
+        if (WALL == 0) return kLost;
+        // Specular reflection about the plane lc_mi = 0.  grad lc_mi is proportional to
+        // g = (-beta, gamma, -delta) / (6 V); x' = x - 2 lc g (6V)/|g|^2, v' = v - 2 (v.g) g/|g|^2.
+        const double gx = -t->c[mi][1], gy = t->c[mi][2], gz = -t->c[mi][3];
+        const double gg = (gx * gx + gy * gy) + gz * gz;
+        const double s = ((2.0 * ml) * (6.0 * vol)) / gg;
+        x = x - s * gx;
+        y = y - s * gy;
+        z = z - s * gz;
+        const double w = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
+        vx = vx - w * gx;
+        vy = vy - w * gy;
+        vz = vz - w * gz;
+        st.refl++;
+    }
+}
+
+}  // namespace mpd

@@ -1,0 +1,494 @@
+// HIP kernels of the particle mover for gfx950 (CDNA4, wave64).
+//
+//   k_push_soa     push + XtoLtet walk + scatter over the resident SoA particles
+//                  (src/mini-pic.cpp:287-308), atomic or deterministic-record scatter
+//   k_push_aos     the same over the reference's 88 B Particle records (drop-in path)
+//   k_normalise    den = raw * (spwt / node volume), raw cleared (src/mini-pic.cpp:319)
+//   k_seg_start / k_fold   deterministic scatter: after sorting (node,pid) records, a
+//                  serial fold per node in ascending pid, then the normalisation
+//   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell
+//   k_scan_*       exclusive prefix sums (counting sort offsets, stable compaction)
+//   k_flag_aos / k_compact_aos     stable erase of cell_index < 0 (src/mini-pic.cpp:316)
+#include "mp_device.h"
+#include "mp_launch.h"
+
+namespace mpd {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Adds this wave's per-lane tallies to the step counters (one atomic per counter per wave).
+__device__ __forceinline__ void tally(unsigned long long* ctr, bool act, bool lost, bool crossed,
+                                      int moves, bool capped, int refl) {
+    const unsigned long long a = __popcll(__ballot(act));
+    const unsigned long long l = __popcll(__ballot(lost));
+    const unsigned long long x = __popcll(__ballot(crossed));
+    const unsigned long long k = __popcll(__ballot(capped));
+    const int m = wave_sum(moves);
+    const int r = wave_sum(refl);
+    if (lane_id() == 0) {
+        if (a) atomicAdd(ctr + kCtrParticleSteps, a);
+        if (l) atomicAdd(ctr + kCtrLost, l);
+        if (x) atomicAdd(ctr + kCtrCrossed, x);
+        if (m) atomicAdd(ctr + kCtrMoves, (unsigned long long)m);
+        if (k) atomicAdd(ctr + kCtrCapHits, k);
+        if (r) atomicAdd(ctr + kCtrReflections, (unsigned long long)r);
+    }
+}
+
+template <int WALL, int DET>
+__global__ void __launch_bounds__(256)
+k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
+           double* __restrict__ den, uint64_t* __restrict__ keys, double* __restrict__ vals,
+           int pid_bits, unsigned long long* __restrict__ ctr, int cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false, lost = false, crossed = false, capped = false;
+    WalkStats st{0, 0};
+    if (i < n) {
+        int32_t c = s.cell[i];
+        if (c >= 0) {
+            act = true;
+            const DevTet* t = tets + c;
+            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+            double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+            vx = vx + (qm * ex) * dt;
+            vy = vy + (qm * ey) * dt;
+            vz = vz + (qm * ez) * dt;
+            double x = s.x[i] + vx * dt;
+            double y = s.y[i] + vy * dt;
+            double z = s.z[i] + vz * dt;
+            double l0, l1, l2, l3;
+            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            crossed = (st.moves - st.refl) > 0;
+            s.x[i] = x; s.y[i] = y; s.z[i] = z;
+            s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
+            if (r == kInside) {
+                s.cell[i] = c;
+                const DevTet* tf = tets + c;
+                const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
+                if (DET) {
+                    const uint64_t p = (uint64_t)(uint32_t)s.pid[i];
+                    keys[4 * i + 0] = ((uint64_t)n0 << pid_bits) | p;
+                    keys[4 * i + 1] = ((uint64_t)n1 << pid_bits) | p;
+                    keys[4 * i + 2] = ((uint64_t)n2 << pid_bits) | p;
+                    keys[4 * i + 3] = ((uint64_t)n3 << pid_bits) | p;
+                    vals[4 * i + 0] = l0;
+                    vals[4 * i + 1] = l1;
+                    vals[4 * i + 2] = l2;
+                    vals[4 * i + 3] = l3;
+                } else {
+                    unsafeAtomicAdd(den + n0, l0);
+                    unsafeAtomicAdd(den + n1, l1);
+                    unsafeAtomicAdd(den + n2, l2);
+                    unsafeAtomicAdd(den + n3, l3);
+                }
+            } else {
+                s.cell[i] = -1;
+                lost = true;
+                capped = (r == kCapped);
+                if (DET) {
+                    keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
+                    keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
+                }
+            }
+        } else if (DET) {
+            keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
+            keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
+        }
+    }
+    tally(ctr, act, lost, crossed, st.moves, capped, st.refl);
+}
+
+// Drop-in kernel over the reference's AoS Particle (88 B: pos@0 vel@24 lc@48 cell@80).
+// Absorbing walls, lc written back as the reference kernel does.
+template <int DET>
+__global__ void __launch_bounds__(256)
+k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int64_t n,
+           double qm, double dt, double* __restrict__ den, uint64_t* __restrict__ keys,
+           double* __restrict__ vals, int pid_bits, unsigned long long* __restrict__ ctr, int cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false, lost = false, crossed = false, capped = false;
+    WalkStats st{0, 0};
+    if (i < n) {
+        mp_particle* p = parts + i;
+        int32_t c = p->cell_index;
+        act = true;
+        const DevTet* t = tets + c;
+        double vx = p->vel[0] + (qm * t->ef[0]) * dt;
+        double vy = p->vel[1] + (qm * t->ef[1]) * dt;
+        double vz = p->vel[2] + (qm * t->ef[2]) * dt;
+        double x = p->pos[0] + vx * dt;
+        double y = p->pos[1] + vy * dt;
+        double z = p->pos[2] + vz * dt;
+        double l0, l1, l2, l3;
+        const int r = walk<0>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+        crossed = st.moves > 0;
+        p->pos[0] = x; p->pos[1] = y; p->pos[2] = z;
+        p->vel[0] = vx; p->vel[1] = vy; p->vel[2] = vz;
+        p->lc[0] = l0; p->lc[1] = l1; p->lc[2] = l2; p->lc[3] = l3;
+        if (r == kInside) {
+            p->cell_index = c;
+            const DevTet* tf = tets + c;
+            const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
+            if (DET) {
+                const uint64_t q = (uint64_t)i;
+                keys[4 * i + 0] = ((uint64_t)n0 << pid_bits) | q;
+                keys[4 * i + 1] = ((uint64_t)n1 << pid_bits) | q;
+                keys[4 * i + 2] = ((uint64_t)n2 << pid_bits) | q;
+                keys[4 * i + 3] = ((uint64_t)n3 << pid_bits) | q;
+                vals[4 * i + 0] = l0; vals[4 * i + 1] = l1;
+                vals[4 * i + 2] = l2; vals[4 * i + 3] = l3;
+            } else {
+                unsafeAtomicAdd(den + n0, l0);
+                unsafeAtomicAdd(den + n1, l1);
+                unsafeAtomicAdd(den + n2, l2);
+                unsafeAtomicAdd(den + n3, l3);
+            }
+        } else {
+            p->cell_index = -1;
+            lost = true;
+            capped = (r == kCapped);
+            if (DET) {
+                keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
+                keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
+            }
+        }
+    }
+    tally(ctr, act, lost, crossed, st.moves, capped, st.refl);
+}
+
+__global__ void k_normalise(double* __restrict__ raw, double* __restrict__ out,
+                            const double* __restrict__ nvol, int32_t nn, double spwt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    out[i] = raw[i] * (spwt / nvol[i]);
+    raw[i] = 0.0;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// First sorted record of each node.
+__global__ void k_seg_start(const uint64_t* __restrict__ keys, int64_t m, int pid_bits, int32_t nn,
+                            int32_t* __restrict__ start) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t node = keys[j] >> pid_bits;
+    if (node >= (uint64_t)nn) return;
+    if (j == 0 || (keys[j - 1] >> pid_bits) != node) start[node] = (int32_t)j;
+}
+
+// Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial
+// MoveParticles adds into den; then den *= spwt / node volume.
+__global__ void k_fold(const uint64_t* __restrict__ keys, const double* __restrict__ vals, int64_t m,
+                       int pid_bits, const int32_t* __restrict__ start, const double* __restrict__ nvol,
+                       int32_t nn, double spwt, double* __restrict__ out) {
+    const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double s = 0.0;
+    int64_t j = start[n];
+    if (j >= 0) {
+        for (; j < m && (keys[j] >> pid_bits) == (uint64_t)n; ++j) s = s + vals[j];
+    }
+    out[n] = s * (spwt / nvol[n]);
+}
+
+// ---- counting sort by cell -----------------------------------------------------------
+
+// Lanes holding equal consecutive cells form a run; the run head does one atomic.
+__device__ __forceinline__ void run_info(int32_t c, bool valid, int& head_lane, int& run_len) {
+    const int lane = lane_id();
+    const int32_t prev = __shfl_up(c, 1, kWave);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long hm = __ballot(head);
+    // head of my run: highest head lane <= lane
+    const unsigned long long below = hm & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+    head_lane = below ? 63 - __clzll(below) : lane;
+    const unsigned long long above = hm & ~((head_lane == 63) ? ~0ull : ((1ull << (head_lane + 1)) - 1));
+    const unsigned long long vm = __ballot(valid);
+    // run ends at the next head or at the first invalid lane after head_lane
+    const unsigned long long inval_above = ~vm & ~((head_lane == 63) ? ~0ull : ((1ull << (head_lane + 1)) - 1));
+    const unsigned long long stop = above | inval_above;
+    const int end = stop ? __ffsll((long long)stop) - 1 : 64;
+    run_len = end - head_lane;
+}
+
+__global__ void __launch_bounds__(256)
+k_cell_hist(const int32_t* __restrict__ cell, int64_t n, int32_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = (i < n) ? cell[i] : -1;
+    const bool valid = c >= 0;
+    int head_lane, run_len;
+    run_info(c, valid, head_lane, run_len);
+    if (valid && head_lane == lane_id()) atomicAdd(cnt + c, run_len);
+}
+
+__global__ void __launch_bounds__(256)
+k_cell_scatter(Soa src, int64_t n, int32_t* __restrict__ cursor, Soa dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = (i < n) ? src.cell[i] : -1;
+    const bool valid = c >= 0;
+    int head_lane, run_len;
+    run_info(c, valid, head_lane, run_len);
+    int32_t base = 0;
+    if (valid && head_lane == lane_id()) base = atomicAdd(cursor + c, run_len);
+    base = __shfl(base, head_lane, kWave);
+    if (!valid) return;
+    const int64_t d = (int64_t)base + (lane_id() - head_lane);
+    dst.x[d] = src.x[i]; dst.y[d] = src.y[i]; dst.z[d] = src.z[i];
+    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    dst.cell[d] = c;
+    dst.pid[d] = src.pid[i];
+}
+
+// ---- exclusive scan (int32), 3 phases --------------------------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* lds, int32_t& total) {
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    int32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int32_t t = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += t;
+    }
+    if (lane == kWave - 1) lds[w] = inc;
+    __syncthreads();
+    int32_t wbase = 0;
+    total = 0;
+    for (int k = 0; k < kScanThreads / kWave; k++) {
+        if (k < w) wbase += lds[k];
+        total += lds[k];
+    }
+    __syncthreads();
+    return wbase + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ part) {
+    __shared__ int32_t lds[kScanThreads / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) s += in[base + k];
+    int32_t total;
+    block_excl_scan(s, lds, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_down(const int32_t* __restrict__ in, int64_t n, const int32_t* __restrict__ part_off,
+            int32_t* __restrict__ out) {
+    __shared__ int32_t lds[kScanThreads / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int32_t total;
+    int32_t run = block_excl_scan(s, lds, total) + (part_off ? part_off[blockIdx.x] : 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// ---- stable compaction of the AoS drop-in array ---------------------------------------
+
+__global__ void k_flag_aos(const mp_particle* __restrict__ parts, int64_t n, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = parts[i].cell_index >= 0 ? 1 : 0;
+}
+
+__global__ void k_compact_aos(const mp_particle* __restrict__ in, int64_t n,
+                              const int32_t* __restrict__ off, mp_particle* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (in[i].cell_index >= 0) out[off[i]] = in[i];
+}
+
+// ---- field / particle staging --------------------------------------------------------
+
+__global__ void k_set_field(DevTet* __restrict__ tets, const double* __restrict__ ef, int32_t nt) {
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nt) return;
+    tets[e].ef[0] = ef[3 * (int64_t)e + 0];
+    tets[e].ef[1] = ef[3 * (int64_t)e + 1];
+    tets[e].ef[2] = ef[3 * (int64_t)e + 2];
+}
+
+__global__ void k_unpack(const double* __restrict__ pos, const double* __restrict__ vel,
+                         const int32_t* __restrict__ cell, int64_t n, int64_t pid0, Soa s, int64_t at) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t d = at + i;
+    s.x[d] = pos[3 * i + 0]; s.y[d] = pos[3 * i + 1]; s.z[d] = pos[3 * i + 2];
+    s.vx[d] = vel[3 * i + 0]; s.vy[d] = vel[3 * i + 1]; s.vz[d] = vel[3 * i + 2];
+    s.cell[d] = cell[i];
+    s.pid[d] = (int32_t)(pid0 + i);
+}
+
+__global__ void k_pack(Soa s, int64_t n, double* __restrict__ pos, double* __restrict__ vel) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos[3 * i + 0] = s.x[i]; pos[3 * i + 1] = s.y[i]; pos[3 * i + 2] = s.z[i];
+    vel[3 * i + 0] = s.vx[i]; vel[3 * i + 1] = s.vy[i]; vel[3 * i + 2] = s.vz[i];
+}
+
+// ---- launchers ------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
+    if (a.n <= 0) return hipSuccess;
+    const int bs = a.block;
+    const unsigned g = grid_for(a.n, bs);
+#define MP_PUSH(W, D)                                                                          \
+    hipLaunchKernelGGL((k_push_soa<W, D>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
+                       a.dt, a.den, a.keys, a.vals, a.pid_bits, a.ctr, a.cap)
+    if (a.wall == 0 && !a.det) MP_PUSH(0, 0);
+    else if (a.wall == 0 && a.det) MP_PUSH(0, 1);
+    else if (a.wall == 1 && !a.det) MP_PUSH(1, 0);
+    else MP_PUSH(1, 1);
+#undef MP_PUSH
+    return hipGetLastError();
+}
+
+hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
+                           double* den, uint64_t* keys, double* vals, int pid_bits,
+                           unsigned long long* ctr, int cap, bool det, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned g = grid_for(n, 256);
+    if (det)
+        hipLaunchKernelGGL((k_push_aos<1>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
+                           keys, vals, pid_bits, ctr, cap);
+    else
+        hipLaunchKernelGGL((k_push_aos<0>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
+                           keys, vals, pid_bits, ctr, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_t nn, double spwt,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_normalise, dim3(grid_for(nn, 256)), dim3(256), 0, st, raw, out, nvol, nn, spwt);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_det_fold(const uint64_t* keys, const double* vals, int64_t m, int pid_bits,
+                           int32_t* start, const double* nvol, int32_t nn, double spwt, double* out,
+                           hipStream_t st) {
+    hipError_t e = launch_fill_i32(start, nn, -1, st);
+    if (e != hipSuccess) return e;
+    if (m > 0)
+        hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, pid_bits, nn,
+                           start);
+    hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, keys, vals, m, pid_bits, start,
+                       nvol, nn, spwt, out);
+    return hipGetLastError();
+}
+
+int64_t scan_scratch_ints(int64_t n) {
+    int64_t total = 0;
+    while (n > kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        total += 2 * n;
+    }
+    return total + 16;
+}
+
+hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (n <= kScanTile) {
+        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanThreads), 0, st, in, n, (const int32_t*)nullptr, out);
+        return hipGetLastError();
+    }
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    int32_t* part = scratch;
+    int32_t* part_off = scratch + nb;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, part);
+    hipError_t e = launch_excl_scan(part, nb, part_off, scratch + 2 * nb, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
+                       (const int32_t*)part_off, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int32_t* cnt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_scatter(const Soa& src, int64_t n, int32_t* cursor, const Soa& dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, cursor, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flag_aos, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* off, mp_particle* out,
+                              hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_aos, dim3(grid_for(n, 256)), dim3(256), 0, st, in, n, off, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_field(DevTet* tets, const double* ef, int32_t nt, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_field, dim3(grid_for(nt, 256)), dim3(256), 0, st, tets, ef, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const double* pos, const double* vel, const int32_t* cell, int64_t n, int64_t pid0,
+                         const Soa& s, int64_t at, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(n, 256)), dim3(256), 0, st, pos, vel, cell, n, pid0, s, at);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const Soa& s, int64_t n, double* pos, double* vel, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, st, s, n, pos, vel);
+    return hipGetLastError();
+}
+
+}  // namespace mpd

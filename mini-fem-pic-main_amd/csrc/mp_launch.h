@@ -1,0 +1,66 @@
+// Host-side launch wrappers of the mover kernels (implemented in mp_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mp_abi.h"
+#include "mp_device.h"
+
+namespace mpd {
+
+enum CounterIndex : int {
+    kCtrParticleSteps = 0,
+    kCtrLost = 1,
+    kCtrCrossed = 2,
+    kCtrMoves = 3,
+    kCtrCapHits = 4,
+    kCtrReflections = 5,
+    kNumCounters = 8,
+};
+
+struct PushArgs {
+    const DevTet* tets;
+    Soa soa;
+    int64_t n;
+    double qm;
+    double dt;
+    double* den;
+    uint64_t* keys;
+    double* vals;
+    int pid_bits;
+    unsigned long long* ctr;
+    int cap;
+    int wall;
+    bool det;
+    int block;
+};
+
+hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
+hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
+                           double* den, uint64_t* keys, double* vals, int pid_bits,
+                           unsigned long long* ctr, int cap, bool det, hipStream_t st);
+hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_t nn, double spwt,
+                            hipStream_t st);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
+hipError_t launch_det_fold(const uint64_t* keys, const double* vals, int64_t m, int pid_bits,
+                           int32_t* start, const double* nvol, int32_t nn, double spwt, double* out,
+                           hipStream_t st);
+int64_t scan_scratch_ints(int64_t n);
+hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
+hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int32_t* cnt, hipStream_t st);
+hipError_t launch_cell_scatter(const Soa& src, int64_t n, int32_t* cursor, const Soa& dst, hipStream_t st);
+hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st);
+hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* off, mp_particle* out,
+                              hipStream_t st);
+hipError_t launch_set_field(DevTet* tets, const double* ef, int32_t nt, hipStream_t st);
+hipError_t launch_unpack(const double* pos, const double* vel, const int32_t* cell, int64_t n, int64_t pid0,
+                         const Soa& s, int64_t at, hipStream_t st);
+hipError_t launch_pack(const Soa& s, int64_t n, double* pos, double* vel, hipStream_t st);
+
+// Radix sort of (key, value) pairs on bits [0, end_bit) (mp_sort.hip).
+size_t radix_sort_temp_bytes(int64_t n, int end_bit);
+hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                            const double* vals_in, double* vals_out, int64_t n, int end_bit, hipStream_t st);
+
+}  // namespace mpd
