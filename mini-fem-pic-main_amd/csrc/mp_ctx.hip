@@ -56,6 +56,9 @@ struct mp_ctx {
     int32_t* offs = nullptr;
     int64_t aos_cap = 0;
     int32_t* pinned_i32 = nullptr;
+    // per-block tallies of the push kernels
+    int32_t* tally = nullptr;
+    int64_t tally_n = 0;
     // push-kernel timing (events recorded around each push launch while enabled)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -189,6 +192,18 @@ int reserve_scan(mp_ctx* c, int64_t n) {
     c->scan_tmp_n = 0;
     if (dalloc(&c->scan_tmp, need) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "scan scratch allocation failed");
     c->scan_tmp_n = need;
+    return MP_OK;
+}
+
+int reserve_tally(mp_ctx* c, int64_t n, int block) {
+    const int64_t need = mpd::tally_ints(std::max<int64_t>(n, 1), block);
+    if (need <= c->tally_n) return MP_OK;
+    dfree(c->tally);
+    c->tally = nullptr;
+    c->tally_n = 0;
+    const int64_t nc = need + need / 2;
+    if (dalloc(&c->tally, nc) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "tally buffer allocation failed");
+    c->tally_n = nc;
     return MP_OK;
 }
 
@@ -334,7 +349,7 @@ void mp_destroy(mp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
-                    c->sort_tmp, c->seg_start, c->aos, c->aos_out, c->flags, c->offs};
+                    c->sort_tmp, c->seg_start, c->aos, c->aos_out, c->flags, c->offs, c->tally};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -409,8 +424,11 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
     int rc = reserve_scan(c, n + 1);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(c->aos, parts, sizeof(mp_particle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    rc = reserve_tally(c, n, 256);
+    if (rc) return rc;
     HIP_TRY(c, mpd::launch_push_aos(c->tets, c->aos, n, q_over_m, dt, c->den_raw, det ? c->keys[0] : nullptr,
-                                    det ? c->vals[0] : nullptr, pid_bits, c->ctr, c->opt.walk_cap, det, c->stream));
+                                    det ? c->vals[0] : nullptr, pid_bits, c->tally, c->ctr, c->opt.walk_cap,
+                                    det, c->stream));
     if (det) {
         rc = det_finish(c, 4 * n, pid_bits, spwt);
         if (rc) return rc;
@@ -489,6 +507,10 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         int rc = reserve_records(c, 4 * c->n_slots, pid_bits + bit_width((uint64_t)c->n_nodes));
         if (rc) return rc;
     }
+    {
+        int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
+        if (rc) return rc;
+    }
     mpd::PushArgs a;
     a.tets = c->tets;
     a.soa = c->soa[c->cur];
@@ -499,6 +521,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.keys = det ? c->keys[0] : nullptr;
     a.vals = det ? c->vals[0] : nullptr;
     a.pid_bits = pid_bits;
+    a.part = c->tally;
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;
     a.wall = c->opt.wall_mode;

@@ -24,22 +24,91 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
-// Adds this wave's per-lane tallies to the step counters (one atomic per counter per wave).
-__device__ __forceinline__ void tally(unsigned long long* ctr, bool act, bool lost, bool crossed,
+// Per-block step tallies: waves reduce with ballots/shuffles, the block sums its waves in
+// LDS and thread 0 stores kTally int32 partials to part[blockIdx.x * kTally + k] with plain
+// stores.  k_tally_finish folds the partials into the 64-bit counters.  (One atomic per
+// wave on a shared counter line serialises in a single L2 channel: ~10 ms at 20M particles.)
+constexpr int kTally = 6;
+
+__device__ __forceinline__ void tally(int32_t* __restrict__ part, bool act, bool lost, bool crossed,
                                       int moves, bool capped, int refl) {
-    const unsigned long long a = __popcll(__ballot(act));
-    const unsigned long long l = __popcll(__ballot(lost));
-    const unsigned long long x = __popcll(__ballot(crossed));
-    const unsigned long long k = __popcll(__ballot(capped));
-    const int m = wave_sum(moves);
-    const int r = wave_sum(refl);
-    if (lane_id() == 0) {
-        if (a) atomicAdd(ctr + kCtrParticleSteps, a);
-        if (l) atomicAdd(ctr + kCtrLost, l);
-        if (x) atomicAdd(ctr + kCtrCrossed, x);
-        if (m) atomicAdd(ctr + kCtrMoves, (unsigned long long)m);
-        if (k) atomicAdd(ctr + kCtrCapHits, k);
-        if (r) atomicAdd(ctr + kCtrReflections, (unsigned long long)r);
+    __shared__ int32_t sh[1024 / kWave][kTally];
+    int v[kTally];
+    v[0] = __popcll(__ballot(act));
+    v[1] = __popcll(__ballot(lost));
+    v[2] = __popcll(__ballot(crossed));
+    v[3] = wave_sum(moves);
+    v[4] = __popcll(__ballot(capped));
+    v[5] = wave_sum(refl);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < kTally; k++) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < kTally) {
+        int s = 0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); q++) s += sh[q][threadIdx.x];
+        part[(int64_t)blockIdx.x * kTally + threadIdx.x] = s;
+    }
+}
+
+// Sums per-block partials [nb][kTally] into the counters (one block, 6 atomics).
+__global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict__ part, int64_t nb,
+                                                       unsigned long long* __restrict__ ctr) {
+    __shared__ unsigned long long sh[1024 / kWave][kTally];
+    unsigned long long s[kTally] = {0, 0, 0, 0, 0, 0};
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < kTally; k++) s[k] += (unsigned long long)(uint32_t)part[b * kTally + k];
+#pragma unroll
+    for (int k = 0; k < kTally; k++)
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, kWave);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int k = 0; k < kTally; k++) sh[w][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x < kTally) {
+        unsigned long long t = 0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); q++) t += sh[q][threadIdx.x];
+        if (t) atomicAdd(ctr + threadIdx.x, t);
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Atomic scatter with the wave's contributions grouped by final tet: one leader per distinct
+// tet adds the group's summed weights to its 4 nodes.  Cell-sorted particles put ~15 per
+// tet, so this turns 256 same-line atomics per wave into a few groups of 4.  Must be called
+// by every lane of the wave (the loop is wave-uniform).
+__device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                                bool ok, int32_t fc, double l0, double l1, double l2,
+                                                double l3) {
+    unsigned long long pending = __ballot(ok);
+    const int lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int32_t gc = __shfl(fc, leader, kWave);
+        const bool mine = ok && fc == gc;
+        const unsigned long long m = __ballot(mine);
+        pending &= ~m;
+        double a0 = l0, a1 = l1, a2 = l2, a3 = l3;
+        if (__popcll(m) > 1) {
+            a0 = wave_sum_f64(mine ? l0 : 0.0);
+            a1 = wave_sum_f64(mine ? l1 : 0.0);
+            a2 = wave_sum_f64(mine ? l2 : 0.0);
+            a3 = wave_sum_f64(mine ? l3 : 0.0);
+        }
+        if (lane == leader) {
+            const DevTet* t = tets + gc;
+            unsafeAtomicAdd(den + t->con[0], a0);
+            unsafeAtomicAdd(den + t->con[1], a1);
+            unsafeAtomicAdd(den + t->con[2], a2);
+            unsafeAtomicAdd(den + t->con[3], a3);
+        }
     }
 }
 
@@ -47,9 +116,11 @@ template <int WALL, int DET>
 __global__ void __launch_bounds__(256)
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint64_t* __restrict__ keys, double* __restrict__ vals,
-           int pid_bits, unsigned long long* __restrict__ ctr, int cap) {
+           int pid_bits, int32_t* __restrict__ part, int cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = false, lost = false, crossed = false, capped = false;
+    bool act = false, lost = false, crossed = false, capped = false, ok = false;
+    int32_t fc = -1;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     WalkStats st{0, 0};
     if (i < n) {
         int32_t c = s.cell[i];
@@ -64,16 +135,17 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             double x = s.x[i] + vx * dt;
             double y = s.y[i] + vy * dt;
             double z = s.z[i] + vz * dt;
-            double l0, l1, l2, l3;
             const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             crossed = (st.moves - st.refl) > 0;
             s.x[i] = x; s.y[i] = y; s.z[i] = z;
             s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
             if (r == kInside) {
                 s.cell[i] = c;
-                const DevTet* tf = tets + c;
-                const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
+                ok = true;
+                fc = c;
                 if (DET) {
+                    const DevTet* tf = tets + c;
+                    const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
                     const uint64_t p = (uint64_t)(uint32_t)s.pid[i];
                     keys[4 * i + 0] = ((uint64_t)n0 << pid_bits) | p;
                     keys[4 * i + 1] = ((uint64_t)n1 << pid_bits) | p;
@@ -83,11 +155,6 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                     vals[4 * i + 1] = l1;
                     vals[4 * i + 2] = l2;
                     vals[4 * i + 3] = l3;
-                } else {
-                    unsafeAtomicAdd(den + n0, l0);
-                    unsafeAtomicAdd(den + n1, l1);
-                    unsafeAtomicAdd(den + n2, l2);
-                    unsafeAtomicAdd(den + n3, l3);
                 }
             } else {
                 s.cell[i] = -1;
@@ -103,7 +170,8 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
         }
     }
-    tally(ctr, act, lost, crossed, st.moves, capped, st.refl);
+    if (!DET) scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
+    tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
 // Drop-in kernel over the reference's AoS Particle (88 B: pos@0 vel@24 lc@48 cell@80).
@@ -112,7 +180,7 @@ template <int DET>
 __global__ void __launch_bounds__(256)
 k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int64_t n,
            double qm, double dt, double* __restrict__ den, uint64_t* __restrict__ keys,
-           double* __restrict__ vals, int pid_bits, unsigned long long* __restrict__ ctr, int cap) {
+           double* __restrict__ vals, int pid_bits, int32_t* __restrict__ part, int cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool act = false, lost = false, crossed = false, capped = false;
     WalkStats st{0, 0};
@@ -161,7 +229,7 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
             }
         }
     }
-    tally(ctr, act, lost, crossed, st.moves, capped, st.refl);
+    tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
 __global__ void k_normalise(double* __restrict__ raw, double* __restrict__ out,
@@ -194,17 +262,28 @@ __global__ void k_seg_start(const uint64_t* __restrict__ keys, int64_t m, int pi
 
 // Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial
 // MoveParticles adds into den; then den *= spwt / node volume.
-__global__ void k_fold(const uint64_t* __restrict__ keys, const double* __restrict__ vals, int64_t m,
-                       int pid_bits, const int32_t* __restrict__ start, const double* __restrict__ nvol,
-                       int32_t nn, double spwt, double* __restrict__ out) {
-    const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nn) return;
+// One wave per node: the wave loads 64 consecutive sorted records at a time (coalesced);
+// the node's records are a prefix of each chunk, folded in order by broadcasting lane q.
+__global__ void __launch_bounds__(256)
+k_fold(const uint64_t* __restrict__ keys, const double* __restrict__ vals, int64_t m, int pid_bits,
+       const int32_t* __restrict__ start, const double* __restrict__ nvol, int32_t nn, double spwt,
+       double* __restrict__ out) {
+    const int64_t n = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (n >= nn) return;  // wave-uniform
+    const int lane = lane_id();
     double s = 0.0;
-    int64_t j = start[n];
-    if (j >= 0) {
-        for (; j < m && (keys[j] >> pid_bits) == (uint64_t)n; ++j) s = s + vals[j];
+    const int64_t j0 = start[n];
+    if (j0 >= 0) {
+        for (int64_t base = j0;; base += kWave) {
+            const int64_t j = base + lane;
+            const bool mine = j < m && (keys[j] >> pid_bits) == (uint64_t)n;
+            const double v = mine ? vals[j] : 0.0;
+            const int cnt = __popcll(__ballot(mine));
+            for (int q = 0; q < cnt; q++) s = s + __shfl(v, q, kWave);
+            if (cnt < kWave) break;
+        }
     }
-    out[n] = s * (spwt / nvol[n]);
+    if (lane == 0) out[n] = s * (spwt / nvol[n]);
 }
 
 // ---- counting sort by cell -----------------------------------------------------------
@@ -368,27 +447,38 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     const unsigned g = grid_for(a.n, bs);
 #define MP_PUSH(W, D)                                                                          \
     hipLaunchKernelGGL((k_push_soa<W, D>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
-                       a.dt, a.den, a.keys, a.vals, a.pid_bits, a.ctr, a.cap)
+                       a.dt, a.den, a.keys, a.vals, a.pid_bits, a.part, a.cap)
     if (a.wall == 0 && !a.det) MP_PUSH(0, 0);
     else if (a.wall == 0 && a.det) MP_PUSH(0, 1);
     else if (a.wall == 1 && !a.det) MP_PUSH(1, 0);
     else MP_PUSH(1, 1);
 #undef MP_PUSH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
+hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st) {
+    hipLaunchKernelGGL(k_tally_finish, dim3(1), dim3(1024), 0, st, part, nb, ctr);
     return hipGetLastError();
 }
 
+int64_t tally_ints(int64_t n, int block) { return ((n + block - 1) / block) * kTally; }
+
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
-                           double* den, uint64_t* keys, double* vals, int pid_bits,
+                           double* den, uint64_t* keys, double* vals, int pid_bits, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const unsigned g = grid_for(n, 256);
     if (det)
         hipLaunchKernelGGL((k_push_aos<1>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
-                           keys, vals, pid_bits, ctr, cap);
+                           keys, vals, pid_bits, part, cap);
     else
         hipLaunchKernelGGL((k_push_aos<0>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
-                           keys, vals, pid_bits, ctr, cap);
-    return hipGetLastError();
+                           keys, vals, pid_bits, part, cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(part, g, ctr, st);
 }
 
 hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_t nn, double spwt,
@@ -417,7 +507,7 @@ hipError_t launch_det_fold(const uint64_t* keys, const double* vals, int64_t m, 
     if (m > 0)
         hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, pid_bits, nn,
                            start);
-    hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, keys, vals, m, pid_bits, start,
+    hipLaunchKernelGGL(k_fold, dim3(grid_for((int64_t)nn * kWave, 256)), dim3(256), 0, st, keys, vals, m, pid_bits, start,
                        nvol, nn, spwt, out);
     return hipGetLastError();
 }

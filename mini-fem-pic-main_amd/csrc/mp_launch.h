@@ -28,6 +28,7 @@ struct PushArgs {
     uint64_t* keys;
     double* vals;
     int pid_bits;
+    int32_t* part;  // per-block tallies, tally_ints(n, block) ints
     unsigned long long* ctr;
     int cap;
     int wall;
@@ -37,8 +38,11 @@ struct PushArgs {
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
-                           double* den, uint64_t* keys, double* vals, int pid_bits,
+                           double* den, uint64_t* keys, double* vals, int pid_bits, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st);
+// Step counters: the push kernels leave per-block partials; this folds them into ctr.
+hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st);
+int64_t tally_ints(int64_t n, int block);
 hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_t nn, double spwt,
                             hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
