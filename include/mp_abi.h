@@ -69,6 +69,8 @@ typedef struct mp_options {
     int32_t threads_per_particle;  /* 1 or 4 (4: one lane per barycentric coordinate)  */
     int32_t particles_per_thread;  /* 1, 2 or 4 (TPP=1 only)                           */
     int32_t walk_cap;              /* max face moves per particle per step             */
+    int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
+                                      2 per workgroup by node (LDS hash)               */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */

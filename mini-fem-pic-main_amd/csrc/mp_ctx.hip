@@ -43,7 +43,7 @@ struct mp_ctx {
     int32_t* scan_tmp = nullptr;
     int64_t scan_tmp_n = 0;
     // deterministic-scatter scratch
-    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* keys[2] = {nullptr, nullptr};
     double* vals[2] = {nullptr, nullptr};
     int64_t rec_cap = 0;
     void* sort_tmp = nullptr;
@@ -123,6 +123,7 @@ mp_options default_options() {
     o.threads_per_particle = 1;
     o.particles_per_thread = 1;
     o.walk_cap = 4096;
+    o.scatter_agg = 2;
     return o;
 }
 
@@ -236,13 +237,14 @@ int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
     return MP_OK;
 }
 
-// Deterministic scatter tail: sort the m records on their used bits, fold per node.
-int det_finish(mp_ctx* c, int64_t m, int pid_bits, double spwt) {
-    const int end_bit = pid_bits + bit_width((uint64_t)c->n_nodes);
+// Deterministic scatter tail: stable sort of the m pid-ordered records on the node bits,
+// then the serial fold per node.
+int det_finish(mp_ctx* c, int64_t m, double spwt) {
+    const int end_bit = bit_width((uint64_t)c->n_nodes);
     HIP_TRY(c, mpd::radix_sort_pairs(c->sort_tmp, c->sort_tmp_bytes, c->keys[0], c->keys[1], c->vals[0],
                                      c->vals[1], m, end_bit, c->stream));
-    HIP_TRY(c, mpd::launch_det_fold(c->keys[1], c->vals[1], m, pid_bits, c->seg_start, c->node_vol,
-                                    c->n_nodes, spwt, c->den_out, c->stream));
+    HIP_TRY(c, mpd::launch_det_fold(c->keys[1], c->vals[1], m, c->seg_start, c->node_vol, c->n_nodes, spwt,
+                                    c->den_out, c->stream));
     return MP_OK;
 }
 
@@ -269,7 +271,8 @@ int sort_now(mp_ctx* c) {
 bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
-           o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1;
+           o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1 &&
+           o.scatter_agg >= 0 && o.scatter_agg <= 2;
 }
 
 }  // namespace
@@ -416,10 +419,10 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
         c->aos_cap = nc;
     }
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    const int pid_bits = std::max(1, bit_width((uint64_t)std::max<int64_t>(n - 1, 1)));
     if (det) {
-        int rc = reserve_records(c, 4 * n, pid_bits + bit_width((uint64_t)c->n_nodes));
+        int rc = reserve_records(c, 4 * n, bit_width((uint64_t)c->n_nodes));
         if (rc) return rc;
+        HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], 4 * n, 0xFFFFFFFFu, c->stream));
     }
     int rc = reserve_scan(c, n + 1);
     if (rc) return rc;
@@ -427,10 +430,10 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
     rc = reserve_tally(c, n, 256);
     if (rc) return rc;
     HIP_TRY(c, mpd::launch_push_aos(c->tets, c->aos, n, q_over_m, dt, c->den_raw, det ? c->keys[0] : nullptr,
-                                    det ? c->vals[0] : nullptr, pid_bits, c->tally, c->ctr, c->opt.walk_cap,
+                                    det ? c->vals[0] : nullptr, c->tally, c->ctr, c->opt.walk_cap,
                                     det, c->stream));
     if (det) {
-        rc = det_finish(c, 4 * n, pid_bits, spwt);
+        rc = det_finish(c, 4 * n, spwt);
         if (rc) return rc;
     } else {
         HIP_TRY(c, mpd::launch_normalise(c->den_raw, c->den_out, c->node_vol, c->n_nodes, spwt, c->stream));
@@ -502,10 +505,11 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         if (rc) return rc;
     }
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    const int pid_bits = std::max(1, bit_width((uint64_t)std::max<int64_t>(c->next_pid - 1, 1)));
+    const int64_t n_rec = 4 * c->next_pid;  // record slots 4*pid+v for every pid ever issued
     if (det) {
-        int rc = reserve_records(c, 4 * c->n_slots, pid_bits + bit_width((uint64_t)c->n_nodes));
+        int rc = reserve_records(c, n_rec, bit_width((uint64_t)c->n_nodes));
         if (rc) return rc;
+        HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, c->stream));
     }
     {
         int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
@@ -520,18 +524,18 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.den = c->den_raw;
     a.keys = det ? c->keys[0] : nullptr;
     a.vals = det ? c->vals[0] : nullptr;
-    a.pid_bits = pid_bits;
     a.part = c->tally;
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;
     a.wall = c->opt.wall_mode;
     a.det = det;
+    a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));
     if (det) {
-        int rc = det_finish(c, 4 * c->n_slots, pid_bits, spwt);
+        int rc = det_finish(c, n_rec, spwt);
         if (rc) return rc;
     } else {
         HIP_TRY(c, mpd::launch_normalise(c->den_raw, c->den_out, c->node_vol, c->n_nodes, spwt, c->stream));

@@ -112,11 +112,74 @@ __device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets,
     }
 }
 
-template <int WALL, int DET>
+// Atomic scatter aggregated per workgroup by node: the block's (node, weight) pairs are summed
+// in an LDS hash table (open addressing, linear probing, ds_cmpst + ds_add_f64), then each
+// occupied slot issues one global atomic.  A block of 256 cell-sorted particles touches a few
+// dozen nodes, so ~1024 contributions become ~50 global atomics, however the particles
+// reordered since the last sort.  Every thread of the block must call it.
+constexpr int kHashSlots = 2048;  // >= 4 * max block size, load factor <= 1/2
+
+__device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node, double w) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 11);
+    for (;;) {
+        const int32_t prev = atomicCAS(&hk[h], -1, node);
+        if (prev == -1 || prev == node) {
+            atomicAdd(&hv[h], w);
+            return;
+        }
+        h = (h + 1) & (kHashSlots - 1);
+    }
+}
+
+__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                            bool ok, int32_t fc, double l0, double l1, double l2,
+                                            double l3) {
+    __shared__ int32_t hk[kHashSlots];
+    __shared__ double hv[kHashSlots];
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        hk[s] = -1;
+        hv[s] = 0.0;
+    }
+    __syncthreads();
+    if (ok) {
+        const DevTet* t = tets + fc;
+        const int32_t n0 = t->con[0], n1 = t->con[1], n2 = t->con[2], n3 = t->con[3];
+        lds_insert(hk, hv, n0, l0);
+        lds_insert(hk, hv, n1, l1);
+        lds_insert(hk, hv, n2, l2);
+        lds_insert(hk, hv, n3, l3);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        const int32_t node = hk[s];
+        if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
+    }
+}
+
+// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block.
+// Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the
+// record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
+// (pre-filled).  A stable sort on the node bits then leaves each node's weights in
+// ascending pid: the order of the serial MoveParticles fold.
+__device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
+                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
+                                            double l1, double l2, double l3) {
+    const int64_t r = 4 * pid;
+    keys[r + 0] = (uint32_t)tf->con[0];
+    keys[r + 1] = (uint32_t)tf->con[1];
+    keys[r + 2] = (uint32_t)tf->con[2];
+    keys[r + 3] = (uint32_t)tf->con[3];
+    vals[r + 0] = l0;
+    vals[r + 1] = l1;
+    vals[r + 2] = l2;
+    vals[r + 3] = l3;
+}
+
+template <int WALL, int DET, int AGG>
 __global__ void __launch_bounds__(256)
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
-           double* __restrict__ den, uint64_t* __restrict__ keys, double* __restrict__ vals,
-           int pid_bits, int32_t* __restrict__ part, int cap) {
+           double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
+           int32_t* __restrict__ part, int cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool act = false, lost = false, crossed = false, capped = false, ok = false;
     int32_t fc = -1;
@@ -143,44 +206,37 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                 s.cell[i] = c;
                 ok = true;
                 fc = c;
-                if (DET) {
-                    const DevTet* tf = tets + c;
-                    const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
-                    const uint64_t p = (uint64_t)(uint32_t)s.pid[i];
-                    keys[4 * i + 0] = ((uint64_t)n0 << pid_bits) | p;
-                    keys[4 * i + 1] = ((uint64_t)n1 << pid_bits) | p;
-                    keys[4 * i + 2] = ((uint64_t)n2 << pid_bits) | p;
-                    keys[4 * i + 3] = ((uint64_t)n3 << pid_bits) | p;
-                    vals[4 * i + 0] = l0;
-                    vals[4 * i + 1] = l1;
-                    vals[4 * i + 2] = l2;
-                    vals[4 * i + 3] = l3;
-                }
+                if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3);
             } else {
                 s.cell[i] = -1;
                 lost = true;
                 capped = (r == kCapped);
-                if (DET) {
-                    keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
-                    keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
-                }
             }
-        } else if (DET) {
-            keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
-            keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
         }
     }
-    if (!DET) scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
+    if (!DET) {
+        if (AGG == 2) {
+            scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
+        } else if (AGG == 1) {
+            scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
+        } else if (ok) {
+            const DevTet* t = tets + fc;
+            unsafeAtomicAdd(den + t->con[0], l0);
+            unsafeAtomicAdd(den + t->con[1], l1);
+            unsafeAtomicAdd(den + t->con[2], l2);
+            unsafeAtomicAdd(den + t->con[3], l3);
+        }
+    }
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
 // Drop-in kernel over the reference's AoS Particle (88 B: pos@0 vel@24 lc@48 cell@80).
-// Absorbing walls, lc written back as the reference kernel does.
+// Absorbing walls, lc written back as the reference kernel does; pid = array index.
 template <int DET>
 __global__ void __launch_bounds__(256)
 k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int64_t n,
-           double qm, double dt, double* __restrict__ den, uint64_t* __restrict__ keys,
-           double* __restrict__ vals, int pid_bits, int32_t* __restrict__ part, int cap) {
+           double qm, double dt, double* __restrict__ den, uint32_t* __restrict__ keys,
+           double* __restrict__ vals, int32_t* __restrict__ part, int cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool act = false, lost = false, crossed = false, capped = false;
     WalkStats st{0, 0};
@@ -204,29 +260,18 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
         if (r == kInside) {
             p->cell_index = c;
             const DevTet* tf = tets + c;
-            const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
             if (DET) {
-                const uint64_t q = (uint64_t)i;
-                keys[4 * i + 0] = ((uint64_t)n0 << pid_bits) | q;
-                keys[4 * i + 1] = ((uint64_t)n1 << pid_bits) | q;
-                keys[4 * i + 2] = ((uint64_t)n2 << pid_bits) | q;
-                keys[4 * i + 3] = ((uint64_t)n3 << pid_bits) | q;
-                vals[4 * i + 0] = l0; vals[4 * i + 1] = l1;
-                vals[4 * i + 2] = l2; vals[4 * i + 3] = l3;
+                put_records(keys, vals, i, tf, l0, l1, l2, l3);
             } else {
-                unsafeAtomicAdd(den + n0, l0);
-                unsafeAtomicAdd(den + n1, l1);
-                unsafeAtomicAdd(den + n2, l2);
-                unsafeAtomicAdd(den + n3, l3);
+                unsafeAtomicAdd(den + tf->con[0], l0);
+                unsafeAtomicAdd(den + tf->con[1], l1);
+                unsafeAtomicAdd(den + tf->con[2], l2);
+                unsafeAtomicAdd(den + tf->con[3], l3);
             }
         } else {
             p->cell_index = -1;
             lost = true;
             capped = (r == kCapped);
-            if (DET) {
-                keys[4 * i + 0] = ~0ull; keys[4 * i + 1] = ~0ull;
-                keys[4 * i + 2] = ~0ull; keys[4 * i + 3] = ~0ull;
-            }
         }
     }
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
@@ -250,22 +295,28 @@ __global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
     if (i < n) p[i] = v;
 }
 
-// First sorted record of each node.
-__global__ void k_seg_start(const uint64_t* __restrict__ keys, int64_t m, int pid_bits, int32_t nn,
+// First sorted record of each node (keys are node ids; sentinels sort last).
+__global__ void k_seg_start(const uint32_t* __restrict__ keys, int64_t m, int32_t nn,
                             int32_t* __restrict__ start) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
-    const uint64_t node = keys[j] >> pid_bits;
-    if (node >= (uint64_t)nn) return;
-    if (j == 0 || (keys[j - 1] >> pid_bits) != node) start[node] = (int32_t)j;
+    const uint32_t node = keys[j];
+    if (node >= (uint32_t)nn) return;
+    if (j == 0 || keys[j - 1] != node) start[node] = (int32_t)j;
 }
 
-// Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial
-// MoveParticles adds into den; then den *= spwt / node volume.
-// One wave per node: the wave loads 64 consecutive sorted records at a time (coalesced);
-// the node's records are a prefix of each chunk, folded in order by broadcasting lane q.
+__device__ __forceinline__ double readlane_f64(double v, int q) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
+    return __hiloint2double(hi, lo);
+}
+
+// Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial MoveParticles
+// adds into den; then den *= spwt / node volume.  One wave per node: the wave loads 64
+// consecutive sorted records at a time (coalesced); the node's records are a prefix of each
+// chunk and are folded in order through lane broadcasts.
 __global__ void __launch_bounds__(256)
-k_fold(const uint64_t* __restrict__ keys, const double* __restrict__ vals, int64_t m, int pid_bits,
+k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64_t m,
        const int32_t* __restrict__ start, const double* __restrict__ nvol, int32_t nn, double spwt,
        double* __restrict__ out) {
     const int64_t n = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -276,10 +327,10 @@ k_fold(const uint64_t* __restrict__ keys, const double* __restrict__ vals, int64
     if (j0 >= 0) {
         for (int64_t base = j0;; base += kWave) {
             const int64_t j = base + lane;
-            const bool mine = j < m && (keys[j] >> pid_bits) == (uint64_t)n;
+            const bool mine = j < m && keys[j] == (uint32_t)n;
             const double v = mine ? vals[j] : 0.0;
             const int cnt = __popcll(__ballot(mine));
-            for (int q = 0; q < cnt; q++) s = s + __shfl(v, q, kWave);
+            for (int q = 0; q < cnt; q++) s = s + readlane_f64(v, q);
             if (cnt < kWave) break;
         }
     }
@@ -445,13 +496,19 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
     const int bs = a.block;
     const unsigned g = grid_for(a.n, bs);
-#define MP_PUSH(W, D)                                                                          \
-    hipLaunchKernelGGL((k_push_soa<W, D>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
-                       a.dt, a.den, a.keys, a.vals, a.pid_bits, a.part, a.cap)
-    if (a.wall == 0 && !a.det) MP_PUSH(0, 0);
-    else if (a.wall == 0 && a.det) MP_PUSH(0, 1);
-    else if (a.wall == 1 && !a.det) MP_PUSH(1, 0);
-    else MP_PUSH(1, 1);
+#define MP_PUSH(W, D, G)                                                                          \
+    hipLaunchKernelGGL((k_push_soa<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
+                       a.dt, a.den, a.keys, a.vals, a.part, a.cap)
+#define MP_PUSH_W(W)                          \
+    do {                                      \
+        if (a.det) MP_PUSH(W, 1, 0);          \
+        else if (a.agg == 2) MP_PUSH(W, 0, 2); \
+        else if (a.agg == 1) MP_PUSH(W, 0, 1); \
+        else MP_PUSH(W, 0, 0);                \
+    } while (0)
+    if (a.wall == 0) MP_PUSH_W(0);
+    else MP_PUSH_W(1);
+#undef MP_PUSH_W
 #undef MP_PUSH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -466,16 +523,16 @@ hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long lo
 int64_t tally_ints(int64_t n, int block) { return ((n + block - 1) / block) * kTally; }
 
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
-                           double* den, uint64_t* keys, double* vals, int pid_bits, int32_t* part,
+                           double* den, uint32_t* keys, double* vals, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const unsigned g = grid_for(n, 256);
     if (det)
         hipLaunchKernelGGL((k_push_aos<1>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
-                           keys, vals, pid_bits, part, cap);
+                           keys, vals, part, cap);
     else
         hipLaunchKernelGGL((k_push_aos<0>), dim3(g), dim3(256), 0, st, tets, parts, n, qm, dt, den,
-                           keys, vals, pid_bits, part, cap);
+                           keys, vals, part, cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_tally_finish(part, g, ctr, st);
@@ -485,6 +542,10 @@ hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_
                             hipStream_t st) {
     hipLaunchKernelGGL(k_normalise, dim3(grid_for(nn, 256)), dim3(256), 0, st, raw, out, nvol, nn, spwt);
     return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st) {
+    return launch_fill_i32((int32_t*)p, n, (int32_t)v, st);
 }
 
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
@@ -499,15 +560,13 @@ hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_det_fold(const uint64_t* keys, const double* vals, int64_t m, int pid_bits,
-                           int32_t* start, const double* nvol, int32_t nn, double spwt, double* out,
-                           hipStream_t st) {
+hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
+                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st) {
     hipError_t e = launch_fill_i32(start, nn, -1, st);
     if (e != hipSuccess) return e;
     if (m > 0)
-        hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, pid_bits, nn,
-                           start);
-    hipLaunchKernelGGL(k_fold, dim3(grid_for((int64_t)nn * kWave, 256)), dim3(256), 0, st, keys, vals, m, pid_bits, start,
+        hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start);
+    hipLaunchKernelGGL(k_fold, dim3(grid_for((int64_t)nn * kWave, 256)), dim3(256), 0, st, keys, vals, m, start,
                        nvol, nn, spwt, out);
     return hipGetLastError();
 }

@@ -25,20 +25,20 @@ struct PushArgs {
     double qm;
     double dt;
     double* den;
-    uint64_t* keys;
-    double* vals;
-    int pid_bits;
+    uint32_t* keys;  // deterministic records: node of slot 4*pid+v
+    double* vals;    //                        weight of slot 4*pid+v
     int32_t* part;  // per-block tallies, tally_ints(n, block) ints
     unsigned long long* ctr;
     int cap;
     int wall;
     bool det;
+    int agg;
     int block;
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
-                           double* den, uint64_t* keys, double* vals, int pid_bits, int32_t* part,
+                           double* den, uint32_t* keys, double* vals, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st);
 // Step counters: the push kernels leave per-block partials; this folds them into ctr.
 hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st);
@@ -47,9 +47,9 @@ hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_
                             hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
-hipError_t launch_det_fold(const uint64_t* keys, const double* vals, int64_t m, int pid_bits,
-                           int32_t* start, const double* nvol, int32_t nn, double spwt, double* out,
-                           hipStream_t st);
+hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
+                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 int64_t scan_scratch_ints(int64_t n);
 hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
 hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int32_t* cnt, hipStream_t st);
@@ -62,9 +62,9 @@ hipError_t launch_unpack(const double* pos, const double* vel, const int32_t* ce
                          const Soa& s, int64_t at, hipStream_t st);
 hipError_t launch_pack(const Soa& s, int64_t n, double* pos, double* vel, hipStream_t st);
 
-// Radix sort of (key, value) pairs on bits [0, end_bit) (mp_sort.hip).
+// Stable radix sort of (key, value) pairs on key bits [0, end_bit) (mp_sort.hip).
 size_t radix_sort_temp_bytes(int64_t n, int end_bit);
-hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                             const double* vals_in, double* vals_out, int64_t n, int end_bit, hipStream_t st);
 
 }  // namespace mpd

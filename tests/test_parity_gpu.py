@@ -215,3 +215,21 @@ def test_cpp_dropin_program(mp):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count(" ok") == 2
+
+
+@pytest.mark.parametrize("agg", [0, 1, 2], ids=["plain", "wave-by-tet", "lds-by-node"])
+def test_atomic_aggregation_modes(mp, orc, agg):
+    """Every atomic-scatter aggregation gives the oracle's density to 1e-12 relative."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=13, n=80000)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(scatter_agg=agg, sort_interval=3)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for _ in range(4):
+            mv.step(QM, dt, SPWT, mp.SCATTER_ATOMIC)
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density(), ref, mp.SCATTER_ATOMIC, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
