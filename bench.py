@@ -56,11 +56,11 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 8.0 * n_nodes)
 
 
-def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2):
+def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=sort_interval, scatter_agg=agg)
+        mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         for _ in range(warmup):
@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--particles", type=int, default=N_PART)
     ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 none, 1 wave/tet, 2 LDS/node")
+    ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
@@ -126,7 +127,8 @@ def main():
     log(f"setup {time.perf_counter() - t:.1f}s: {mesh.n_tets} tets, {mesh.n_nodes} nodes, {n} particles")
 
     mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
-    r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg)
+    r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg,
+                args.sort_shift)
     log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
@@ -163,6 +165,7 @@ def main():
                                "20M FP64 particles, frozen sinusoidal field, absorbing walls",
                    "particles": n, "tets": mesh.n_tets, "nodes": mesh.n_nodes, "dt": DT,
                    "scatter": args.mode, "sort_interval": args.sort_interval, "scatter_agg": args.agg,
+                   "sort_shift": args.sort_shift,
                    "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
                    "lost": c["lost"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

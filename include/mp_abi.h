@@ -71,6 +71,7 @@ typedef struct mp_options {
     int32_t walk_cap;              /* max face moves per particle per step             */
     int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
                                       2 per workgroup by node (LDS hash)               */
+    int32_t sort_shift;            /* cell sort key = cell >> sort_shift (0: exact cell) */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */

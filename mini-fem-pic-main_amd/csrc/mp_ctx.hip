@@ -124,6 +124,7 @@ mp_options default_options() {
     o.particles_per_thread = 1;
     o.walk_cap = 4096;
     o.scatter_agg = 2;
+    o.sort_shift = 0;
     return o;
 }
 
@@ -248,20 +249,24 @@ int det_finish(mp_ctx* c, int64_t m, double spwt) {
     return MP_OK;
 }
 
+// Counting sort of the resident particles by bucket = cell >> sort_shift, dropping the lost
+// ones; order inside a bucket is whatever the atomics hand out (pid is carried along).
 int sort_now(mp_ctx* c) {
     c->since_sort = 0;
     if (c->n_slots == 0) return MP_OK;
-    const int64_t nt1 = (int64_t)c->n_tets + 1;
-    int rc = reserve_scan(c, nt1);
+    const int shift = c->opt.sort_shift;
+    const int64_t nb = (((int64_t)c->n_tets - 1) >> shift) + 1;
+    const int64_t nb1 = nb + 1;
+    int rc = reserve_scan(c, nb1);
     if (rc) return rc;
-    HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nt1, 0, c->stream));
+    HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
     const Soa& src = c->soa[c->cur];
     const Soa& dst = c->soa[1 - c->cur];
-    HIP_TRY(c, mpd::launch_cell_hist(src.cell, c->n_slots, c->cell_cnt, c->stream));
-    HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nt1, c->cell_cur, c->scan_tmp, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + c->n_tets, sizeof(int32_t), hipMemcpyDeviceToHost,
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, c->n_slots, shift, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
-    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, c->cell_cur, dst, c->stream));
+    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, c->cell_cur, dst, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->cur = 1 - c->cur;
     c->n_slots = *c->pinned_i32;
@@ -272,7 +277,7 @@ bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
            o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1 &&
-           o.scatter_agg >= 0 && o.scatter_agg <= 2;
+           o.scatter_agg >= 0 && o.scatter_agg <= 2 && o.sort_shift >= 0 && o.sort_shift <= 16;
 }
 
 }  // namespace

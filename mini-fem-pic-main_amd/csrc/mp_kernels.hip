@@ -6,7 +6,7 @@
 //   k_normalise    den = raw * (spwt / node volume), raw cleared (src/mini-pic.cpp:319)
 //   k_seg_start / k_fold   deterministic scatter: after sorting (node,pid) records, a
 //                  serial fold per node in ascending pid, then the normalisation
-//   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell
+//   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell >> shift
 //   k_scan_*       exclusive prefix sums (counting sort offsets, stable compaction)
 //   k_flag_aos / k_compact_aos     stable erase of cell_index < 0 (src/mini-pic.cpp:316)
 #include "mp_device.h"
@@ -15,6 +15,35 @@
 namespace mpd {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;
+
+#ifndef MP_NT_STREAM
+#define MP_NT_STREAM 1
+#endif
+// Streaming accesses of the resident particle arrays (read once, written once per step):
+// non-temporal, so they do not evict the tet records the walk gathers from L2.
+__device__ __forceinline__ double ld(const double* p) {
+#if MP_NT_STREAM
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void st_(double* p, double v) {
+#if MP_NT_STREAM
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin.  Map physical block b to a logical
+// chunk so that each XCD walks one contiguous eighth of the (cell-sorted) particle array and
+// its L2 holds the tets of one slab of space; launchers pad the grid to a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    return (b % kXcds) * (g / kXcds) + b / kXcds;
+}
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
@@ -117,10 +146,10 @@ __device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets,
 // occupied slot issues one global atomic.  A block of 256 cell-sorted particles touches a few
 // dozen nodes, so ~1024 contributions become ~50 global atomics, however the particles
 // reordered since the last sort.  Every thread of the block must call it.
-constexpr int kHashSlots = 2048;  // >= 4 * max block size, load factor <= 1/2
+constexpr int kHashSlots = 1024;  // >= 4 * max block size: every insert finds a slot
 
 __device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node, double w) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 11);
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 10);
     for (;;) {
         const int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
@@ -156,7 +185,6 @@ __device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, dou
     }
 }
 
-// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block.
 // Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the
 // record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
 // (pre-filled).  A stable sort on the node bits then leaves each node's weights in
@@ -175,12 +203,16 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
     vals[r + 3] = l3;
 }
 
+// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block.
+#ifndef MP_PUSH_WAVES
+#define MP_PUSH_WAVES 5
+#endif
 template <int WALL, int DET, int AGG>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
            int32_t* __restrict__ part, int cap) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     bool act = false, lost = false, crossed = false, capped = false, ok = false;
     int32_t fc = -1;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
@@ -191,17 +223,18 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             act = true;
             const DevTet* t = tets + c;
             const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-            double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+            double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
+            const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
             vx = vx + (qm * ex) * dt;
             vy = vy + (qm * ey) * dt;
             vz = vz + (qm * ez) * dt;
-            double x = s.x[i] + vx * dt;
-            double y = s.y[i] + vy * dt;
-            double z = s.z[i] + vz * dt;
+            double x = x0 + vx * dt;
+            double y = y0 + vy * dt;
+            double z = z0 + vz * dt;
             const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             crossed = (st.moves - st.refl) > 0;
-            s.x[i] = x; s.y[i] = y; s.z[i] = z;
-            s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
+            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             if (r == kInside) {
                 s.cell[i] = c;
                 ok = true;
@@ -337,48 +370,58 @@ k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64
     if (lane == 0) out[n] = s * (spwt / nvol[n]);
 }
 
-// ---- counting sort by cell -----------------------------------------------------------
+// ---- counting sort by cell bucket (cell >> shift) ---------------------------------------
 
-// Lanes holding equal consecutive cells form a run; the run head does one atomic.
-__device__ __forceinline__ void run_info(int32_t c, bool valid, int& head_lane, int& run_len) {
+// Groups the wave's valid lanes by bucket key: for every lane, the lowest lane of its group
+// (leader), its rank among the group's lanes and the group size.  The loop runs once per
+// distinct key in the wave (wave-uniform).
+__device__ __forceinline__ void key_groups(int32_t key, bool valid, int& leader, int& rank, int& count) {
     const int lane = lane_id();
-    const int32_t prev = __shfl_up(c, 1, kWave);
-    const bool head = valid && (lane == 0 || prev != c);
-    const unsigned long long hm = __ballot(head);
-    // head of my run: highest head lane <= lane
-    const unsigned long long below = hm & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
-    head_lane = below ? 63 - __clzll(below) : lane;
-    const unsigned long long above = hm & ~((head_lane == 63) ? ~0ull : ((1ull << (head_lane + 1)) - 1));
-    const unsigned long long vm = __ballot(valid);
-    // run ends at the next head or at the first invalid lane after head_lane
-    const unsigned long long inval_above = ~vm & ~((head_lane == 63) ? ~0ull : ((1ull << (head_lane + 1)) - 1));
-    const unsigned long long stop = above | inval_above;
-    const int end = stop ? __ffsll((long long)stop) - 1 : 64;
-    run_len = end - head_lane;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long pending = __ballot(valid);
+    leader = lane;
+    rank = 0;
+    count = 0;
+    while (pending) {
+        const int l = __ffsll((long long)pending) - 1;
+        const int32_t k = __shfl(key, l, kWave);
+        const bool mine = valid && key == k;
+        const unsigned long long m = __ballot(mine);
+        if (mine) {
+            leader = l;
+            rank = __popcll(m & lt);
+            count = __popcll(m);
+        }
+        pending &= ~m;
+    }
 }
 
 __global__ void __launch_bounds__(256)
-k_cell_hist(const int32_t* __restrict__ cell, int64_t n, int32_t* __restrict__ cnt) {
+k_cell_hist(const int32_t* __restrict__ cell, int64_t n, int shift, int32_t* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (i < n) ? cell[i] : -1;
     const bool valid = c >= 0;
-    int head_lane, run_len;
-    run_info(c, valid, head_lane, run_len);
-    if (valid && head_lane == lane_id()) atomicAdd(cnt + c, run_len);
+    const int32_t key = valid ? (c >> shift) : -1;
+    int leader, rank, count;
+    key_groups(key, valid, leader, rank, count);
+    if (valid && leader == lane_id()) atomicAdd(cnt + key, count);
 }
 
+// Moves every particle still in the domain to its bucket's range; lanes of one bucket in a
+// wave take consecutive slots (one returning atomic per bucket per wave).
 __global__ void __launch_bounds__(256)
-k_cell_scatter(Soa src, int64_t n, int32_t* __restrict__ cursor, Soa dst) {
+k_cell_scatter(Soa src, int64_t n, int shift, int32_t* __restrict__ cursor, Soa dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (i < n) ? src.cell[i] : -1;
     const bool valid = c >= 0;
-    int head_lane, run_len;
-    run_info(c, valid, head_lane, run_len);
+    const int32_t key = valid ? (c >> shift) : -1;
+    int leader, rank, count;
+    key_groups(key, valid, leader, rank, count);
     int32_t base = 0;
-    if (valid && head_lane == lane_id()) base = atomicAdd(cursor + c, run_len);
-    base = __shfl(base, head_lane, kWave);
+    if (valid && leader == lane_id()) base = atomicAdd(cursor + key, count);
+    base = __shfl(base, leader, kWave);
     if (!valid) return;
-    const int64_t d = (int64_t)base + (lane_id() - head_lane);
+    const int64_t d = (int64_t)base + rank;
     dst.x[d] = src.x[i]; dst.y[d] = src.y[i]; dst.z[d] = src.z[i];
     dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
     dst.cell[d] = c;
@@ -495,7 +538,7 @@ static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs -
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
     const int bs = a.block;
-    const unsigned g = grid_for(a.n, bs);
+    const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                          \
     hipLaunchKernelGGL((k_push_soa<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
                        a.dt, a.den, a.keys, a.vals, a.part, a.cap)
@@ -520,7 +563,7 @@ hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long lo
     return hipGetLastError();
 }
 
-int64_t tally_ints(int64_t n, int block) { return ((n + block - 1) / block) * kTally; }
+int64_t tally_ints(int64_t n, int block) { return ((n + block - 1) / block + kXcds) * kTally; }
 
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
                            double* den, uint32_t* keys, double* vals, int32_t* part,
@@ -597,15 +640,16 @@ hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int32_t* cnt, hipStream_t st) {
+hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int shift, int32_t* cnt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, n, cnt);
+    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, n, shift, cnt);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_scatter(const Soa& src, int64_t n, int32_t* cursor, const Soa& dst, hipStream_t st) {
+hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t* cursor, const Soa& dst,
+                               hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, cursor, dst);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, shift, cursor, dst);
     return hipGetLastError();
 }
 

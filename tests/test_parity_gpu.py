@@ -233,3 +233,23 @@ def test_atomic_aggregation_modes(mp, orc, agg):
             check_den(mv.density(), ref, mp.SCATTER_ATOMIC, mp)
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+
+
+@pytest.mark.parametrize("shift", [3, 7])
+def test_sort_shift_matches_oracle(mp, orc, shift):
+    """Sorting by coarser cell buckets (cell >> shift) changes only the memory order."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=17, n=50000)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_shift=shift, sort_interval=2)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(5):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density(), ref, mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
