@@ -81,6 +81,55 @@ def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2,
             "push_ms": push_ms, "launches": launches, "counters": c}
 
 
+def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
+    """Batched ensemble: n_runs independent runs on a 40^3 six-tet box (side 1 m) under the
+    frozen sinusoidal field; per-run counts log-uniform in [1e5, 4e6], drift U[2, 8] km/s,
+    thermal sigma U[500, 3000] m/s.  One graph-captured launch per step for all runs versus
+    one launch sequence per run."""
+    mesh = mp.Mesh.box_mesh(40, 1.0)
+    ef = mesh.field_sinusoid(AMP)
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(np.log(1e5), np.log(4e6), n_runs))).astype(np.int64)
+    drifts = rng.uniform(2000.0, 8000.0, n_runs)
+    sigmas = rng.uniform(500.0, 3000.0, n_runs)
+    dt = 4.0e-7
+    runs = []
+    for r in range(n_runs):
+        pos, vel = mp.synth_uniform(seed * 1000 + r, int(counts[r]), [0, 0, 0], [1, 1, 1], sigmas[r],
+                                    [drifts[r], 0, 0])
+        runs.append((pos, vel, mesh.locate_box(pos)))
+    out = {}
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=10, sort_shift=3)
+        mv.upload_field(ef)
+        for how in ("batched", "sequential"):
+            with mp.Batch(mv, n_runs) as b:
+                for r, (pos, vel, cell) in enumerate(runs):
+                    b.set_run(r, pos, vel, cell, QM, dt, SPWT)
+                go = b.step if how == "batched" else b.step_sequential
+                for _ in range(warmup):
+                    go()
+                b.sync()
+                b.reset_counters()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    go()
+                b.sync()
+                wall = time.perf_counter() - t0
+                c = b.counters()
+            out[how] = {"value": c["particle_steps"] / wall, "ms_per_step": 1e3 * wall / steps}
+            log(f"ensemble {how}: {out[how]['value'] / 1e9:.3f} G particle-steps/s, {out[how]['ms_per_step']:.3f} ms/step")
+    return {"value": out["batched"]["value"], "unit": "particle-steps/s",
+            "ms_per_step": out["batched"]["ms_per_step"],
+            "sequential_value": out["sequential"]["value"],
+            "sequential_ms_per_step": out["sequential"]["ms_per_step"],
+            "speedup_vs_sequential": out["batched"]["value"] / out["sequential"]["value"],
+            "steps": steps, "runs": n_runs, "particles": int(counts.sum()),
+            "workload": "64 runs on a 40^3 six-tet box (384,000 tets), counts log-uniform 1e5..4e6, "
+                        "drift U[2,8] km/s, sigma U[500,3000] m/s, frozen sinusoidal field, dt 4e-7 s; "
+                        "one hipGraph launch per step vs 64 per-run launch sequences"}
+
+
 def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30):
     import oracle
 
@@ -112,6 +161,7 @@ def main():
     ap.add_argument("--particles", type=int, default=N_PART)
     ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 none, 1 wave/tet, 2 LDS/node")
     ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
@@ -139,6 +189,8 @@ def main():
         extra["deterministic"] = {"value": d["rate"], "ms_per_step": 1e3 * d["wall_s"] / args.det_steps,
                                   "steps": args.det_steps}
         log(f"gpu deterministic: {d['rate'] / 1e9:.3f} G particle-steps/s")
+    if args.ensemble_steps > 0:
+        extra["batched"] = run_ensemble(args.ensemble_steps, 3)
     cpu = None
     if args.cpu_seconds > 0:
         cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)

@@ -146,6 +146,33 @@ void* mp_stream(mp_ctx* ctx);
 int  mp_set_timing(mp_ctx* ctx, int32_t enable);
 int  mp_get_timing(mp_ctx* ctx, double* push_ms, int64_t* push_launches);
 
+/* ---- batched ensemble: many independent runs on one mesh (no reference counterpart) -- */
+/* An ensemble of n_runs independent particle sets pushed through the context's mesh and
+ * field.  Each run has its own particles, q/m, dt, spwt and nodal density.  One step of
+ * every run is one push launch over all particles (captured once in a hipGraph and
+ * replayed when graphs are enabled); particle counts live in device memory so the graph
+ * stays valid as particles are lost.  The batch uses the context's stream and options. */
+typedef struct mp_batch mp_batch;
+int  mp_batch_create(mp_ctx* ctx, int32_t n_runs, mp_batch** out);
+void mp_batch_destroy(mp_batch* b);
+/* Set run `run`'s particles (pos, vel [n][3], cell [n]) and parameters; every run must be
+ * set before the first step.  Local pids are 0..n-1 in array order. */
+int  mp_batch_set_run(mp_batch* b, int32_t run, const double* pos, const double* vel,
+                      const int32_t* cell, int64_t n, double q_over_m, double dt, double spwt);
+/* One step of every run in one launch (graph replay when enabled, the default). */
+int  mp_batch_step(mp_batch* b, int32_t mode);
+/* The same step as n_runs separate per-run launches (the sequential baseline). */
+int  mp_batch_step_sequential(mp_batch* b, int32_t mode);
+int  mp_batch_set_graph(mp_batch* b, int32_t enable);
+int  mp_batch_sync(mp_batch* b);
+/* Normalised density of run `run` after the last step, den[n_nodes] (blocks). */
+int  mp_batch_get_density(mp_batch* b, int32_t run, double* den);
+/* Run `run`'s particles still in the domain, ascending local pid (blocks). */
+int  mp_batch_get_particles(mp_batch* b, int32_t run, double* pos, double* vel, int32_t* cell,
+                            int64_t* pid, int64_t* n_inout);
+int  mp_batch_get_counters(mp_batch* b, mp_counters* out);
+int  mp_batch_reset_counters(mp_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,177 @@
+// Device helpers shared by the mover kernels (mp_kernels.hip, mp_batch.hip): wave
+// reductions, XCD-aware block mapping, streaming loads, per-block step tallies, the
+// aggregated atomic scatters and the deterministic record writer.
+#pragma once
+#include "mp_device.h"
+
+namespace mpd {
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;
+
+#ifndef MP_NT_STREAM
+#define MP_NT_STREAM 1
+#endif
+// Streaming accesses of the resident particle arrays (read once, written once per step):
+// non-temporal, so they do not evict the tet records the walk gathers from L2.
+__device__ __forceinline__ double ld(const double* p) {
+#if MP_NT_STREAM
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void st_(double* p, double v) {
+#if MP_NT_STREAM
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin.  Map physical block b to a logical
+// chunk so that each XCD walks one contiguous eighth of the (cell-sorted) particle array and
+// its L2 holds the tets of one slab of space; launchers pad the grid to a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    return (b % kXcds) * (g / kXcds) + b / kXcds;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Per-block step tallies: waves reduce with ballots/shuffles, the block sums its waves in
+// LDS and thread 0 stores kTally int32 partials to part[blockIdx.x * kTally + k] with plain
+// stores.  k_tally_finish folds the partials into the 64-bit counters.  (One atomic per
+// wave on a shared counter line serialises in a single L2 channel: ~10 ms at 20M particles.)
+constexpr int kTally = 6;
+
+__device__ __forceinline__ void tally(int32_t* __restrict__ part, bool act, bool lost, bool crossed,
+                                      int moves, bool capped, int refl) {
+    __shared__ int32_t sh[1024 / kWave][kTally];
+    int v[kTally];
+    v[0] = __popcll(__ballot(act));
+    v[1] = __popcll(__ballot(lost));
+    v[2] = __popcll(__ballot(crossed));
+    v[3] = wave_sum(moves);
+    v[4] = __popcll(__ballot(capped));
+    v[5] = wave_sum(refl);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < kTally; k++) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < kTally) {
+        int s = 0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); q++) s += sh[q][threadIdx.x];
+        part[(int64_t)blockIdx.x * kTally + threadIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Atomic scatter with the wave's contributions grouped by final tet: one leader per distinct
+// tet adds the group's summed weights to its 4 nodes.  Cell-sorted particles put ~15 per
+// tet, so this turns 256 same-line atomics per wave into a few groups of 4.  Must be called
+// by every lane of the wave (the loop is wave-uniform).
+__device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                                bool ok, int32_t fc, double l0, double l1, double l2,
+                                                double l3) {
+    unsigned long long pending = __ballot(ok);
+    const int lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int32_t gc = __shfl(fc, leader, kWave);
+        const bool mine = ok && fc == gc;
+        const unsigned long long m = __ballot(mine);
+        pending &= ~m;
+        double a0 = l0, a1 = l1, a2 = l2, a3 = l3;
+        if (__popcll(m) > 1) {
+            a0 = wave_sum_f64(mine ? l0 : 0.0);
+            a1 = wave_sum_f64(mine ? l1 : 0.0);
+            a2 = wave_sum_f64(mine ? l2 : 0.0);
+            a3 = wave_sum_f64(mine ? l3 : 0.0);
+        }
+        if (lane == leader) {
+            const DevTet* t = tets + gc;
+            unsafeAtomicAdd(den + t->con[0], a0);
+            unsafeAtomicAdd(den + t->con[1], a1);
+            unsafeAtomicAdd(den + t->con[2], a2);
+            unsafeAtomicAdd(den + t->con[3], a3);
+        }
+    }
+}
+
+// Atomic scatter aggregated per workgroup by node: the block's (node, weight) pairs are summed
+// in an LDS hash table (open addressing, linear probing, ds_cmpst + ds_add_f64), then each
+// occupied slot issues one global atomic.  A block of 256 cell-sorted particles touches a few
+// dozen nodes, so ~1024 contributions become ~50 global atomics, however the particles
+// reordered since the last sort.  Every thread of the block must call it.
+constexpr int kHashSlots = 1024;  // >= 4 * max block size: every insert finds a slot
+
+__device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node, double w) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 10);
+    for (;;) {
+        const int32_t prev = atomicCAS(&hk[h], -1, node);
+        if (prev == -1 || prev == node) {
+            atomicAdd(&hv[h], w);
+            return;
+        }
+        h = (h + 1) & (kHashSlots - 1);
+    }
+}
+
+__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                            bool ok, int32_t fc, double l0, double l1, double l2,
+                                            double l3, int32_t base = 0) {
+    __shared__ int32_t hk[kHashSlots];
+    __shared__ double hv[kHashSlots];
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        hk[s] = -1;
+        hv[s] = 0.0;
+    }
+    __syncthreads();
+    if (ok) {
+        const DevTet* t = tets + fc;
+        const int32_t n0 = base + t->con[0], n1 = base + t->con[1], n2 = base + t->con[2],
+                      n3 = base + t->con[3];
+        lds_insert(hk, hv, n0, l0);
+        lds_insert(hk, hv, n1, l1);
+        lds_insert(hk, hv, n2, l2);
+        lds_insert(hk, hv, n3, l3);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        const int32_t node = hk[s];
+        if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
+    }
+}
+
+// Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the
+// record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
+// (pre-filled).  A stable sort on the node bits then leaves each node's weights in
+// ascending pid: the order of the serial MoveParticles fold.
+__device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
+                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
+                                            double l1, double l2, double l3, uint32_t base = 0) {
+    const int64_t r = 4 * pid;
+    keys[r + 0] = base + (uint32_t)tf->con[0];
+    keys[r + 1] = base + (uint32_t)tf->con[1];
+    keys[r + 2] = base + (uint32_t)tf->con[2];
+    keys[r + 3] = base + (uint32_t)tf->con[3];
+    vals[r + 0] = l0;
+    vals[r + 1] = l1;
+    vals[r + 2] = l2;
+    vals[r + 3] = l3;
+}
+
+}  // namespace mpd

@@ -13,86 +13,18 @@
 #include <string>
 #include <vector>
 
-#include "mp_abi.h"
-#include "mp_launch.h"
-
-using mpd::DevTet;
-using mpd::Soa;
-
-struct mp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int32_t n_tets = 0, n_nodes = 0;
-    DevTet* tets = nullptr;
-    double* node_vol = nullptr;
-    double* den_raw = nullptr;
-    double* den_out = nullptr;
-    unsigned long long* ctr = nullptr;
-    bool have_field = false;
-    // resident particles, double-buffered for the cell sort
-    void* soa_mem[2] = {nullptr, nullptr};
-    Soa soa[2] = {};
-    int cur = 0;
-    int64_t cap = 0;
-    int64_t n_slots = 0;
-    int64_t next_pid = 0;
-    int64_t since_sort = 0;
-    // counting-sort and scan scratch
-    int32_t* cell_cnt = nullptr;
-    int32_t* cell_cur = nullptr;
-    int32_t* scan_tmp = nullptr;
-    int64_t scan_tmp_n = 0;
-    // deterministic-scatter scratch
-    uint32_t* keys[2] = {nullptr, nullptr};
-    double* vals[2] = {nullptr, nullptr};
-    int64_t rec_cap = 0;
-    void* sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
-    int32_t* seg_start = nullptr;
-    // drop-in AoS scratch
-    mp_particle* aos = nullptr;
-    mp_particle* aos_out = nullptr;
-    int32_t* flags = nullptr;
-    int32_t* offs = nullptr;
-    int64_t aos_cap = 0;
-    int32_t* pinned_i32 = nullptr;
-    // per-block tallies of the push kernels
-    int32_t* tally = nullptr;
-    int64_t tally_n = 0;
-    // push-kernel timing (events recorded around each push launch while enabled)
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    mp_options opt{};
-    std::string err;
-};
+#include "mp_ctx_impl.h"
 
 namespace {
-
 thread_local std::string g_create_err;
+}  // namespace
+
+namespace mpctx {
 
 int set_err(mp_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     else g_create_err = msg;
     return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                  \
-    do {                                                                                    \
-        hipError_t e__ = (expr);                                                            \
-        if (e__ != hipSuccess)                                                              \
-            return set_err((ctx), MP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e__)); \
-    } while (0)
-
-// Records an event on the context stream from the pool (grows on demand).
-hipError_t record_event(mp_ctx* c) {
-    if (c->ev_used == c->ev_pool.size()) {
-        hipEvent_t e;
-        hipError_t r = hipEventCreate(&e);
-        if (r != hipSuccess) return r;
-        c->ev_pool.push_back(e);
-    }
-    return hipEventRecord(c->ev_pool[c->ev_used++], c->stream);
 }
 
 int bit_width(uint64_t v) {
@@ -104,29 +36,13 @@ int bit_width(uint64_t v) {
     return b;
 }
 
-template <typename T>
-hipError_t dalloc(T** p, int64_t n) {
-    *p = nullptr;
-    if (n <= 0) n = 1;
-    return hipMalloc((void**)p, sizeof(T) * (size_t)n);
-}
-
 void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-mp_options default_options() {
-    mp_options o;
-    o.sort_interval = 10;
-    o.wall_mode = MP_WALL_ABSORB;
-    o.block_size = 256;
-    o.threads_per_particle = 1;
-    o.particles_per_thread = 1;
-    o.walk_cap = 4096;
-    o.scatter_agg = 2;
-    o.sort_shift = 0;
-    return o;
-}
+}  // namespace mpctx
+
+namespace mpctx {
 
 // Lays out one SoA buffer of capacity n in a single allocation.
 hipError_t alloc_soa(void** mem, Soa* s, int64_t n) {
@@ -147,7 +63,49 @@ hipError_t alloc_soa(void** mem, Soa* s, int64_t n) {
     s->vz = (double*)(b + 5 * d);
     s->cell = (int32_t*)(b + 6 * d);
     s->pid = (int32_t*)(b + 6 * d + i);
+    s->run = nullptr;
     return hipSuccess;
+}
+
+int reserve_scan(mp_ctx* c, int64_t n) {
+    const int64_t need = mpd::scan_scratch_ints(n);
+    if (need <= c->scan_tmp_n) return MP_OK;
+    dfree(c->scan_tmp);
+    c->scan_tmp = nullptr;
+    c->scan_tmp_n = 0;
+    if (dalloc(&c->scan_tmp, need) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "scan scratch allocation failed");
+    c->scan_tmp_n = need;
+    return MP_OK;
+}
+
+}  // namespace mpctx
+
+namespace {
+using namespace mpctx;
+
+// Records an event on the context stream from the pool (grows on demand).
+hipError_t record_event(mp_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        c->ev_pool.push_back(e);
+    }
+    return hipEventRecord(c->ev_pool[c->ev_used++], c->stream);
+}
+
+
+mp_options default_options() {
+    mp_options o;
+    o.sort_interval = 10;
+    o.wall_mode = MP_WALL_ABSORB;
+    o.block_size = 256;
+    o.threads_per_particle = 1;
+    o.particles_per_thread = 1;
+    o.walk_cap = 4096;
+    o.scatter_agg = 2;
+    o.sort_shift = 0;
+    return o;
 }
 
 int reserve_soa(mp_ctx* c, int64_t n, bool keep) {
@@ -183,17 +141,6 @@ int reserve_soa(mp_ctx* c, int64_t n, bool keep) {
     c->soa[1] = s[1];
     c->cur = 0;
     c->cap = nc;
-    return MP_OK;
-}
-
-int reserve_scan(mp_ctx* c, int64_t n) {
-    const int64_t need = mpd::scan_scratch_ints(n);
-    if (need <= c->scan_tmp_n) return MP_OK;
-    dfree(c->scan_tmp);
-    c->scan_tmp = nullptr;
-    c->scan_tmp_n = 0;
-    if (dalloc(&c->scan_tmp, need) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "scan scratch allocation failed");
-    c->scan_tmp_n = need;
     return MP_OK;
 }
 
@@ -262,11 +209,11 @@ int sort_now(mp_ctx* c) {
     HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
     const Soa& src = c->soa[c->cur];
     const Soa& dst = c->soa[1 - c->cur];
-    HIP_TRY(c, mpd::launch_cell_hist(src.cell, c->n_slots, shift, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream));
     HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
-    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, c->cell_cur, dst, c->stream));
+    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, (int32_t)nb, c->cell_cur, dst, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->cur = 1 - c->cur;
     c->n_slots = *c->pinned_i32;

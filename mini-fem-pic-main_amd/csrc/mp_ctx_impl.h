@@ -1,0 +1,83 @@
+// Private: the context behind mp_ctx* and the helpers shared by mp_ctx.hip and
+// mp_batch.hip.  Not part of the C ABI.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "mp_abi.h"
+#include "mp_launch.h"
+
+using mpd::DevTet;
+using mpd::Soa;
+
+struct mp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t n_tets = 0, n_nodes = 0;
+    DevTet* tets = nullptr;
+    double* node_vol = nullptr;
+    double* den_raw = nullptr;
+    double* den_out = nullptr;
+    unsigned long long* ctr = nullptr;
+    bool have_field = false;
+    // resident particles, double-buffered for the cell sort
+    void* soa_mem[2] = {nullptr, nullptr};
+    Soa soa[2] = {};
+    int cur = 0;
+    int64_t cap = 0;
+    int64_t n_slots = 0;
+    int64_t next_pid = 0;
+    int64_t since_sort = 0;
+    // counting-sort and scan scratch
+    int32_t* cell_cnt = nullptr;
+    int32_t* cell_cur = nullptr;
+    int32_t* scan_tmp = nullptr;
+    int64_t scan_tmp_n = 0;
+    // deterministic-scatter scratch
+    uint32_t* keys[2] = {nullptr, nullptr};
+    double* vals[2] = {nullptr, nullptr};
+    int64_t rec_cap = 0;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int32_t* seg_start = nullptr;
+    // drop-in AoS scratch
+    mp_particle* aos = nullptr;
+    mp_particle* aos_out = nullptr;
+    int32_t* flags = nullptr;
+    int32_t* offs = nullptr;
+    int64_t aos_cap = 0;
+    int32_t* pinned_i32 = nullptr;
+    // per-block tallies of the push kernels
+    int32_t* tally = nullptr;
+    int64_t tally_n = 0;
+    // push-kernel timing (events recorded around each push launch while enabled)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    mp_options opt{};
+    std::string err;
+};
+
+namespace mpctx {
+
+int set_err(mp_ctx* c, int code, const std::string& msg);
+int bit_width(uint64_t v);
+void dfree(void* p);
+int reserve_scan(mp_ctx* c, int64_t n);
+hipError_t alloc_soa(void** mem, Soa* s, int64_t n);
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)n);
+}
+
+}  // namespace mpctx
+
+#define HIP_TRY(ctx, expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e__ = (expr);                                                            \
+        if (e__ != hipSuccess)                                                              \
+            return mpctx::set_err((ctx), MP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e__)); \
+    } while (0)

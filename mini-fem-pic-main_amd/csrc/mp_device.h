@@ -36,6 +36,7 @@ struct Soa {
     double* vz;
     int32_t* cell;
     int32_t* pid;
+    uint16_t* run;  // batch: run of each particle (nullptr for a single run)
 };
 
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };

@@ -9,77 +9,10 @@
 //   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell >> shift
 //   k_scan_*       exclusive prefix sums (counting sort offsets, stable compaction)
 //   k_flag_aos / k_compact_aos     stable erase of cell_index < 0 (src/mini-pic.cpp:316)
-#include "mp_device.h"
+#include "mp_common.h"
 #include "mp_launch.h"
 
 namespace mpd {
-
-constexpr int kWave = 64;
-constexpr int kXcds = 8;
-
-#ifndef MP_NT_STREAM
-#define MP_NT_STREAM 1
-#endif
-// Streaming accesses of the resident particle arrays (read once, written once per step):
-// non-temporal, so they do not evict the tet records the walk gathers from L2.
-__device__ __forceinline__ double ld(const double* p) {
-#if MP_NT_STREAM
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-__device__ __forceinline__ void st_(double* p, double v) {
-#if MP_NT_STREAM
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
-// Workgroups are dispatched to the 8 XCDs round-robin.  Map physical block b to a logical
-// chunk so that each XCD walks one contiguous eighth of the (cell-sorted) particle array and
-// its L2 holds the tets of one slab of space; launchers pad the grid to a multiple of 8.
-__device__ __forceinline__ int64_t xcd_block() {
-    const int64_t g = gridDim.x, b = blockIdx.x;
-    return (b % kXcds) * (g / kXcds) + b / kXcds;
-}
-
-__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
-
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-// Per-block step tallies: waves reduce with ballots/shuffles, the block sums its waves in
-// LDS and thread 0 stores kTally int32 partials to part[blockIdx.x * kTally + k] with plain
-// stores.  k_tally_finish folds the partials into the 64-bit counters.  (One atomic per
-// wave on a shared counter line serialises in a single L2 channel: ~10 ms at 20M particles.)
-constexpr int kTally = 6;
-
-__device__ __forceinline__ void tally(int32_t* __restrict__ part, bool act, bool lost, bool crossed,
-                                      int moves, bool capped, int refl) {
-    __shared__ int32_t sh[1024 / kWave][kTally];
-    int v[kTally];
-    v[0] = __popcll(__ballot(act));
-    v[1] = __popcll(__ballot(lost));
-    v[2] = __popcll(__ballot(crossed));
-    v[3] = wave_sum(moves);
-    v[4] = __popcll(__ballot(capped));
-    v[5] = wave_sum(refl);
-    const int w = threadIdx.x / kWave;
-    if (lane_id() == 0)
-#pragma unroll
-        for (int k = 0; k < kTally; k++) sh[w][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x < kTally) {
-        int s = 0;
-        for (int q = 0; q < (int)(blockDim.x / kWave); q++) s += sh[q][threadIdx.x];
-        part[(int64_t)blockIdx.x * kTally + threadIdx.x] = s;
-    }
-}
 
 // Sums per-block partials [nb][kTally] into the counters (one block, 6 atomics).
 __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict__ part, int64_t nb,
@@ -101,106 +34,6 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
         for (int q = 0; q < (int)(blockDim.x / kWave); q++) t += sh[q][threadIdx.x];
         if (t) atomicAdd(ctr + threadIdx.x, t);
     }
-}
-
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-// Atomic scatter with the wave's contributions grouped by final tet: one leader per distinct
-// tet adds the group's summed weights to its 4 nodes.  Cell-sorted particles put ~15 per
-// tet, so this turns 256 same-line atomics per wave into a few groups of 4.  Must be called
-// by every lane of the wave (the loop is wave-uniform).
-__device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                                bool ok, int32_t fc, double l0, double l1, double l2,
-                                                double l3) {
-    unsigned long long pending = __ballot(ok);
-    const int lane = lane_id();
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int32_t gc = __shfl(fc, leader, kWave);
-        const bool mine = ok && fc == gc;
-        const unsigned long long m = __ballot(mine);
-        pending &= ~m;
-        double a0 = l0, a1 = l1, a2 = l2, a3 = l3;
-        if (__popcll(m) > 1) {
-            a0 = wave_sum_f64(mine ? l0 : 0.0);
-            a1 = wave_sum_f64(mine ? l1 : 0.0);
-            a2 = wave_sum_f64(mine ? l2 : 0.0);
-            a3 = wave_sum_f64(mine ? l3 : 0.0);
-        }
-        if (lane == leader) {
-            const DevTet* t = tets + gc;
-            unsafeAtomicAdd(den + t->con[0], a0);
-            unsafeAtomicAdd(den + t->con[1], a1);
-            unsafeAtomicAdd(den + t->con[2], a2);
-            unsafeAtomicAdd(den + t->con[3], a3);
-        }
-    }
-}
-
-// Atomic scatter aggregated per workgroup by node: the block's (node, weight) pairs are summed
-// in an LDS hash table (open addressing, linear probing, ds_cmpst + ds_add_f64), then each
-// occupied slot issues one global atomic.  A block of 256 cell-sorted particles touches a few
-// dozen nodes, so ~1024 contributions become ~50 global atomics, however the particles
-// reordered since the last sort.  Every thread of the block must call it.
-constexpr int kHashSlots = 1024;  // >= 4 * max block size: every insert finds a slot
-
-__device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node, double w) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 10);
-    for (;;) {
-        const int32_t prev = atomicCAS(&hk[h], -1, node);
-        if (prev == -1 || prev == node) {
-            atomicAdd(&hv[h], w);
-            return;
-        }
-        h = (h + 1) & (kHashSlots - 1);
-    }
-}
-
-__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                            bool ok, int32_t fc, double l0, double l1, double l2,
-                                            double l3) {
-    __shared__ int32_t hk[kHashSlots];
-    __shared__ double hv[kHashSlots];
-    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
-        hk[s] = -1;
-        hv[s] = 0.0;
-    }
-    __syncthreads();
-    if (ok) {
-        const DevTet* t = tets + fc;
-        const int32_t n0 = t->con[0], n1 = t->con[1], n2 = t->con[2], n3 = t->con[3];
-        lds_insert(hk, hv, n0, l0);
-        lds_insert(hk, hv, n1, l1);
-        lds_insert(hk, hv, n2, l2);
-        lds_insert(hk, hv, n3, l3);
-    }
-    __syncthreads();
-    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
-        const int32_t node = hk[s];
-        if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
-    }
-}
-
-// Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the
-// record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
-// (pre-filled).  A stable sort on the node bits then leaves each node's weights in
-// ascending pid: the order of the serial MoveParticles fold.
-__device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
-                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
-                                            double l1, double l2, double l3) {
-    const int64_t r = 4 * pid;
-    keys[r + 0] = (uint32_t)tf->con[0];
-    keys[r + 1] = (uint32_t)tf->con[1];
-    keys[r + 2] = (uint32_t)tf->con[2];
-    keys[r + 3] = (uint32_t)tf->con[3];
-    vals[r + 0] = l0;
-    vals[r + 1] = l1;
-    vals[r + 2] = l2;
-    vals[r + 3] = l3;
 }
 
 // AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block.
@@ -351,7 +184,7 @@ __device__ __forceinline__ double readlane_f64(double v, int q) {
 __global__ void __launch_bounds__(256)
 k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64_t m,
        const int32_t* __restrict__ start, const double* __restrict__ nvol, int32_t nn, double spwt,
-       double* __restrict__ out) {
+       const double* __restrict__ run_spwt, int32_t nn_mesh, double* __restrict__ out) {
     const int64_t n = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     if (n >= nn) return;  // wave-uniform
     const int lane = lane_id();
@@ -367,7 +200,11 @@ k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64
             if (cnt < kWave) break;
         }
     }
-    if (lane == 0) out[n] = s * (spwt / nvol[n]);
+    if (lane == 0) {
+        // batch: key n = run * nn_mesh + node, per-run spwt (stride 3 in the run table)
+        const double sp = run_spwt ? run_spwt[3 * (n / nn_mesh)] : spwt;
+        out[n] = s * (sp / nvol[n % nn_mesh]);
+    }
 }
 
 // ---- counting sort by cell bucket (cell >> shift) ---------------------------------------
@@ -397,11 +234,12 @@ __device__ __forceinline__ void key_groups(int32_t key, bool valid, int& leader,
 }
 
 __global__ void __launch_bounds__(256)
-k_cell_hist(const int32_t* __restrict__ cell, int64_t n, int shift, int32_t* __restrict__ cnt) {
+k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift,
+            int32_t nb, int32_t* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (i < n) ? cell[i] : -1;
     const bool valid = c >= 0;
-    const int32_t key = valid ? (c >> shift) : -1;
+    const int32_t key = valid ? (run ? (int32_t)run[i] * nb : 0) + (c >> shift) : -1;
     int leader, rank, count;
     key_groups(key, valid, leader, rank, count);
     if (valid && leader == lane_id()) atomicAdd(cnt + key, count);
@@ -410,11 +248,11 @@ k_cell_hist(const int32_t* __restrict__ cell, int64_t n, int shift, int32_t* __r
 // Moves every particle still in the domain to its bucket's range; lanes of one bucket in a
 // wave take consecutive slots (one returning atomic per bucket per wave).
 __global__ void __launch_bounds__(256)
-k_cell_scatter(Soa src, int64_t n, int shift, int32_t* __restrict__ cursor, Soa dst) {
+k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ cursor, Soa dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (i < n) ? src.cell[i] : -1;
     const bool valid = c >= 0;
-    const int32_t key = valid ? (c >> shift) : -1;
+    const int32_t key = valid ? (src.run ? (int32_t)src.run[i] * nb : 0) + (c >> shift) : -1;
     int leader, rank, count;
     key_groups(key, valid, leader, rank, count);
     int32_t base = 0;
@@ -426,6 +264,7 @@ k_cell_scatter(Soa src, int64_t n, int shift, int32_t* __restrict__ cursor, Soa 
     dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
     dst.cell[d] = c;
     dst.pid[d] = src.pid[i];
+    if (src.run) dst.run[d] = src.run[i];
 }
 
 // ---- exclusive scan (int32), 3 phases --------------------------------------------------
@@ -604,13 +443,15 @@ hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st) {
 }
 
 hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
-                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st) {
+                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
+                           const double* run_spwt, int32_t nn_mesh) {
+    if (nn_mesh <= 0) nn_mesh = nn;
     hipError_t e = launch_fill_i32(start, nn, -1, st);
     if (e != hipSuccess) return e;
     if (m > 0)
         hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start);
     hipLaunchKernelGGL(k_fold, dim3(grid_for((int64_t)nn * kWave, 256)), dim3(256), 0, st, keys, vals, m, start,
-                       nvol, nn, spwt, out);
+                       nvol, nn, spwt, run_spwt, nn_mesh, out);
     return hipGetLastError();
 }
 
@@ -640,16 +481,17 @@ hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int shift, int32_t* cnt, hipStream_t st) {
+hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
+                            int32_t* cnt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, n, shift, cnt);
+    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, run, n, shift, nb, cnt);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t* cursor, const Soa& dst,
-                               hipStream_t st) {
+hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
+                               const Soa& dst, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, shift, cursor, dst);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, shift, nb, cursor, dst);
     return hipGetLastError();
 }
 

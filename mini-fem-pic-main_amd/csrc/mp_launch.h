@@ -47,14 +47,19 @@ hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_
                             hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
+// Deterministic fold over nn keys; with run_spwt (batch) key k = run * nn_mesh + node and the
+// run's spwt is run_spwt[3 * run].
 hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
-                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st);
+                           const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
+                           const double* run_spwt = nullptr, int32_t nn_mesh = 0);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 int64_t scan_scratch_ints(int64_t n);
 hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
-hipError_t launch_cell_hist(const int32_t* cell, int64_t n, int shift, int32_t* cnt, hipStream_t st);
-hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t* cursor, const Soa& dst,
-                               hipStream_t st);
+// Counting sort by key = run * nb + (cell >> shift) (run term only when the Soa has runs).
+hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
+                            int32_t* cnt, hipStream_t st);
+hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
+                               const Soa& dst, hipStream_t st);
 hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st);
 hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* off, mp_particle* out,
                               hipStream_t st);

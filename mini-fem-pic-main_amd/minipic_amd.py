@@ -82,6 +82,17 @@ SIGNATURES = {
     "mp_stream": (_P, [_P]),
     "mp_set_timing": (C.c_int, [_P, _I32]),
     "mp_get_timing": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64)]),
+    "mp_batch_create": (C.c_int, [_P, _I32, C.POINTER(_P)]),
+    "mp_batch_destroy": (None, [_P]),
+    "mp_batch_set_run": (C.c_int, [_P, _I32, _P, _P, _P, _I64, _D, _D, _D]),
+    "mp_batch_step": (C.c_int, [_P, _I32]),
+    "mp_batch_step_sequential": (C.c_int, [_P, _I32]),
+    "mp_batch_set_graph": (C.c_int, [_P, _I32]),
+    "mp_batch_sync": (C.c_int, [_P]),
+    "mp_batch_get_density": (C.c_int, [_P, _I32, _P]),
+    "mp_batch_get_particles": (C.c_int, [_P, _I32, _P, _P, _P, _P, C.POINTER(_I64)]),
+    "mp_batch_get_counters": (C.c_int, [_P, C.POINTER(MpCounters)]),
+    "mp_batch_reset_counters": (C.c_int, [_P]),
     # mp_mesh.h
     "mp_mesh_box": (C.c_int, [_I32, _I32, _I32, _D, _D, _D, C.POINTER(_P)]),
     "mp_mesh_load_dat": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
@@ -399,6 +410,78 @@ class Mover:
         ms, n = C.c_double(), C.c_int64()
         self._check(lib().mp_get_timing(self._h, C.byref(ms), C.byref(n)))
         return float(ms.value), int(n.value)
+
+
+class Batch:
+    """Ensemble of independent runs on one Mover's mesh and field (mp_batch_*)."""
+
+    def __init__(self, mover, n_runs):
+        self.mover = mover
+        self.n_runs = n_runs
+        self.n_nodes = mover.n_nodes
+        h = C.c_void_p()
+        mover._check(lib().mp_batch_create(mover.handle, n_runs, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mp_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_run(self, run, pos, vel, cell, q_over_m, dt, spwt):
+        pos, vel = _f64(pos, (-1, 3)), _f64(vel, (-1, 3))
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        self.mover._check(lib().mp_batch_set_run(self._h, run, _ptr(pos), _ptr(vel), _ptr(cell), len(pos),
+                                                 q_over_m, dt, spwt))
+
+    def step(self, mode=SCATTER_ATOMIC):
+        self.mover._check(lib().mp_batch_step(self._h, mode))
+
+    def step_sequential(self, mode=SCATTER_ATOMIC):
+        self.mover._check(lib().mp_batch_step_sequential(self._h, mode))
+
+    def set_graph(self, enable=True):
+        self.mover._check(lib().mp_batch_set_graph(self._h, int(bool(enable))))
+
+    def sync(self):
+        self.mover._check(lib().mp_batch_sync(self._h))
+
+    def density(self, run):
+        den = np.empty(self.n_nodes, np.float64)
+        self.mover._check(lib().mp_batch_get_density(self._h, run, _ptr(den)))
+        return den
+
+    def get_particles(self, run):
+        n = C.c_int64(0)
+        lib().mp_batch_get_particles(self._h, run, None, None, None, None, C.byref(n))
+        cap = n.value
+        pos, vel = np.empty((cap, 3)), np.empty((cap, 3))
+        cell, pid = np.empty(cap, np.int32), np.empty(cap, np.int64)
+        n = C.c_int64(cap)
+        self.mover._check(lib().mp_batch_get_particles(self._h, run, _ptr(pos), _ptr(vel), _ptr(cell), _ptr(pid),
+                                                       C.byref(n)))
+        k = n.value
+        return pos[:k], vel[:k], cell[:k], pid[:k]
+
+    def counters(self):
+        c = MpCounters()
+        self.mover._check(lib().mp_batch_get_counters(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self):
+        self.mover._check(lib().mp_batch_reset_counters(self._h))
 
 
 def particles_from_arrays(pos, vel, cell):

@@ -253,3 +253,63 @@ def test_sort_shift_matches_oracle(mp, orc, shift):
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
         assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+
+
+def _ensemble(mp, n_runs=5, seed=21, n_hex=6, side=0.1):
+    mesh = mp.Mesh.box_mesh(n_hex, side)
+    ef = mesh.field_sinusoid(10.0)
+    rng = np.random.default_rng(seed)
+    runs = []
+    for r in range(n_runs):
+        n = int(rng.integers(2000, 12000))
+        drift, sigma = rng.uniform(2000, 8000), rng.uniform(500, 3000)
+        pos, vel = mp.synth_uniform(seed * 100 + r, n, [0, 0, 0], [side] * 3, sigma, [drift, 0, 0])
+        runs.append(dict(pos=pos, vel=vel, cell=mesh.locate_box(pos), qm=QM * (1 + 0.1 * r),
+                         dt=4e-7 * (1 + 0.05 * r), spwt=SPWT * (r + 1)))
+    return mesh, ef, runs
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "deterministic"])
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "stream"])
+def test_batch_matches_per_run_oracle(mp, orc, mode, graph):
+    """Every run of the ensemble equals its own serial MoveParticles (per-run q/m, dt, spwt)."""
+    mesh, ef, runs = _ensemble(mp)
+    oracles = [orc.Run(mesh.tets, ef, mesh.node_volume, r["pos"], r["vel"], r["cell"]) for r in runs]
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=2, sort_shift=2)
+        mv.upload_field(ef)
+        with mp.Batch(mv, len(runs)) as b:
+            b.set_graph(graph)
+            for k, r in reversed(list(enumerate(runs))):  # set order does not matter
+                b.set_run(k, r["pos"], r["vel"], r["cell"], r["qm"], r["dt"], r["spwt"])
+            for step in range(5):
+                b.step(mode)
+                for k, r in enumerate(runs):
+                    ref = oracles[k].step(r["qm"], r["dt"], r["spwt"])
+                    check_den(b.density(k), ref, mode, mp)
+            for k in range(len(runs)):
+                gp, gv, gc, gid = b.get_particles(k)
+                o = oracles[k]
+                assert np.array_equal(gid, o.pid) and np.array_equal(gc, o.cell)
+                assert gp.tobytes() == o.pos.tobytes() and gv.tobytes() == o.vel.tobytes()
+            c = b.counters()
+            assert c["particle_steps"] == sum(o.stats["particle_steps"] for o in oracles)
+            assert c["lost"] == sum(o.stats["lost"] for o in oracles) > 0
+
+
+def test_batch_sequential_equals_batched(mp):
+    mesh, ef, runs = _ensemble(mp, n_runs=4, seed=5)
+    out = {}
+    for how in ("batched", "sequential"):
+        with mp.Mover.from_mesh(mesh) as mv:
+            mv.upload_field(ef)
+            with mp.Batch(mv, len(runs)) as b:
+                for k, r in enumerate(runs):
+                    b.set_run(k, r["pos"], r["vel"], r["cell"], r["qm"], r["dt"], r["spwt"])
+                for _ in range(4):
+                    b.step() if how == "batched" else b.step_sequential()
+                out[how] = [(b.density(k), b.get_particles(k)) for k in range(len(runs))]
+    for (da, pa), (db, pb) in zip(out["batched"], out["sequential"]):
+        np.testing.assert_allclose(da, db, rtol=ATOMIC_RTOL, atol=0)
+        for x, y in zip(pa, pb):
+            assert x.tobytes() == y.tobytes()
