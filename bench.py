@@ -130,6 +130,45 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
                         "one hipGraph launch per step vs 64 per-run launch sequences"}
 
 
+def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
+    """Adjoint: 2M particles on a 30^3 six-tet box (side 1 m), frozen sinusoidal field, K = 50
+    steps, checkpoint every 10; L = |den_K - den_target|^2 with the target from a forward run
+    at 1.05 x drift.  Compares mp_adjoint (forward with checkpoints + replayed reverse sweep)
+    with the plain K-step forward (upload + K atomic steps) on the same particles."""
+    mesh = mp.Mesh.box_mesh(30, 1.0)
+    ef = mesh.field_sinusoid(AMP)
+    pos, u = mp.synth_uniform(seed, n, [0, 0, 0], [1, 1, 1], SIGMA, [0, 0, 0])
+    cell = mesh.locate_box(pos)
+    d = np.array([1.0, 0.0, 0.0])
+    dt = 4.0e-7
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=steps, sort_shift=3)
+        mv.upload_field(ef)
+
+        def forward(drift):
+            mv.set_particles(pos, u + drift * d, cell)
+            for _ in range(steps):
+                mv.step(QM, dt, SPWT, mp.SCATTER_ATOMIC)
+            mv.sync()
+
+        forward(1.05 * DRIFT)
+        tgt = mv.density()
+        fwd, adj = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            forward(DRIFT)
+            fwd.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            loss, g, gd = mv.adjoint(pos, u + DRIFT * d, cell, QM, dt, SPWT, steps, tgt, d, 10)
+            adj.append(time.perf_counter() - t0)
+    f, a = min(fwd), min(adj)
+    log(f"adjoint: forward {1e3 * f:.1f} ms, adjoint {1e3 * a:.1f} ms, ratio {a / f:.2f}")
+    return {"forward_ms": 1e3 * f, "adjoint_ms": 1e3 * a, "ratio": a / f, "steps": steps, "particles": n,
+            "loss": loss, "grad_norm": float(np.linalg.norm(g)), "dL_ddrift": gd,
+            "workload": "2M particles on a 30^3 six-tet box (162,000 tets), 50 steps, checkpoint every 10; both "
+                        "timings include the particle upload (best of 3)"}
+
+
 def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30):
     import oracle
 
@@ -162,6 +201,7 @@ def main():
     ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 none, 1 wave/tet, 2 LDS/node")
     ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
@@ -191,6 +231,8 @@ def main():
         log(f"gpu deterministic: {d['rate'] / 1e9:.3f} G particle-steps/s")
     if args.ensemble_steps > 0:
         extra["batched"] = run_ensemble(args.ensemble_steps, 3)
+    if args.adjoint_steps > 0:
+        extra["adjoint"] = run_adjoint(args.adjoint_steps)
     cpu = None
     if args.cpu_seconds > 0:
         cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)

@@ -173,6 +173,20 @@ int  mp_batch_get_particles(mp_batch* b, int32_t run, double* pos, double* vel, 
 int  mp_batch_get_counters(mp_batch* b, mp_counters* out);
 int  mp_batch_reset_counters(mp_batch* b);
 
+/* ---- adjoint of K frozen-field steps (no reference counterpart) ------------------------ */
+/* Runs K MoveParticles steps (absorbing walls) from the given initial particles under the
+ * context's field, with L = sum_n (den_K[n] - den_target[n])^2 (den_K normalised by
+ * spwt / node volume), and returns L, dL/dE [n_tets][3] and dL/d(drift) where the initial
+ * velocities are u + drift * drift_dir.  Derivatives are those of the piece of the
+ * piecewise-smooth loss the inputs sit in (every particle's tet path held fixed).  The
+ * forward pass stores the particle state every checkpoint_every steps; the backward pass
+ * replays each segment, recording the tets crossed, and sweeps it in reverse.  Replaces the
+ * context's resident particle set (it holds the state after K steps on return). */
+int  mp_adjoint(mp_ctx* ctx, const double* pos0, const double* vel0, const int32_t* cell0, int64_t n,
+                double q_over_m, double dt, double spwt, int32_t steps, const double* den_target,
+                const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,
+                double* dL_ddrift);
+
 #ifdef __cplusplus
 }
 #endif

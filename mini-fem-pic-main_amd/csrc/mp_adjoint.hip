@@ -1,0 +1,324 @@
+// Adjoint of K push + walk + scatter steps under a frozen per-element field (mp_adjoint in
+// include/mp_abi.h).  No reference counterpart: the forward pass is MoveParticles
+// (src/mini-pic.cpp:287-308) with absorbing walls, repeated K times on the same particles.
+//
+// With every particle's tet path c_0..c_{K-1} held fixed, one step is
+//     v_{k+1} = v_k + (qm E[c_k]) dt,   x_{k+1} = x_k + v_{k+1} dt
+// and den_K[n] = (spwt / V_n) sum over particles of lc_v(x_K) in the final tet, with
+// grad lc_v = (-beta_v, gamma_v, -delta_v) / (6 V).  The reverse sweep keeps
+//     lambda_x(k) = lambda_x(K) = sum_v dL/dlc_v grad lc_v,
+//     lambda_v(k+1) = lambda_v(k+2) + lambda_x dt,   dL/dE[c_k] += qm dt lambda_v(k+1),
+// and dL/d(drift) = lambda_v(0) . drift_dir.
+//
+// Memory: the state (x, v, cell) is copied every checkpoint_every steps during the forward
+// pass; the backward pass restores each checkpoint into the second particle buffer, replays
+// its segment recording c_k (4 B per particle per step), then sweeps the segment backwards.
+// dL/dE contributions are summed per workgroup in an LDS hash keyed by tet (3 values per
+// key) before the global FP64 atomics.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mp_common.h"
+#include "mp_ctx_impl.h"
+
+namespace mpd {
+
+// One forward step without density scatter (SCATTER = 0) or with the LDS-hash scatter of
+// the final step (SCATTER = 1).  path (optional) receives the cell at the start of the step.
+template <int SCATTER>
+__global__ void __launch_bounds__(256)
+k_adj_step(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, int cap,
+           int32_t* __restrict__ path, double* __restrict__ den) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    int32_t fc = -1;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+    if (i < n) {
+        int32_t c = s.cell[i];
+        if (path) path[i] = c;
+        if (c >= 0) {
+            const DevTet* t = tets + c;
+            double vx = s.vx[i] + (qm * t->ef[0]) * dt;
+            double vy = s.vy[i] + (qm * t->ef[1]) * dt;
+            double vz = s.vz[i] + (qm * t->ef[2]) * dt;
+            double x = s.x[i] + vx * dt;
+            double y = s.y[i] + vy * dt;
+            double z = s.z[i] + vz * dt;
+            WalkStats st{0, 0};
+            const int r = walk<0>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            s.x[i] = x; s.y[i] = y; s.z[i] = z;
+            s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
+            if (r == kInside) {
+                s.cell[i] = c;
+                ok = true;
+                fc = c;
+            } else {
+                s.cell[i] = -1;
+            }
+        }
+    }
+    if (SCATTER) scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
+}
+
+// den = raw * spwt / V, g = 2 (den - target) spwt / V (dL/d raw weight), raw cleared;
+// per-block partial sums of (den - target)^2 go to part[block].
+__global__ void __launch_bounds__(256)
+k_adj_loss(double* __restrict__ raw, const double* __restrict__ tgt, const double* __restrict__ nvol,
+           int32_t nn, double spwt, double* __restrict__ g, double* __restrict__ part) {
+    __shared__ double sh[256 / kWave];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double r2 = 0.0;
+    if (i < nn) {
+        const double w = spwt / nvol[i];
+        const double r = raw[i] * w - tgt[i];
+        g[i] = 2.0 * r * w;
+        r2 = r * r;
+        raw[i] = 0.0;
+    }
+    r2 = wave_sum_f64(r2);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = r2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) s += sh[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// lambda_x of each particle in the domain after K steps; lambda_v cleared.
+__global__ void k_adj_lambda(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ g,
+                             double* __restrict__ lx, double* __restrict__ lv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0, b = 0.0, c3 = 0.0;
+    const int32_t c = s.cell[i];
+    if (c >= 0) {
+        const DevTet* t = tets + c;
+        for (int q = 0; q < 4; q++) {
+            const double f = g[t->con[q]] / (6.0 * t->vol);
+            a += f * -t->c[q][1];
+            b += f * t->c[q][2];
+            c3 += f * -t->c[q][3];
+        }
+    }
+    lx[3 * i + 0] = a; lx[3 * i + 1] = b; lx[3 * i + 2] = c3;
+    lv[3 * i + 0] = 0.0; lv[3 * i + 1] = 0.0; lv[3 * i + 2] = 0.0;
+}
+
+// LDS hash with three values per key (tet id), for the dL/dE sums.
+__device__ __forceinline__ void lds_insert3(int32_t* hk, double* hv, int32_t key, double a, double b, double c) {
+    uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - 10);
+    for (;;) {
+        const int32_t prev = atomicCAS(&hk[h], -1, key);
+        if (prev == -1 || prev == key) {
+            atomicAdd(&hv[3 * h + 0], a);
+            atomicAdd(&hv[3 * h + 1], b);
+            atomicAdd(&hv[3 * h + 2], c);
+            return;
+        }
+        h = (h + 1) & (kHashSlots - 1);
+    }
+}
+
+// Reverse sweep of steps [k0, k1) of one segment, path[(k - k0) * n + i] = c_k of particle i.
+__global__ void __launch_bounds__(256)
+k_adj_sweep(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n, const double* __restrict__ lx,
+            double* __restrict__ lv, double qm, double dt, double* __restrict__ dLdE) {
+    __shared__ int32_t hk[kHashSlots];
+    __shared__ double hv[3 * kHashSlots];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    double ax = 0, ay = 0, az = 0, vx = 0, vy = 0, vz = 0;
+    if (live) {
+        ax = lx[3 * i]; ay = lx[3 * i + 1]; az = lx[3 * i + 2];
+        vx = lv[3 * i]; vy = lv[3 * i + 1]; vz = lv[3 * i + 2];
+    }
+    const bool any = live && (ax != 0.0 || ay != 0.0 || az != 0.0);
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        hk[s] = -1;
+        hv[3 * s] = hv[3 * s + 1] = hv[3 * s + 2] = 0.0;
+    }
+    __syncthreads();
+    for (int32_t k = k1 - 1; k >= k0; k--) {
+        vx += ax * dt;
+        vy += ay * dt;
+        vz += az * dt;
+        if (any) {
+            const int32_t c = path[(int64_t)(k - k0) * n + i];
+            if (c >= 0) lds_insert3(hk, hv, c, (qm * dt) * vx, (qm * dt) * vy, (qm * dt) * vz);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+            const int32_t key = hk[s];
+            if (key >= 0) {
+                unsafeAtomicAdd(dLdE + 3 * (int64_t)key + 0, hv[3 * s + 0]);
+                unsafeAtomicAdd(dLdE + 3 * (int64_t)key + 1, hv[3 * s + 1]);
+                unsafeAtomicAdd(dLdE + 3 * (int64_t)key + 2, hv[3 * s + 2]);
+                hk[s] = -1;
+                hv[3 * s] = hv[3 * s + 1] = hv[3 * s + 2] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        lv[3 * i] = vx; lv[3 * i + 1] = vy; lv[3 * i + 2] = vz;
+    }
+}
+
+// Per-block partial sums of lambda_v . dir.
+__global__ void __launch_bounds__(256)
+k_adj_drift(const double* __restrict__ lv, int64_t n, double d0, double d1, double d2, double* __restrict__ part) {
+    __shared__ double sh[256 / kWave];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) v = (lv[3 * i] * d0 + lv[3 * i + 1] * d1) + lv[3 * i + 2] * d2;
+    v = wave_sum_f64(v);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) s += sh[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+static inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace mpd
+
+using namespace mpctx;
+
+namespace {
+
+struct AdjBuffers {
+    std::vector<void*> owned;
+    template <typename T>
+    T* get(int64_t n) {
+        T* p = nullptr;
+        if (dalloc(&p, n) != hipSuccess) return nullptr;
+        owned.push_back(p);
+        return p;
+    }
+    ~AdjBuffers() {
+        for (void* p : owned) dfree(p);
+    }
+};
+
+hipError_t copy_state(const Soa& src, const Soa& dst, int64_t n, hipStream_t st) {
+    const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int32_t) * (size_t)n;
+    const double* s6[6] = {src.x, src.y, src.z, src.vx, src.vy, src.vz};
+    double* d6[6] = {dst.x, dst.y, dst.z, dst.vx, dst.vy, dst.vz};
+    for (int k = 0; k < 6; k++) {
+        hipError_t e = hipMemcpyAsync(d6[k], s6[k], nd, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(dst.cell, src.cell, ni, hipMemcpyDeviceToDevice, st);
+}
+
+}  // namespace
+
+extern "C" int mp_adjoint(mp_ctx* c, const double* pos0, const double* vel0, const int32_t* cell0, int64_t n,
+                          double q_over_m, double dt, double spwt, int32_t steps, const double* den_target,
+                          const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,
+                          double* dL_ddrift) {
+    if (!c) return MP_ERR_ARG;
+    if (steps < 1 || checkpoint_every < 1 || !den_target || !drift_dir || !loss || !dL_dE || !dL_ddrift)
+        return set_err(c, MP_ERR_ARG, "mp_adjoint: bad arguments");
+    if (c->opt.wall_mode != MP_WALL_ABSORB) return set_err(c, MP_ERR_ARG, "mp_adjoint: absorbing walls only");
+    int rc = mp_set_particles(c, pos0, vel0, cell0, n);
+    if (rc) return rc;
+    // one cell sort for locality (pids are irrelevant to the summed gradients)
+    {
+        const int32_t keep = c->opt.sort_interval;
+        c->opt.sort_interval = 1;
+        rc = mp_sort_now(c);
+        c->opt.sort_interval = keep;
+        if (rc) return rc;
+    }
+    const int64_t m = c->n_slots;
+    const int32_t nn = c->n_nodes, T = c->n_tets;
+    const int32_t C = std::min(checkpoint_every, steps);
+    const int32_t nseg = (steps + C - 1) / C;
+    hipStream_t st = c->stream;
+    AdjBuffers buf;
+    std::vector<void*> ck_mem((size_t)nseg, nullptr);
+    std::vector<Soa> cks((size_t)nseg);
+    for (int32_t k = 0; k < nseg; k++) {
+        if (alloc_soa(&ck_mem[k], &cks[k], std::max<int64_t>(m, 1)) != hipSuccess) {
+            for (void* p : ck_mem) dfree(p);
+            return set_err(c, MP_ERR_ALLOC, "mp_adjoint: checkpoint allocation failed");
+        }
+        buf.owned.push_back(ck_mem[k]);
+    }
+    const int64_t mm = std::max<int64_t>(m, 1);
+    int32_t* path = buf.get<int32_t>((int64_t)C * mm);
+    double* lx = buf.get<double>(3 * mm);
+    double* lv = buf.get<double>(3 * mm);
+    double* den = buf.get<double>(nn);
+    double* tgt = buf.get<double>(nn);
+    double* g = buf.get<double>(nn);
+    double* dE = buf.get<double>(3 * (int64_t)T);
+    const int64_t nb_loss = mpd::g256(nn), nb_part = mpd::g256(mm);
+    double* part = buf.get<double>(std::max(nb_loss, nb_part));
+    if (!path || !lx || !lv || !den || !tgt || !g || !dE || !part)
+        return set_err(c, MP_ERR_ALLOC, "mp_adjoint: workspace allocation failed");
+    HIP_TRY(c, hipMemcpyAsync(tgt, den_target, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice, st));
+    HIP_TRY(c, mpd::launch_fill_f64(den, nn, 0.0, st));
+    HIP_TRY(c, mpd::launch_fill_f64(dE, 3 * (int64_t)T, 0.0, st));
+    const Soa& S = c->soa[c->cur];
+    const Soa& W = c->soa[1 - c->cur];
+    const int cap = c->opt.walk_cap;
+    const unsigned gm = mpd::g256(mm);
+    // forward with checkpoints
+    for (int32_t k = 0; k < steps; k++) {
+        if (k % C == 0) HIP_TRY(c, copy_state(S, cks[k / C], m, st));
+        if (m > 0) {
+            if (k == steps - 1)
+                hipLaunchKernelGGL((mpd::k_adj_step<1>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap,
+                                   (int32_t*)nullptr, den);
+            else
+                hipLaunchKernelGGL((mpd::k_adj_step<0>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap,
+                                   (int32_t*)nullptr, (double*)nullptr);
+            HIP_TRY(c, hipGetLastError());
+        }
+    }
+    hipLaunchKernelGGL(mpd::k_adj_loss, dim3((unsigned)nb_loss), dim3(256), 0, st, den, tgt, c->node_vol, nn, spwt, g,
+                       part);
+    HIP_TRY(c, hipGetLastError());
+    std::vector<double> hp((size_t)std::max(nb_loss, nb_part));
+    HIP_TRY(c, hipMemcpyAsync(hp.data(), part, sizeof(double) * (size_t)nb_loss, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    double L = 0.0;
+    for (int64_t b = 0; b < nb_loss; b++) L += hp[(size_t)b];
+    // backward: lambda_x from the final state, then segments in reverse
+    if (m > 0) {
+        hipLaunchKernelGGL(mpd::k_adj_lambda, dim3(gm), dim3(256), 0, st, c->tets, S, m, g, lx, lv);
+        HIP_TRY(c, hipGetLastError());
+    }
+    for (int32_t sgi = nseg - 1; sgi >= 0 && m > 0; sgi--) {
+        const int32_t k0 = sgi * C, k1 = std::min(steps, k0 + C);
+        HIP_TRY(c, copy_state(cks[sgi], W, m, st));
+        for (int32_t k = k0; k < k1; k++) {
+            hipLaunchKernelGGL((mpd::k_adj_step<0>), dim3(gm), dim3(256), 0, st, c->tets, W, m, q_over_m, dt, cap,
+                               path + (int64_t)(k - k0) * m, (double*)nullptr);
+            HIP_TRY(c, hipGetLastError());
+        }
+        hipLaunchKernelGGL(mpd::k_adj_sweep, dim3(gm), dim3(256), 0, st, path, k0, k1, m, lx, lv, q_over_m, dt, dE);
+        HIP_TRY(c, hipGetLastError());
+    }
+    double D = 0.0;
+    if (m > 0) {
+        hipLaunchKernelGGL(mpd::k_adj_drift, dim3((unsigned)nb_part), dim3(256), 0, st, lv, m, drift_dir[0], drift_dir[1],
+                           drift_dir[2], part);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(hp.data(), part, sizeof(double) * (size_t)nb_part, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(c, hipMemcpyAsync(dL_dE, dE, sizeof(double) * 3 * (size_t)T, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    if (m > 0)
+        for (int64_t b = 0; b < nb_part; b++) D += hp[(size_t)b];
+    *loss = L;
+    *dL_ddrift = D;
+    return MP_OK;
+}
