@@ -52,6 +52,19 @@ def cpu_model():
     return platform.processor()
 
 
+def measured_traffic():
+    """HBM bytes per push launch from the committed rocprofv3 --pmc summary (FETCH_SIZE and
+    WRITE_SIZE runs of this workload, calibrated on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "k_push_soa_box60_20M_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes"):
+                return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 8.0 * n_nodes)
 
@@ -263,7 +276,7 @@ def main():
                    "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
                    "lost": c["lost"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_push_soa", "avg_launch_ms": avg_push_s * 1e3,
                      "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,

@@ -9,17 +9,20 @@
 //   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell >> shift
 //   k_scan_*       exclusive prefix sums (counting sort offsets, stable compaction)
 //   k_flag_aos / k_compact_aos     stable erase of cell_index < 0 (src/mini-pic.cpp:316)
+#include <algorithm>
+
 #include "mp_common.h"
 #include "mp_launch.h"
 
 namespace mpd {
 
-// Sums per-block partials [nb][kTally] into the counters (one block, 6 atomics).
+// Sums per-block partials [nb][kTally] into the counters: each of up to 64 workgroups folds
+// a strided share of the partials and adds 6 totals.
 __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict__ part, int64_t nb,
                                                        unsigned long long* __restrict__ ctr) {
     __shared__ unsigned long long sh[1024 / kWave][kTally];
     unsigned long long s[kTally] = {0, 0, 0, 0, 0, 0};
-    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x)
 #pragma unroll
         for (int k = 0; k < kTally; k++) s[k] += (unsigned long long)(uint32_t)part[b * kTally + k];
 #pragma unroll
@@ -398,7 +401,8 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st) {
-    hipLaunchKernelGGL(k_tally_finish, dim3(1), dim3(1024), 0, st, part, nb, ctr);
+    const unsigned g = (unsigned)std::min<int64_t>(64, std::max<int64_t>(1, (nb + 4095) / 4096));
+    hipLaunchKernelGGL(k_tally_finish, dim3(g), dim3(1024), 0, st, part, nb, ctr);
     return hipGetLastError();
 }
 
