@@ -69,11 +69,11 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 8.0 * n_nodes)
 
 
-def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0):
+def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift)
+        mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         for _ in range(warmup):
@@ -214,6 +214,7 @@ def main():
     ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 none, 1 wave/tet, 2 LDS/node")
     ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
+    ap.add_argument("--block", type=int, default=256, help="push workgroup size (64, 128, 256)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -231,7 +232,7 @@ def main():
 
     mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
     r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg,
-                args.sort_shift)
+                args.sort_shift, args.block)
     log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)

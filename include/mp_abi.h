@@ -70,7 +70,8 @@ typedef struct mp_options {
     int32_t particles_per_thread;  /* 1, 2 or 4 (TPP=1 only)                           */
     int32_t walk_cap;              /* max face moves per particle per step             */
     int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
-                                      2 per workgroup by node (LDS hash)               */
+                                      2 per workgroup by node (LDS hash), 3 persistent
+                                      pipelined workgroups, LDS hash kept across chunks */
     int32_t sort_shift;            /* cell sort key = cell >> sort_shift (0: exact cell) */
 } mp_options;
 

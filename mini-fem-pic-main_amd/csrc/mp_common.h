@@ -45,6 +45,23 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
+// Block-wide step tallies from per-thread counts (persistent kernels accumulate over chunks).
+__device__ __forceinline__ void tally_counts(int32_t* __restrict__ part, int a, int l, int x, int m, int k,
+                                             int r) {
+    __shared__ int32_t shc[1024 / kWave][6];
+    int v[6] = {wave_sum(a), wave_sum(l), wave_sum(x), wave_sum(m), wave_sum(k), wave_sum(r)};
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < 6; q++) shc[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int s = 0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); q++) s += shc[q][threadIdx.x];
+        part[(int64_t)blockIdx.x * 6 + threadIdx.x] = s;
+    }
+}
+
 // Per-block step tallies: waves reduce with ballots/shuffles, the block sums its waves in
 // LDS and thread 0 stores kTally int32 partials to part[blockIdx.x * kTally + k] with plain
 // stores.  k_tally_finish folds the partials into the 64-bit counters.  (One atomic per

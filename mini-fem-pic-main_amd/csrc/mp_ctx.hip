@@ -224,7 +224,7 @@ bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
            o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1 &&
-           o.scatter_agg >= 0 && o.scatter_agg <= 2 && o.sort_shift >= 0 && o.sort_shift <= 16;
+           o.scatter_agg >= 0 && o.scatter_agg <= 3 && o.sort_shift >= 0 && o.sort_shift <= 16;
 }
 
 }  // namespace

@@ -100,4 +100,55 @@ __device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c,
     }
 }
 
+// The start cell's fields, loaded in one batch with the particle state (they share the tet
+// record's cache lines with the field the push needs), so the first walk iteration does not
+// wait for a second round trip.  Plain scalars: no dynamically indexed register arrays.
+struct StartTet {
+    double a0, b0, g0, d0, a1, b1, g1, d1, a2, b2, g2, d2, a3, b3, g3, d3, vol;
+    int32_t n0, n1, n2, n3;
+};
+
+__device__ __forceinline__ void load_start(const DevTet* __restrict__ t, StartTet& s) {
+    s.a0 = t->c[0][0]; s.b0 = t->c[0][1]; s.g0 = t->c[0][2]; s.d0 = t->c[0][3];
+    s.a1 = t->c[1][0]; s.b1 = t->c[1][1]; s.g1 = t->c[1][2]; s.d1 = t->c[1][3];
+    s.a2 = t->c[2][0]; s.b2 = t->c[2][1]; s.g2 = t->c[2][2]; s.d2 = t->c[2][3];
+    s.a3 = t->c[3][0]; s.b3 = t->c[3][1]; s.g3 = t->c[3][2]; s.d3 = t->c[3][3];
+    s.vol = t->vol;
+    s.n0 = t->nbr[0]; s.n1 = t->nbr[1]; s.n2 = t->nbr[2]; s.n3 = t->nbr[3];
+}
+
+// XtoLtet from cell c whose start-tet fields s are already in registers: the first
+// iteration runs on s; a particle that moves continues in walk() from the neighbour with
+// one move counted, exactly as walk() would have counted it.
+template <int WALL>
+__device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const StartTet& s, int32_t& c,
+                                          double& x, double& y, double& z, double& vx, double& vy,
+                                          double& vz, double& l0, double& l1, double& l2, double& l3,
+                                          int cap, WalkStats& st) {
+    l0 = bary(s.a0, s.b0, s.g0, s.d0, x, y, z, s.vol);
+    l1 = bary(s.a1, s.b1, s.g1, s.d1, x, y, z, s.vol);
+    l2 = bary(s.a2, s.b2, s.g2, s.d2, x, y, z, s.vol);
+    l3 = bary(s.a3, s.b3, s.g3, s.d3, x, y, z, s.vol);
+    const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
+                     (l3 < 0 || l3 > 1.0);
+    if (!out) return kInside;
+    int mi = 0;
+    double ml = l0;
+    if (l1 < ml) { ml = l1; mi = 1; }
+    if (l2 < ml) { ml = l2; mi = 2; }
+    if (l3 < ml) { ml = l3; mi = 3; }
+    const int32_t nb = mi == 0 ? s.n0 : mi == 1 ? s.n1 : mi == 2 ? s.n2 : s.n3;
+    if (nb >= 0) {  // cap >= 1, so the first move is never capped
+        st.moves = 1;
+        c = nb;
+        return walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+    }
+    if (WALL == 0) {
+        st.moves = 1;
+        return kLost;
+    }
+    // reflecting wall: let walk() redo this tet (it counts and performs the reflection)
+    return walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+}
+
 }  // namespace mpd

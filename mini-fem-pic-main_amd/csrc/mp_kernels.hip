@@ -58,6 +58,8 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
         if (c >= 0) {
             act = true;
             const DevTet* t = tets + c;
+            StartTet g0;
+            load_start(t, g0);
             const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
             double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
             const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
@@ -67,7 +69,7 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             double x = x0 + vx * dt;
             double y = y0 + vy * dt;
             double z = z0 + vz * dt;
-            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             crossed = (st.moves - st.refl) > 0;
             st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
             st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
@@ -146,6 +148,125 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
+// Persistent, software-pipelined push (atomic scatter).  Each workgroup walks one contiguous
+// XCD-local range of the particle array 256 particles at a time; the cell and state of the
+// next chunk are loaded while the current chunk is pushed and walked, so the cell -> tet
+// gather chain of one chunk overlaps the streaming loads of the next.  The LDS hash keeps
+// accumulating across chunks (2048 slots) and is flushed to HBM with global atomics only
+// when more than 1024 slots are occupied, and once at the end: a chunk adds at most 1024
+// keys, so an insert always finds a slot.
+constexpr int kPHash = 2048;
+constexpr int kPFlushAt = kPHash - 4 * 256;
+
+__device__ __forceinline__ void lds_insert_occ(int32_t* hk, double* hv, int32_t* occ, int32_t node, double w) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 11);
+    for (;;) {
+        const int32_t prev = atomicCAS(&hk[h], -1, node);
+        if (prev == -1 || prev == node) {
+            if (prev == -1) atomicAdd(occ, 1);
+            atomicAdd(&hv[h], w);
+            return;
+        }
+        h = (h + 1) & (kPHash - 1);
+    }
+}
+
+template <int WALL>
+__global__ void __launch_bounds__(256)
+k_push_persist(const DevTet* __restrict__ tets, Soa s, int64_t n, int64_t chunk, double qm, double dt,
+               double* __restrict__ den, int32_t* __restrict__ part, int cap) {
+    __shared__ int32_t hk[kPHash];
+    __shared__ double hv[kPHash];
+    __shared__ int32_t occ;
+    for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
+        hk[q] = -1;
+        hv[q] = 0.0;
+    }
+    if (threadIdx.x == 0) occ = 0;
+    const int64_t beg = xcd_block() * chunk;
+    const int64_t end = beg + chunk < n ? beg + chunk : n;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    // prefetch of the first chunk
+    int32_t cn = -1;
+    double xn = 0, yn = 0, zn = 0, vxn = 0, vyn = 0, vzn = 0;
+    {
+        const int64_t j = beg + threadIdx.x;
+        if (j < end) {
+            cn = s.cell[j];
+            xn = ld(s.x + j); yn = ld(s.y + j); zn = ld(s.z + j);
+            vxn = ld(s.vx + j); vyn = ld(s.vy + j); vzn = ld(s.vz + j);
+        }
+    }
+    __syncthreads();
+    for (int64_t base = beg; base < end; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        int32_t c = cn;
+        const double x0 = xn, y0 = yn, z0 = zn;
+        double vx = vxn, vy = vyn, vz = vzn;
+        {
+            const int64_t j = i + blockDim.x;
+            cn = -1;
+            if (j < end) {
+                cn = s.cell[j];
+                xn = ld(s.x + j); yn = ld(s.y + j); zn = ld(s.z + j);
+                vxn = ld(s.vx + j); vyn = ld(s.vy + j); vzn = ld(s.vz + j);
+            }
+        }
+        if (i < end && c >= 0) {
+            n_act++;
+            const DevTet* t = tets + c;
+            vx = vx + (qm * t->ef[0]) * dt;
+            vy = vy + (qm * t->ef[1]) * dt;
+            vz = vz + (qm * t->ef[2]) * dt;
+            double x = x0 + vx * dt;
+            double y = y0 + vy * dt;
+            double z = z0 + vz * dt;
+            double l0, l1, l2, l3;
+            WalkStats st{0, 0};
+            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            n_moves += st.moves;
+            n_refl += st.refl;
+            n_cross += (st.moves - st.refl) > 0;
+            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+            if (r == kInside) {
+                s.cell[i] = c;
+                const DevTet* tf = tets + c;
+                const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
+                lds_insert_occ(hk, hv, &occ, n0, l0);
+                lds_insert_occ(hk, hv, &occ, n1, l1);
+                lds_insert_occ(hk, hv, &occ, n2, l2);
+                lds_insert_occ(hk, hv, &occ, n3, l3);
+            } else {
+                s.cell[i] = -1;
+                n_lost++;
+                n_cap += r == kCapped;
+            }
+        }
+        __syncthreads();
+        const int32_t occupied = occ;  // every thread reads it before any next-chunk insert
+        __syncthreads();
+        if (occupied > kPFlushAt) {  // block-uniform
+            for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
+                const int32_t node = hk[q];
+                if (node >= 0) {
+                    unsafeAtomicAdd(den + node, hv[q]);
+                    hk[q] = -1;
+                    hv[q] = 0.0;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) occ = 0;
+            __syncthreads();
+        }
+    }
+    for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
+        const int32_t node = hk[q];
+        if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
+    }
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
 __global__ void k_normalise(double* __restrict__ raw, double* __restrict__ out,
                             const double* __restrict__ nvol, int32_t nn, double spwt) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,57 +333,74 @@ k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64
 
 // ---- counting sort by cell bucket (cell >> shift) ---------------------------------------
 
-// Groups the wave's valid lanes by bucket key: for every lane, the lowest lane of its group
-// (leader), its rank among the group's lanes and the group size.  The loop runs once per
-// distinct key in the wave (wave-uniform).
-__device__ __forceinline__ void key_groups(int32_t key, bool valid, int& leader, int& rank, int& count) {
-    const int lane = lane_id();
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    unsigned long long pending = __ballot(valid);
-    leader = lane;
-    rank = 0;
-    count = 0;
-    while (pending) {
-        const int l = __ffsll((long long)pending) - 1;
-        const int32_t k = __shfl(key, l, kWave);
-        const bool mine = valid && key == k;
-        const unsigned long long m = __ballot(mine);
-        if (mine) {
-            leader = l;
-            rank = __popcll(m & lt);
-            count = __popcll(m);
-        }
-        pending &= ~m;
+// Workgroup-level grouping of bucket keys: every thread inserts its key into an LDS hash
+// (kSortHash slots, >= 2 x workgroup size) and takes a rank within its key; the workgroup's
+// particles of one bucket then occupy one contiguous range claimed with a single global
+// atomic, and are written by one workgroup within a few microseconds, so L2 merges the
+// partial lines instead of writing each 8 B particle field as its own DRAM burst.
+constexpr int kSortThreads = 1024;
+constexpr int kSortHash = 2048;
+
+struct SortSlots {
+    int32_t key[kSortHash];
+    int32_t cnt[kSortHash];
+    int32_t base[kSortHash];
+};
+
+__device__ __forceinline__ int sort_slot(SortSlots& t, int32_t key) {
+    uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - 11);
+    for (;;) {
+        const int32_t prev = atomicCAS(&t.key[h], -1, key);
+        if (prev == -1 || prev == key) return (int)h;
+        h = (h + 1) & (kSortHash - 1);
     }
 }
 
-__global__ void __launch_bounds__(256)
-k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift,
-            int32_t nb, int32_t* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t c = (i < n) ? cell[i] : -1;
-    const bool valid = c >= 0;
-    const int32_t key = valid ? (run ? (int32_t)run[i] * nb : 0) + (c >> shift) : -1;
-    int leader, rank, count;
-    key_groups(key, valid, leader, rank, count);
-    if (valid && leader == lane_id()) atomicAdd(cnt + key, count);
+__device__ __forceinline__ int32_t bucket_key(int32_t c, const uint16_t* __restrict__ run, int64_t i, int shift,
+                                              int32_t nb) {
+    return (run ? (int32_t)run[i] * nb : 0) + (c >> shift);
 }
 
-// Moves every particle still in the domain to its bucket's range; lanes of one bucket in a
-// wave take consecutive slots (one returning atomic per bucket per wave).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSortThreads)
+k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift,
+            int32_t nb, int32_t* __restrict__ cnt) {
+    __shared__ SortSlots t;
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x) {
+        t.key[q] = -1;
+        t.cnt[q] = 0;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = (i < n) ? cell[i] : -1;
+    if (c >= 0) atomicAdd(&t.cnt[sort_slot(t, bucket_key(c, run, i, shift, nb))], 1);
+    __syncthreads();
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x)
+        if (t.key[q] >= 0) atomicAdd(cnt + t.key[q], t.cnt[q]);
+}
+
+// Moves every particle still in the domain into its bucket's range (cursor = bucket offsets,
+// advanced atomically, one returning atomic per bucket per workgroup).
+__global__ void __launch_bounds__(kSortThreads)
 k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ cursor, Soa dst) {
+    __shared__ SortSlots t;
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x) {
+        t.key[q] = -1;
+        t.cnt[q] = 0;
+    }
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t c = (i < n) ? src.cell[i] : -1;
-    const bool valid = c >= 0;
-    const int32_t key = valid ? (src.run ? (int32_t)src.run[i] * nb : 0) + (c >> shift) : -1;
-    int leader, rank, count;
-    key_groups(key, valid, leader, rank, count);
-    int32_t base = 0;
-    if (valid && leader == lane_id()) base = atomicAdd(cursor + key, count);
-    base = __shfl(base, leader, kWave);
-    if (!valid) return;
-    const int64_t d = (int64_t)base + rank;
+    int slot = -1, rank = 0;
+    if (c >= 0) {
+        slot = sort_slot(t, bucket_key(c, src.run, i, shift, nb));
+        rank = atomicAdd(&t.cnt[slot], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x)
+        if (t.key[q] >= 0) t.base[q] = atomicAdd(cursor + t.key[q], t.cnt[q]);
+    __syncthreads();
+    if (c < 0) return;
+    const int64_t d = (int64_t)t.base[slot] + rank;
     dst.x[d] = src.x[i]; dst.y[d] = src.y[i]; dst.z[d] = src.z[i];
     dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
     dst.cell[d] = c;
@@ -377,8 +515,39 @@ __global__ void k_pack(Soa s, int64_t n, double* __restrict__ pos, double* __res
 
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Persistent grid: as many 256-thread workgroups as fit on the device at once (multiple of 8
+// for the XCD mapping), each owning a contiguous range of ceil(n / grid) particles.
+static hipError_t launch_push_persist(const PushArgs& a, hipStream_t st) {
+    static int per_cu = 0, cus = 0;
+    if (per_cu == 0) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = a.wall == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_persist<0>, 256, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_persist<1>, 256, 0);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) per_cu = 1;
+    }
+    int64_t g = (int64_t)per_cu * cus;
+    g = std::min<int64_t>(g, (a.n + 255) / 256);
+    g = (g + kXcds - 1) / kXcds * kXcds;
+    int64_t chunk = (a.n + g - 1) / g;
+    chunk = (chunk + 255) / 256 * 256;
+    if (a.wall == 0)
+        hipLaunchKernelGGL((k_push_persist<0>), dim3((unsigned)g), dim3(256), 0, st, a.tets, a.soa, a.n, chunk, a.qm,
+                           a.dt, a.den, a.part, a.cap);
+    else
+        hipLaunchKernelGGL((k_push_persist<1>), dim3((unsigned)g), dim3(256), 0, st, a.tets, a.soa, a.n, chunk, a.qm,
+                           a.dt, a.den, a.part, a.cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
+    if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
     const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                          \
@@ -488,14 +657,16 @@ hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t*
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
                             int32_t* cnt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, run, n, shift, nb, cnt);
+    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, kSortThreads)), dim3(kSortThreads), 0, st, cell, run, n, shift,
+                       nb, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
                                const Soa& dst, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, shift, nb, cursor, dst);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, kSortThreads)), dim3(kSortThreads), 0, st, src, n, shift,
+                       nb, cursor, dst);
     return hipGetLastError();
 }
 

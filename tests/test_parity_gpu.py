@@ -217,7 +217,7 @@ def test_cpp_dropin_program(mp):
     assert r.stdout.count(" ok") == 2
 
 
-@pytest.mark.parametrize("agg", [0, 1, 2], ids=["plain", "wave-by-tet", "lds-by-node"])
+@pytest.mark.parametrize("agg", [0, 1, 2, 3], ids=["plain", "wave-by-tet", "lds-by-node", "persistent"])
 def test_atomic_aggregation_modes(mp, orc, agg):
     """Every atomic-scatter aggregation gives the oracle's density to 1e-12 relative."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=13, n=80000)
