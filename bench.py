@@ -143,6 +143,63 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
                         "one hipGraph launch per step vs 64 per-run launch sequences"}
 
 
+def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
+    """Contention stress test: a 100^3 six-tet box (side 1 m, 6,000,000 tets, 1,030,301 nodes),
+    n particles from a 3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2, seeded centres in
+    [0.3, 0.7]^3) so a few thousand nodes take most of the scatter, Maxwellian velocities
+    (sigma 3000 m/s, no drift), frozen sinusoidal field, reflecting walls, cell sort every 8
+    steps.  The same initial state is run in deterministic mode (the configuration's mode) and
+    in atomic mode; the difference of the two step times is the cost of the sorted reduce."""
+    t = time.perf_counter()
+    mesh = mp.Mesh.box_mesh(100, 1.0)
+    ef = mesh.field_sinusoid(AMP)
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.3, 0.7, (3, 3))
+    pos, vel = mp.synth_mixture(seed, n, [0, 0, 0], [1, 1, 1], [0.5, 0.3, 0.2], centres, 0.05, 3000.0,
+                                [0, 0, 0])
+    cell = mesh.locate_box(pos)
+    log(f"contention setup {time.perf_counter() - t:.1f}s: {mesh.n_tets} tets, {n} particles")
+    dt = 1.7e-7
+    res = {}
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=8, wall_mode=mp.WALL_REFLECT, sort_shift=shift)
+        mv.upload_field(ef)
+        for name, mode in (("deterministic", mp.SCATTER_DETERMINISTIC), ("atomic", mp.SCATTER_ATOMIC)):
+            mv.set_particles(pos, vel, cell)
+            for _ in range(warmup):
+                mv.step(QM, dt, SPWT, mode)
+            mv.sync()
+            mv.reset_counters()
+            mv.set_timing(True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                mv.step(QM, dt, SPWT, mode)
+            mv.sync()
+            wall = time.perf_counter() - t0
+            push_ms, launches = mv.timing()
+            mv.set_timing(False)
+            c = mv.counters()
+            # push-kernel algorithmic bytes; the deterministic push also writes 4 (u32 node,
+            # f64 weight) records per particle
+            per_part = 104.0 + (48.0 if mode == mp.SCATTER_DETERMINISTIC else 0.0)
+            avg_s = push_ms / 1e3 / max(launches, 1)
+            bpl = (per_part * c["particle_steps"] + launches * (192.0 * mesh.n_tets + 8.0 * mesh.n_nodes)) / max(launches, 1)
+            res[name] = {"value": c["particle_steps"] / wall, "ms_per_step": 1e3 * wall / steps,
+                         "push_ms_per_launch": avg_s * 1e3, "push_gbs": bpl / avg_s / 1e9,
+                         "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
+                         "reflections_per_step": c["reflections"] / steps}
+            log(f"contention {name}: {res[name]['value'] / 1e9:.3f} G particle-steps/s, "
+                f"{res[name]['ms_per_step']:.2f} ms/step, push {avg_s * 1e3:.3f} ms ({res[name]['push_gbs']:.0f} GB/s)")
+    del pos, vel, cell
+    d, a = res["deterministic"], res["atomic"]
+    return {"value": d["value"], "unit": "particle-steps/s", "mode": "deterministic", "steps": steps,
+            "ms_per_step": d["ms_per_step"], "push_gbs": d["push_gbs"], "atomic": a,
+            "deterministic": d, "sorted_reduce_ms_per_step": d["ms_per_step"] - a["ms_per_step"],
+            "workload": f"100^3 six-tet box (6,000,000 tets, 1,030,301 nodes), {n} particles from a "
+                        "3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2), Maxwellian sigma 3000 m/s, "
+                        f"no drift, reflecting walls, sort every 8 steps (shift {shift}), dt {dt:g} s"}
+
+
 def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
     """Adjoint: 2M particles on a 30^3 six-tet box (side 1 m), frozen sinusoidal field, K = 50
     steps, checkpoint every 10; L = |den_K - den_target|^2 with the target from a forward run
@@ -216,6 +273,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
     ap.add_argument("--block", type=int, default=256, help="push workgroup size (64, 128, 256)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
+    ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
+    ap.add_argument("--contention-particles", type=int, default=200_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
@@ -247,6 +306,8 @@ def main():
         extra["batched"] = run_ensemble(args.ensemble_steps, 3)
     if args.adjoint_steps > 0:
         extra["adjoint"] = run_adjoint(args.adjoint_steps)
+    if args.contention_steps > 0:
+        extra["contention"] = run_contention(args.contention_steps, args.contention_particles)
     cpu = None
     if args.cpu_seconds > 0:
         cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)

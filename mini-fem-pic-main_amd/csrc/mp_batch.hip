@@ -139,7 +139,7 @@ struct mp_batch {
     int64_t rec_cap = 0;
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    int32_t* seg_start = nullptr;
+    int32_t* seg_bounds = nullptr;
     // graphs (one per buffer parity)
     bool use_graph = true;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -297,7 +297,7 @@ int det_step(mp_batch* b) {
         if (hipMalloc(&b->sort_tmp, need) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "mp_batch: sort scratch failed");
         b->sort_tmp_bytes = need;
     }
-    if (!b->seg_start && dalloc(&b->seg_start, keys_n) != hipSuccess)
+    if (!b->seg_bounds && dalloc(&b->seg_bounds, 3 * keys_n + 1) != hipSuccess)
         return set_err(c, MP_ERR_ALLOC, "mp_batch: segment table allocation failed");
     HIP_TRY(c, mpd::launch_fill_u32(b->keys[0], m, 0xFFFFFFFFu, c->stream));
     if (b->n_slots > 0) {
@@ -313,7 +313,7 @@ int det_step(mp_batch* b) {
     }
     HIP_TRY(c, mpd::radix_sort_pairs(b->sort_tmp, b->sort_tmp_bytes, b->keys[0], b->keys[1], b->vals[0], b->vals[1], m,
                                      end_bit, c->stream));
-    HIP_TRY(c, mpd::launch_det_fold(b->keys[1], b->vals[1], m, b->seg_start, c->node_vol, (int32_t)keys_n, 0.0,
+    HIP_TRY(c, mpd::launch_det_fold(b->keys[1], b->vals[1], m, b->seg_bounds, c->node_vol, (int32_t)keys_n, 0.0,
                                     b->den_out, c->stream, b->par_d + 2, b->nn));
     return MP_OK;
 }
@@ -374,7 +374,7 @@ void mp_batch_destroy(mp_batch* b) {
     drop_graphs(b);
     void* bufs[] = {b->soa_mem[0], b->soa_mem[1], b->run_mem[0], b->run_mem[1], b->n_dev, b->par_d, b->den_raw,
                     b->den_out, b->ctr, b->tally, b->cnt, b->cursor, b->runs_d, b->keys[0], b->keys[1], b->vals[0],
-                    b->vals[1], b->sort_tmp, b->seg_start};
+                    b->vals[1], b->sort_tmp, b->seg_bounds};
     for (void* p : bufs) dfree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);
     delete b;

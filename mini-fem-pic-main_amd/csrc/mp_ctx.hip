@@ -180,7 +180,7 @@ int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
             return set_err(c, MP_ERR_ALLOC, "radix sort scratch allocation failed");
         c->sort_tmp_bytes = need;
     }
-    if (!c->seg_start && dalloc(&c->seg_start, c->n_nodes) != hipSuccess)
+    if (!c->seg_bounds && dalloc(&c->seg_bounds, 3 * (int64_t)c->n_nodes + 1) != hipSuccess)
         return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
     return MP_OK;
 }
@@ -191,7 +191,7 @@ int det_finish(mp_ctx* c, int64_t m, double spwt) {
     const int end_bit = bit_width((uint64_t)c->n_nodes);
     HIP_TRY(c, mpd::radix_sort_pairs(c->sort_tmp, c->sort_tmp_bytes, c->keys[0], c->keys[1], c->vals[0],
                                      c->vals[1], m, end_bit, c->stream));
-    HIP_TRY(c, mpd::launch_det_fold(c->keys[1], c->vals[1], m, c->seg_start, c->node_vol, c->n_nodes, spwt,
+    HIP_TRY(c, mpd::launch_det_fold(c->keys[1], c->vals[1], m, c->seg_bounds, c->node_vol, c->n_nodes, spwt,
                                     c->den_out, c->stream));
     return MP_OK;
 }
@@ -304,7 +304,7 @@ void mp_destroy(mp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
-                    c->sort_tmp, c->seg_start, c->aos, c->aos_out, c->flags, c->offs, c->tally};
+                    c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

@@ -39,7 +39,7 @@ struct mp_ctx {
     int64_t rec_cap = 0;
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    int32_t* seg_start = nullptr;
+    int32_t* seg_bounds = nullptr;
     // drop-in AoS scratch
     mp_particle* aos = nullptr;
     mp_particle* aos_out = nullptr;

@@ -4,7 +4,7 @@
 //                  (src/mini-pic.cpp:287-308), atomic or deterministic-record scatter
 //   k_push_aos     the same over the reference's 88 B Particle records (drop-in path)
 //   k_normalise    den = raw * (spwt / node volume), raw cleared (src/mini-pic.cpp:319)
-//   k_seg_start / k_fold   deterministic scatter: after sorting (node,pid) records, a
+//   k_seg_bounds / k_fold  deterministic scatter: after sorting (node,pid) records, a
 //                  serial fold per node in ascending pid, then the normalisation
 //   k_cell_hist / k_cell_scatter   counting sort of resident particles by cell >> shift
 //   k_scan_*       exclusive prefix sums (counting sort offsets, stable compaction)
@@ -285,14 +285,16 @@ __global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
     if (i < n) p[i] = v;
 }
 
-// First sorted record of each node (keys are node ids; sentinels sort last).
-__global__ void k_seg_start(const uint32_t* __restrict__ keys, int64_t m, int32_t nn,
-                            int32_t* __restrict__ start) {
+// Record range [start[node], end[node]) of each node in the sorted records (keys are node
+// ids; sentinels sort last).  start is pre-filled with -1: a node with no record keeps it.
+__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int32_t nn,
+                             int32_t* __restrict__ start, int32_t* __restrict__ end) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     const uint32_t node = keys[j];
     if (node >= (uint32_t)nn) return;
     if (j == 0 || keys[j - 1] != node) start[node] = (int32_t)j;
+    if (j == m - 1 || keys[j + 1] != node) end[node] = (int32_t)(j + 1);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int q) {
@@ -302,32 +304,95 @@ __device__ __forceinline__ double readlane_f64(double v, int q) {
 }
 
 // Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial MoveParticles
-// adds into den; then den *= spwt / node volume.  One wave per node: the wave loads 64
-// consecutive sorted records at a time (coalesced); the node's records are a prefix of each
-// chunk and are folded in order through lane broadcasts.
+// adds into den; then den *= spwt / node volume.  The fold is a dependent chain per node, so
+// the work is laid out to run many chains at once:
+//  - k_fold: one lane per node; each lane folds its own segment with kFoldUnroll independent
+//    loads in flight per round (a node of a 60^3 box at 20M particles has ~350 records);
+//  - segments longer than kFoldLong (dense clusters: 10^4..10^5 records on one node) are
+//    appended to a list instead and k_fold_long folds each with a whole wave, one wave per
+//    node: coalesced 64-record loads, four ahead of the chain, broadcast lane by lane.  In
+//    k_fold a lane's long segment would hold its wave for len / kFoldUnroll memory round
+//    trips, and neighbouring hot nodes (consecutive ids) would serialise in one wave.
+constexpr int kFoldUnroll = 16;
+constexpr int kFoldLong = 1024;
+
+__device__ __forceinline__ void fold_out(double s, int64_t n, double spwt, const double* __restrict__ run_spwt,
+                                         int32_t nn_mesh, const double* __restrict__ nvol,
+                                         double* __restrict__ out) {
+    // batch: key n = run * nn_mesh + node, per-run spwt (stride 3 in the run table)
+    const double sp = run_spwt ? run_spwt[3 * (n / nn_mesh)] : spwt;
+    out[n] = s * (sp / nvol[n % nn_mesh]);
+}
+
 __global__ void __launch_bounds__(256)
-k_fold(const uint32_t* __restrict__ keys, const double* __restrict__ vals, int64_t m,
-       const int32_t* __restrict__ start, const double* __restrict__ nvol, int32_t nn, double spwt,
-       const double* __restrict__ run_spwt, int32_t nn_mesh, double* __restrict__ out) {
-    const int64_t n = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    if (n >= nn) return;  // wave-uniform
-    const int lane = lane_id();
-    double s = 0.0;
-    const int64_t j0 = start[n];
-    if (j0 >= 0) {
-        for (int64_t base = j0;; base += kWave) {
-            const int64_t j = base + lane;
-            const bool mine = j < m && keys[j] == (uint32_t)n;
-            const double v = mine ? vals[j] : 0.0;
-            const int cnt = __popcll(__ballot(mine));
-            for (int q = 0; q < cnt; q++) s = s + readlane_f64(v, q);
-            if (cnt < kWave) break;
-        }
+k_fold(const double* __restrict__ vals, const int32_t* __restrict__ start, const int32_t* __restrict__ end,
+       const double* __restrict__ nvol, int32_t nn, double spwt, const double* __restrict__ run_spwt,
+       int32_t nn_mesh, double* __restrict__ out, int32_t* __restrict__ long_list,
+       int32_t* __restrict__ long_count) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = n < nn;
+    const int64_t j0 = valid ? start[n] : -1;
+    const int64_t j1 = j0 >= 0 ? (int64_t)end[n] : j0;
+    const bool longseg = j1 - j0 > kFoldLong;
+    const unsigned long long lm = __ballot(longseg);
+    if (lm) {  // wave-aggregated append of the long segments
+        const int lane = lane_id();
+        int32_t base = 0;
+        const int leader = __ffsll((long long)lm) - 1;
+        if (lane == leader) base = atomicAdd(long_count, __popcll(lm));
+        base = __shfl(base, leader, kWave);
+        if (longseg) long_list[base + __popcll(lm & ((1ull << lane) - 1))] = (int32_t)n;
     }
-    if (lane == 0) {
-        // batch: key n = run * nn_mesh + node, per-run spwt (stride 3 in the run table)
-        const double sp = run_spwt ? run_spwt[3 * (n / nn_mesh)] : spwt;
-        out[n] = s * (sp / nvol[n % nn_mesh]);
+    if (!valid || longseg) return;
+    double s = 0.0;
+    int64_t j = j0;
+    for (; j + kFoldUnroll <= j1; j += kFoldUnroll) {
+        double v[kFoldUnroll];
+#pragma unroll
+        for (int q = 0; q < kFoldUnroll; q++) v[q] = vals[j + q];
+#pragma unroll
+        for (int q = 0; q < kFoldUnroll; q++) s = s + v[q];
+    }
+    for (; j < j1; j++) s = s + vals[j];
+    fold_out(s, n, spwt, run_spwt, nn_mesh, nvol, out);
+}
+
+// One wave per listed node; every wave strides over the list and leaves when it is done.
+__global__ void __launch_bounds__(256)
+k_fold_long(const double* __restrict__ vals, const int32_t* __restrict__ start,
+            const int32_t* __restrict__ end, const double* __restrict__ nvol, double spwt,
+            const double* __restrict__ run_spwt, int32_t nn_mesh, double* __restrict__ out,
+            const int32_t* __restrict__ long_list, const int32_t* __restrict__ long_count) {
+    const int lane = lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int32_t count = *long_count;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < count; i += waves) {
+        const int64_t n = long_list[i];
+        const int64_t b0 = start[n], b1 = end[n];
+        double acc = 0.0;
+        for (int64_t base = b0; base < b1; base += 4 * kWave) {
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t j = base + q * kWave + lane;
+                v[q] = j < b1 ? vals[j] : 0.0;
+            }
+            const int64_t left = b1 - base;
+            if (left >= 4 * kWave) {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int r = 0; r < kWave; r++) acc = acc + readlane_f64(v[q], r);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int64_t rem = left - q * kWave;
+                    const int cnt = rem >= kWave ? kWave : (rem > 0 ? (int)rem : 0);
+                    for (int r = 0; r < cnt; r++) acc = acc + readlane_f64(v[q], r);
+                }
+            }
+        }
+        if (lane == 0) fold_out(acc, n, spwt, run_spwt, nn_mesh, nvol, out);
     }
 }
 
@@ -615,16 +680,25 @@ hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
+hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* bounds,
                            const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
                            const double* run_spwt, int32_t nn_mesh) {
     if (nn_mesh <= 0) nn_mesh = nn;
+    int32_t* start = bounds;
+    int32_t* end = bounds + nn;
+    int32_t* long_list = bounds + 2 * (int64_t)nn;
+    int32_t* long_count = bounds + 3 * (int64_t)nn;
     hipError_t e = launch_fill_i32(start, nn, -1, st);
+    if (e == hipSuccess) e = launch_fill_i32(long_count, 1, 0, st);
     if (e != hipSuccess) return e;
     if (m > 0)
-        hipLaunchKernelGGL(k_seg_start, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start);
-    hipLaunchKernelGGL(k_fold, dim3(grid_for((int64_t)nn * kWave, 256)), dim3(256), 0, st, keys, vals, m, start,
-                       nvol, nn, spwt, run_spwt, nn_mesh, out);
+        hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start, end);
+    hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, vals, start, end, nvol, nn, spwt,
+                       run_spwt, nn_mesh, out, long_list, long_count);
+    // long segments exist only where a few hundred nodes hold most records: 4 waves per CU
+    // of 256 CUs is enough to fold them side by side
+    hipLaunchKernelGGL(k_fold_long, dim3(256), dim3(256), 0, st, vals, start, end, nvol, spwt, run_spwt,
+                       nn_mesh, out, long_list, long_count);
     return hipGetLastError();
 }
 

@@ -49,7 +49,8 @@ hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
 // Deterministic fold over nn keys; with run_spwt (batch) key k = run * nn_mesh + node and the
 // run's spwt is run_spwt[3 * run].
-hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* start,
+// bounds: 3 * nn + 1 ints of scratch (segment start and end per node, long-segment list, count).
+hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, int32_t* bounds,
                            const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
                            const double* run_spwt = nullptr, int32_t nn_mesh = 0);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);

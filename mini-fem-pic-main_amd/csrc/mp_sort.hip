@@ -1,6 +1,11 @@
 // Stable LSD radix sort of the deterministic-scatter records (key = node id, value =
-// barycentric weight, input in pid order) on the node bits only, via rocPRIM's device radix
-// sort.  Stability keeps each node's weights in ascending pid.
+// barycentric weight, input in pid order) on the node bits only, via rocPRIM's onesweep
+// device radix sort.  Stability keeps each node's weights in ascending pid.
+//
+// The digit width is chosen from the key width so that node ids up to 22 bits (4M nodes) sort
+// in two passes over the records: 18-bit keys (a 60^3 box) take 2 passes of 9 bits instead
+// of the 3 passes of 8 bits rocPRIM's default width needs.  Each pass reads and writes 12 B per
+// record, so a pass saved is a third of the sort's HBM traffic.
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -9,10 +14,45 @@
 
 namespace mpd {
 
+namespace {
+
+template <unsigned Bits>
+using SortCfg = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<512, 8>, rocprim::kernel_config<512, 8>,
+                                        Bits, rocprim::block_radix_rank_algorithm::match>>;
+
+// Digit width for an end_bit-wide key: the fewest passes of at most 11 bits, then the
+// narrowest width (>= 8) that keeps that pass count.
+unsigned digit_bits(int end_bit) {
+    if (end_bit <= 8) return 8;
+    const int passes = (end_bit + 10) / 11;
+    const int bits = (end_bit + passes - 1) / passes;
+    return bits < 8 ? 8u : (unsigned)bits;
+}
+
+template <unsigned Bits>
+hipError_t sort_bits(void* temp, size_t& bytes, const uint32_t* ki, uint32_t* ko, const double* vi,
+                     double* vo, size_t n, int end_bit, hipStream_t st) {
+    return rocprim::radix_sort_pairs<SortCfg<Bits>>(temp, bytes, ki, ko, vi, vo, n, 0, (unsigned)end_bit,
+                                                    st);
+}
+
+hipError_t sort_any(void* temp, size_t& bytes, const uint32_t* ki, uint32_t* ko, const double* vi,
+                    double* vo, size_t n, int end_bit, hipStream_t st) {
+    switch (digit_bits(end_bit)) {
+        case 9: return sort_bits<9>(temp, bytes, ki, ko, vi, vo, n, end_bit, st);
+        case 10: return sort_bits<10>(temp, bytes, ki, ko, vi, vo, n, end_bit, st);
+        case 11: return sort_bits<11>(temp, bytes, ki, ko, vi, vo, n, end_bit, st);
+        default: return sort_bits<8>(temp, bytes, ki, ko, vi, vo, n, end_bit, st);
+    }
+}
+
+}  // namespace
+
 size_t radix_sort_temp_bytes(int64_t n, int end_bit) {
     size_t bytes = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                              (const double*)nullptr, (double*)nullptr, (size_t)n, 0, end_bit);
+    (void)sort_any(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, (size_t)n, end_bit, nullptr);
     return bytes;
 }
 
@@ -21,8 +61,7 @@ hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const uint32_t* keys_
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
     size_t bytes = temp_bytes;
-    return rocprim::radix_sort_pairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0,
-                                     end_bit, st);
+    return sort_any(temp, bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, end_bit, st);
 }
 
 }  // namespace mpd

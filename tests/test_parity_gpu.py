@@ -255,6 +255,36 @@ def test_sort_shift_matches_oracle(mp, orc, shift):
         assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "deterministic"])
+def test_clustered_reflecting_matches_oracle(mp, orc, mode):
+    """The contention workload at small size: a 3-Gaussian mixture packs tens of thousands of
+    scatter records onto single nodes (the deterministic fold's whole-wave path for segments
+    longer than 1024 records), reflecting walls, sort every 3 steps."""
+    side = 0.1
+    mesh = mp.Mesh.box_mesh(10, side)
+    ef = mesh.field_sinusoid(10.0)
+    centres = np.array([[0.006, 0.05, 0.05], [0.07, 0.04, 0.06], [0.05, 0.07, 0.03]])  # one at a wall
+    pos, vel = mp.synth_mixture(31, 200000, [0, 0, 0], [side] * 3, [0.5, 0.3, 0.2], centres, 0.003, 3000.0,
+                                [0, 0, 0])
+    cell = mesh.locate_box(pos)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=1)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=3, wall_mode=1)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(6):
+            mv.step(QM, dt, SPWT, mode)
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density(), ref, mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+    per_node = np.bincount(mesh.tets["con"][run.cell].ravel(), minlength=mesh.n_nodes)
+    assert per_node.max() > 16 * 1024 and (per_node > 1024).sum() > 50, "the case must put long segments on nodes"
+    assert run.stats["reflections"] > 10000
+
+
 def _ensemble(mp, n_runs=5, seed=21, n_hex=6, side=0.1):
     mesh = mp.Mesh.box_mesh(n_hex, side)
     ef = mesh.field_sinusoid(10.0)
