@@ -8,7 +8,7 @@ walls, cell sort every 10 steps.  Particles stay resident in HBM between steps.
 
 Prints ONE JSON line.  value = particle-steps/s over the timed steps (whole step: push,
 walk, scatter, normalisation, periodic sort).  roofline = the push kernel's algorithmic
-HBM bytes per launch (104 B per active particle + 192 B per tet + 8 B per node) over its
+HBM bytes per launch (104 B per active particle + 192 B per tet + 16 B per node) over its
 average launch time from HIP events on the context stream.  cpu_baseline = the oracle's
 OpenMP port of the reference kernel body on the host cores, on a bounded sample.
 """
@@ -66,16 +66,26 @@ def measured_traffic():
 
 
 def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
-    return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 8.0 * n_nodes)
+    return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 16.0 * n_nodes)
 
 
-def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256):
+def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
+            tpp=1, tune=False):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
+    tuned = None
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block)
+        mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block,
+                       particles_per_thread=ppt, threads_per_particle=tpp)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
+        if tune:
+            # untimed: the autotuner restores the particle state it started from
+            t = time.perf_counter()
+            best, best_ms, trials = mv.autotune(QM, DT, SPWT, mode, min_steps=10)
+            tuned = {"options": best, "ms_per_step": best_ms, "trials": len(trials),
+                     "tuner_s": time.perf_counter() - t}
+            log(f"autotune ({len(trials)} trials, {tuned['tuner_s']:.1f} s): {best} at {best_ms:.3f} ms/step")
         for _ in range(warmup):
             mv.step(QM, DT, SPWT, mode)
         mv.sync()
@@ -89,9 +99,10 @@ def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2,
         push_ms, launches = mv.timing()
         mv.set_timing(False)
         c = mv.counters()
+        opts = mv.options
     wall = t1 - t0
     return {"wall_s": wall, "particle_steps": c["particle_steps"], "rate": c["particle_steps"] / wall,
-            "push_ms": push_ms, "launches": launches, "counters": c}
+            "push_ms": push_ms, "launches": launches, "counters": c, "options": opts, "tuned": tuned}
 
 
 def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
@@ -183,7 +194,7 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
             # f64 weight) records per particle
             per_part = 104.0 + (48.0 if mode == mp.SCATTER_DETERMINISTIC else 0.0)
             avg_s = push_ms / 1e3 / max(launches, 1)
-            bpl = (per_part * c["particle_steps"] + launches * (192.0 * mesh.n_tets + 8.0 * mesh.n_nodes)) / max(launches, 1)
+            bpl = (per_part * c["particle_steps"] + launches * (192.0 * mesh.n_tets + 16.0 * mesh.n_nodes)) / max(launches, 1)
             res[name] = {"value": c["particle_steps"] / wall, "ms_per_step": 1e3 * wall / steps,
                          "push_ms_per_launch": avg_s * 1e3, "push_gbs": bpl / avg_s / 1e9,
                          "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
@@ -272,6 +283,9 @@ def main():
     ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
     ap.add_argument("--block", type=int, default=256, help="push workgroup size (64, 128, 256)")
+    ap.add_argument("--ppt", type=int, default=1, help="particles per thread (1, 2, 4)")
+    ap.add_argument("--tpp", type=int, default=1, help="threads per particle (1, 4)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use the options given)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
@@ -291,16 +305,17 @@ def main():
 
     mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
     r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg,
-                args.sort_shift, args.block)
+                args.sort_shift, args.block, args.ppt, args.tpp, tune=not args.no_tune)
     log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
     achieved = bytes_per_launch / avg_push_s / 1e9
     extra = {}
     if args.det_steps > 0 and mode == mp.SCATTER_ATOMIC:
-        d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval)
+        d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval,
+                    shift=args.sort_shift, tune=not args.no_tune)
         extra["deterministic"] = {"value": d["rate"], "ms_per_step": 1e3 * d["wall_s"] / args.det_steps,
-                                  "steps": args.det_steps}
+                                  "steps": args.det_steps, "options": d["options"]}
         log(f"gpu deterministic: {d['rate'] / 1e9:.3f} G particle-steps/s")
     if args.ensemble_steps > 0:
         extra["batched"] = run_ensemble(args.ensemble_steps, 3)
@@ -333,8 +348,7 @@ def main():
         "config": {"workload": "push-only scale test: 60^3 six-tet box, side 1 m (1,296,000 tets, 226,981 nodes), "
                                "20M FP64 particles, frozen sinusoidal field, absorbing walls",
                    "particles": n, "tets": mesh.n_tets, "nodes": mesh.n_nodes, "dt": DT,
-                   "scatter": args.mode, "sort_interval": args.sort_interval, "scatter_agg": args.agg,
-                   "sort_shift": args.sort_shift,
+                   "scatter": args.mode, "options": r["options"], "autotune": r["tuned"],
                    "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
                    "lost": c["lost"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -67,11 +67,12 @@ typedef struct mp_options {
     int32_t wall_mode;             /* MP_WALL_ABSORB / MP_WALL_REFLECT                 */
     int32_t block_size;            /* threads per workgroup of the push kernel         */
     int32_t threads_per_particle;  /* 1 or 4 (4: one lane per barycentric coordinate)  */
-    int32_t particles_per_thread;  /* 1, 2 or 4 (TPP=1 only)                           */
+    int32_t particles_per_thread;  /* 1, 2 or 4: rounds of block_size particles per thread */
     int32_t walk_cap;              /* max face moves per particle per step             */
     int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
                                       2 per workgroup by node (LDS hash), 3 persistent
-                                      pipelined workgroups, LDS hash kept across chunks */
+                                      pipelined workgroups, LDS hash kept across chunks,
+                                      4 per wave by node (LDS hash, no workgroup barrier) */
     int32_t sort_shift;            /* cell sort key = cell >> sort_shift (0: exact cell) */
 } mp_options;
 
@@ -146,6 +147,26 @@ void* mp_stream(mp_ctx* ctx);
  * mp_get_timing blocks and returns the summed push-kernel time (ms) and launch count. */
 int  mp_set_timing(mp_ctx* ctx, int32_t enable);
 int  mp_get_timing(mp_ctx* ctx, double* push_ms, int64_t* push_launches);
+
+/* ---- autotuner (no reference counterpart; the SYCL reference has no tunables) ----------- */
+/* One timed candidate: the options and the measured time per step (ms). */
+typedef struct mp_tune_trial {
+    mp_options opt;
+    double     ms_per_step;
+} mp_tune_trial;
+/* Choose threads_per_particle, particles_per_thread, block_size, scatter_agg (atomic mode),
+ * sort_shift and sort_interval for the resident particles and the given step parameters:
+ * coordinate descent over the knobs, each candidate timed (HIP events) over a window of
+ * max(interval, min_steps) steps that starts from the same saved state and holds one cell
+ * sort per sort interval, as a long run does.  Results do not depend on the options, so
+ * only time is compared.  The particle state, counters and density are restored before
+ * returning; the chosen options are set on ctx and written to *best (may be NULL) with their
+ * time per step.  trials[0..max_trials) receives the timed candidates in order (may be NULL
+ * when max_trials is 0); *n_trials their number.  Needs one extra particle-state copy of
+ * device memory while it runs. */
+int  mp_autotune(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode, int32_t min_steps,
+                 mp_options* best, double* best_ms_per_step, mp_tune_trial* trials, int32_t max_trials,
+                 int32_t* n_trials);
 
 /* ---- batched ensemble: many independent runs on one mesh (no reference counterpart) -- */
 /* An ensemble of n_runs independent particle sets pushed through the context's mesh and

@@ -90,6 +90,17 @@ __device__ __forceinline__ void tally(int32_t* __restrict__ part, bool act, bool
     }
 }
 
+// Per-wave step tallies: lane 0 stores kTally int32 to part[global wave * kTally + k]; no
+// workgroup barrier.
+__device__ __forceinline__ void tally_wave(int32_t* __restrict__ part, int a, int l, int x, int m, int k,
+                                           int r) {
+    const int v[kTally] = {wave_sum(a), wave_sum(l), wave_sum(x), wave_sum(m), wave_sum(k), wave_sum(r)};
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < kTally; q++) part[w * kTally + q] = v[q];
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -171,6 +182,64 @@ __device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, dou
         const int32_t node = hk[s];
         if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
     }
+}
+
+// Atomic scatter aggregated per wave by node: like scatter_lds, in a wave-private LDS table of
+// 256 slots (>= the wave's 4 * 64 contributions, so every insert finds a slot), with no
+// workgroup barrier: LDS operations of one wave complete in issue order, and the wave
+// barriers keep the compiler from moving table accesses across the phases.  Every lane of the
+// wave must call it.
+constexpr int kWaveSlots = 256;
+
+__device__ __forceinline__ void lds_insert_wave(int32_t* hk, double* hv, int32_t node, double w) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 8);
+    for (;;) {
+        const int32_t prev = atomicCAS(&hk[h], -1, node);
+        if (prev == -1 || prev == node) {
+            atomicAdd(&hv[h], w);
+            return;
+        }
+        h = (h + 1) & (kWaveSlots - 1);
+    }
+}
+
+__device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                                 bool ok, int32_t fc, double l0, double l1, double l2,
+                                                 double l3) {
+    __shared__ int32_t hk_all[256 / kWave][kWaveSlots];
+    __shared__ double hv_all[256 / kWave][kWaveSlots];
+    const int w = threadIdx.x / kWave;
+    int32_t* hk = hk_all[w];
+    double* hv = hv_all[w];
+    const int lane = lane_id();
+#pragma unroll
+    for (int q = 0; q < kWaveSlots / kWave; q++) {
+        hk[lane + q * kWave] = -1;
+        hv[lane + q * kWave] = 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (ok) {
+        const DevTet* t = tets + fc;
+        const int32_t n0 = t->con[0], n1 = t->con[1], n2 = t->con[2], n3 = t->con[3];
+        lds_insert_wave(hk, hv, n0, l0);
+        lds_insert_wave(hk, hv, n1, l1);
+        lds_insert_wave(hk, hv, n2, l2);
+        lds_insert_wave(hk, hv, n3, l3);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int q = 0; q < kWaveSlots / kWave; q++) {
+        const int32_t node = hk[lane + q * kWave];
+        if (node >= 0) unsafeAtomicAdd(den + node, hv[lane + q * kWave]);
+    }
+    // the next call's initialisation must not overtake this flush's reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the

@@ -223,8 +223,10 @@ int sort_now(mp_ctx* c) {
 bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
-           o.threads_per_particle == 1 && o.particles_per_thread == 1 && o.walk_cap >= 1 &&
-           o.scatter_agg >= 0 && o.scatter_agg <= 3 && o.sort_shift >= 0 && o.sort_shift <= 16;
+           (o.threads_per_particle == 1 || (o.threads_per_particle == 4 && o.particles_per_thread == 1)) &&
+           (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4) &&
+           o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
+           o.sort_shift <= 16;
 }
 
 }  // namespace
@@ -483,6 +485,8 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.det = det;
     a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
+    a.ppt = c->opt.particles_per_thread;
+    a.tpp = c->opt.threads_per_particle;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));

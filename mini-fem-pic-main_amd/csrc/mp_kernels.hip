@@ -39,66 +39,220 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
     }
 }
 
-// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block.
+// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block,
+// 4 = LDS hash per wave (no workgroup barrier anywhere: a wave whose particles walk far does
+// not hold up the other waves of its block).  Each thread takes ppt particles, one per round,
+// a round being blockDim.x consecutive particles; MULTI == false fixes ppt = 1 at compile time
+// (the round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).
 #ifndef MP_PUSH_WAVES
 #define MP_PUSH_WAVES 5
 #endif
-template <int WALL, int DET, int AGG>
+template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-           int32_t* __restrict__ part, int cap) {
-    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+           int32_t* __restrict__ part, int cap, int ppt_arg) {
+    const int ppt = MULTI ? ppt_arg : 1;
+    const int64_t b0 = xcd_block() * ppt;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    for (int k = 0; k < ppt; k++) {
+        const int64_t i = (b0 + k) * blockDim.x + threadIdx.x;
+        bool ok = false;
+        int32_t fc = -1;
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        if (i < n) {
+            int32_t c = s.cell[i];
+            if (c >= 0) {
+                n_act++;
+                const DevTet* t = tets + c;
+                StartTet g0;
+                load_start(t, g0);
+                const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+                double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
+                const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+                vx = vx + (qm * ex) * dt;
+                vy = vy + (qm * ey) * dt;
+                vz = vz + (qm * ez) * dt;
+                double x = x0 + vx * dt;
+                double y = y0 + vy * dt;
+                double z = z0 + vz * dt;
+                WalkStats st{0, 0};
+                const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+                n_cross += (st.moves - st.refl) > 0;
+                n_moves += st.moves;
+                n_refl += st.refl;
+                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+                if (r == kInside) {
+                    s.cell[i] = c;
+                    ok = true;
+                    fc = c;
+                    if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3);
+                } else {
+                    s.cell[i] = -1;
+                    n_lost++;
+                    n_cap += (r == kCapped);
+                }
+            }
+        }
+        if (!DET) {
+            if (AGG == 4) {
+                scatter_lds_wave(tets, den, ok, fc, l0, l1, l2, l3);
+            } else if (AGG == 2) {
+                scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
+            } else if (AGG == 1) {
+                scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
+            } else if (ok) {
+                const DevTet* t = tets + fc;
+                unsafeAtomicAdd(den + t->con[0], l0);
+                unsafeAtomicAdd(den + t->con[1], l1);
+                unsafeAtomicAdd(den + t->con[2], l2);
+                unsafeAtomicAdd(den + t->con[3], l3);
+            }
+        }
+    }
+    if (AGG == 4 && !DET)
+        tally_wave(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+    else
+        tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
+// ---- four threads per particle ------------------------------------------------------------
+// Lane v (0..3) of a quad owns vertex v of the current tet: it gathers that vertex's 32 B row
+// (alpha, beta, gamma, delta) instead of all 128 B, computes lc_v, and the quad exchanges the
+// four weights through DPP quad broadcasts, so every lane takes the same inside/argmin decision
+// on bit-identical values.  The push and the particle state are replicated across the quad;
+// each lane stores a share of the state and scatters its own vertex.  Control flow is uniform
+// within a quad.  AGG: 0 = one atomic per weight, otherwise the per-workgroup LDS hash.
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v) {
+    constexpr int ctrl = K * 0x55;  // quad_perm [K, K, K, K]
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void lds_scatter_one(double* __restrict__ den, bool ok, int32_t node, double w) {
+    __shared__ int32_t hk[kHashSlots];
+    __shared__ double hv[kHashSlots];
+    for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+        hk[q] = -1;
+        hv[q] = 0.0;
+    }
+    __syncthreads();
+    if (ok) lds_insert(hk, hv, node, w);
+    __syncthreads();
+    for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+        const int32_t nd = hk[q];
+        if (nd >= 0) unsafeAtomicAdd(den + nd, hv[q]);
+    }
+}
+
+template <int WALL, int DET, int AGG>
+__global__ void __launch_bounds__(256)
+k_push_tpp4(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
+            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
+            int32_t* __restrict__ part, int cap) {
+    const int sub = threadIdx.x & 3;
+    const int64_t i = (xcd_block() * blockDim.x + threadIdx.x) >> 2;
     bool act = false, lost = false, crossed = false, capped = false, ok = false;
-    int32_t fc = -1;
-    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-    WalkStats st{0, 0};
+    int32_t node = -1;
+    double lw = 0.0;
+    int moves = 0, refl = 0;
     if (i < n) {
         int32_t c = s.cell[i];
         if (c >= 0) {
             act = true;
             const DevTet* t = tets + c;
-            StartTet g0;
-            load_start(t, g0);
+            double a = t->c[sub][0], b = t->c[sub][1], g = t->c[sub][2], d = t->c[sub][3];
+            double vol = t->vol;
             const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
             double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
-            const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+            double x = ld(s.x + i), y = ld(s.y + i), z = ld(s.z + i);
             vx = vx + (qm * ex) * dt;
             vy = vy + (qm * ey) * dt;
             vz = vz + (qm * ez) * dt;
-            double x = x0 + vx * dt;
-            double y = y0 + vy * dt;
-            double z = z0 + vz * dt;
-            const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
-            crossed = (st.moves - st.refl) > 0;
-            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-            if (r == kInside) {
-                s.cell[i] = c;
+            x = x + vx * dt;
+            y = y + vy * dt;
+            z = z + vz * dt;
+            int status;
+            while (true) {
+                const double l = bary(a, b, g, d, x, y, z, vol);
+                const double l0 = quad_bcast<0>(l), l1 = quad_bcast<1>(l), l2 = quad_bcast<2>(l),
+                             l3 = quad_bcast<3>(l);
+                const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
+                                 (l3 < 0 || l3 > 1.0);
+                if (!out) {
+                    lw = l;
+                    status = kInside;
+                    break;
+                }
+                int mi = 0;
+                double ml = l0;
+                if (l1 < ml) { ml = l1; mi = 1; }
+                if (l2 < ml) { ml = l2; mi = 2; }
+                if (l3 < ml) { ml = l3; mi = 3; }
+                const int32_t nb = t->nbr[mi];
+                if (moves >= cap) {
+                    status = kCapped;
+                    break;
+                }
+                moves++;
+                if (nb >= 0) {
+                    c = nb;
+                } else if (WALL == 0) {
+                    status = kLost;
+                    break;
+                } else {
+                    const double gx = -t->c[mi][1], gy = t->c[mi][2], gz = -t->c[mi][3];
+                    const double gg = (gx * gx + gy * gy) + gz * gz;
+                    const double sc = ((2.0 * ml) * (6.0 * vol)) / gg;
+                    x = x - sc * gx;
+                    y = y - sc * gy;
+                    z = z - sc * gz;
+                    const double w = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
+                    vx = vx - w * gx;
+                    vy = vy - w * gy;
+                    vz = vz - w * gz;
+                    refl++;
+                }
+                t = tets + c;
+                a = t->c[sub][0];
+                b = t->c[sub][1];
+                g = t->c[sub][2];
+                d = t->c[sub][3];
+                vol = t->vol;
+            }
+            crossed = (moves - refl) > 0;
+            // each lane stores a share of the particle's state
+            if (sub == 0) { st_(s.x + i, x); st_(s.vx + i, vx); }
+            if (sub == 1) { st_(s.y + i, y); st_(s.vy + i, vy); }
+            if (sub == 2) { st_(s.z + i, z); st_(s.vz + i, vz); }
+            if (status == kInside) {
+                if (sub == 3) s.cell[i] = c;
                 ok = true;
-                fc = c;
-                if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3);
+                node = t->con[sub];
+                if (DET) {
+                    const int64_t r = 4 * (int64_t)s.pid[i] + sub;
+                    keys[r] = (uint32_t)node;
+                    vals[r] = lw;
+                }
             } else {
-                s.cell[i] = -1;
+                if (sub == 3) s.cell[i] = -1;
                 lost = true;
-                capped = (r == kCapped);
+                capped = status == kCapped;
             }
         }
     }
     if (!DET) {
-        if (AGG == 2) {
-            scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
-        } else if (AGG == 1) {
-            scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
-        } else if (ok) {
-            const DevTet* t = tets + fc;
-            unsafeAtomicAdd(den + t->con[0], l0);
-            unsafeAtomicAdd(den + t->con[1], l1);
-            unsafeAtomicAdd(den + t->con[2], l2);
-            unsafeAtomicAdd(den + t->con[3], l3);
+        if (AGG == 0) {
+            if (ok) unsafeAtomicAdd(den + node, lw);
+        } else {
+            lds_scatter_one(den, ok, node, lw);
         }
     }
-    tally(part, act, lost, crossed, st.moves, capped, st.refl);
+    const bool q0 = sub == 0;  // one lane per particle counts
+    tally(part, act && q0, lost && q0, crossed && q0, q0 ? moves : 0, capped && q0, q0 ? refl : 0);
 }
 
 // Drop-in kernel over the reference's AoS Particle (88 B: pos@0 vel@24 lc@48 cell@80).
@@ -610,17 +764,47 @@ static hipError_t launch_push_persist(const PushArgs& a, hipStream_t st) {
     return launch_tally_finish(a.part, g, a.ctr, st);
 }
 
+static hipError_t launch_push_tpp4(const PushArgs& a, hipStream_t st) {
+    const int bs = a.block;
+    const unsigned g = (grid_for(4 * a.n, bs) + kXcds - 1) / kXcds * kXcds;
+#define MP_TPP4(W, D, G)                                                                            \
+    hipLaunchKernelGGL((k_push_tpp4<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
+                       a.den, a.keys, a.vals, a.part, a.cap)
+    if (a.wall == 0) {
+        if (a.det) MP_TPP4(0, 1, 0);
+        else if (a.agg == 0) MP_TPP4(0, 0, 0);
+        else MP_TPP4(0, 0, 2);
+    } else {
+        if (a.det) MP_TPP4(1, 1, 0);
+        else if (a.agg == 0) MP_TPP4(1, 0, 0);
+        else MP_TPP4(1, 0, 2);
+    }
+#undef MP_TPP4
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
+    if (a.tpp == 4) return launch_push_tpp4(a, st);
     if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
-    const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
-#define MP_PUSH(W, D, G)                                                                          \
-    hipLaunchKernelGGL((k_push_soa<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, \
-                       a.dt, a.den, a.keys, a.vals, a.part, a.cap)
+    const int ppt = a.ppt > 0 ? a.ppt : 1;
+    const unsigned g = (grid_for(a.n, (int64_t)bs * ppt) + kXcds - 1) / kXcds * kXcds;
+#define MP_PUSH(W, D, G)                                                                                  \
+    do {                                                                                                  \
+        if (ppt == 1)                                                                                     \
+            hipLaunchKernelGGL((k_push_soa<W, D, G, false>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1);                      \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_push_soa<W, D, G, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n,  \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt);                    \
+    } while (0)
 #define MP_PUSH_W(W)                          \
     do {                                      \
         if (a.det) MP_PUSH(W, 1, 0);          \
+        else if (a.agg == 4) MP_PUSH(W, 0, 4); \
         else if (a.agg == 2) MP_PUSH(W, 0, 2); \
         else if (a.agg == 1) MP_PUSH(W, 0, 1); \
         else MP_PUSH(W, 0, 0);                \
@@ -631,7 +815,9 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
 #undef MP_PUSH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, g, a.ctr, st);
+    // AGG 4 leaves one tally row per wave, the others one per block
+    const int64_t rows = (!a.det && a.agg == 4) ? (int64_t)g * (bs / kWave) : (int64_t)g;
+    return launch_tally_finish(a.part, rows, a.ctr, st);
 }
 
 hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st) {
@@ -640,7 +826,13 @@ hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long lo
     return hipGetLastError();
 }
 
-int64_t tally_ints(int64_t n, int block) { return ((n + block - 1) / block + kXcds) * kTally; }
+// Tally rows for n particles: one per workgroup of `block` threads, or one per wave; sized
+// for the finest split (a wave of 64 lanes, four lanes per particle) so any block size,
+// threads-per-particle and aggregation fit.
+int64_t tally_ints(int64_t n, int block) {
+    (void)block;
+    return ((4 * n + kWave - 1) / kWave + kXcds * (256 / kWave)) * kTally;
+}
 
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
                            double* den, uint32_t* keys, double* vals, int32_t* part,

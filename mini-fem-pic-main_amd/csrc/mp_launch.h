@@ -27,13 +27,15 @@ struct PushArgs {
     double* den;
     uint32_t* keys;  // deterministic records: node of slot 4*pid+v
     double* vals;    //                        weight of slot 4*pid+v
-    int32_t* part;  // per-block tallies, tally_ints(n, block) ints
+    int32_t* part;  // per-block or per-wave tallies, tally_ints(n, block) ints
     unsigned long long* ctr;
     int cap;
     int wall;
     bool det;
     int agg;
     int block;
+    int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
+    int tpp;  // threads per particle: 1, or 4 (one lane per tet vertex)
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);

@@ -1,0 +1,180 @@
+// Autotuner of the device-resident step (mp_autotune): picks threads per particle, particles
+// per thread, workgroup size, atomic-scatter aggregation, cell-sort key shift and sort
+// interval for the particles currently resident, by timing short windows of mp_step from one
+// saved state.
+//
+// Every candidate gives bit-identical cells, positions and velocities (the options change only
+// the schedule and the memory order), so the tuner only measures time.  Protocol per
+// candidate with sort interval I:
+//   restore the saved particles, sort them once (untimed), run I steps (untimed: the cell
+//   order decays as it does between sorts in a long run), then time the next I steps; the
+//   first of them sorts again, so the timed window holds exactly one sort per I steps, as the
+//   steady state does.
+// Coordinate descent, one knob at a time in the order sort shift, sort interval, aggregation
+// (atomic mode), workgroup size, particles per thread, threads per particle, then the sort
+// interval again under the chosen kernel.  The saved state, counters and density are restored
+// before returning, so tuning leaves the run where it was; only the options change.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mp_ctx_impl.h"
+
+using namespace mpctx;
+
+namespace {
+
+struct Saved {
+    void* mem = nullptr;
+    Soa soa{};
+    int64_t n_slots = 0, next_pid = 0, since_sort = 0;
+    unsigned long long ctr[mpd::kNumCounters] = {};
+    std::vector<double> den;
+};
+
+hipError_t copy_soa(const Soa& dst, const Soa& src, int64_t n, hipStream_t st) {
+    const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int32_t) * (size_t)n;
+    double* const dd[6] = {dst.x, dst.y, dst.z, dst.vx, dst.vy, dst.vz};
+    const double* const sd[6] = {src.x, src.y, src.z, src.vx, src.vy, src.vz};
+    for (int k = 0; k < 6; k++) {
+        hipError_t e = hipMemcpyAsync(dd[k], sd[k], nd, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMemcpyAsync(dst.cell, src.cell, ni, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst.pid, src.pid, ni, hipMemcpyDeviceToDevice, st);
+    return e;
+}
+
+int save(mp_ctx* c, Saved& s) {
+    s.n_slots = c->n_slots;
+    s.next_pid = c->next_pid;
+    s.since_sort = c->since_sort;
+    if (alloc_soa(&s.mem, &s.soa, std::max<int64_t>(c->n_slots, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(c, MP_ERR_ALLOC, "mp_autotune: no device memory for the saved particle state");
+    }
+    HIP_TRY(c, copy_soa(s.soa, c->soa[c->cur], c->n_slots, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(s.ctr, c->ctr, sizeof(s.ctr), hipMemcpyDeviceToHost, c->stream));
+    s.den.resize((size_t)c->n_nodes);
+    HIP_TRY(c, hipMemcpyAsync(s.den.data(), c->den_out, sizeof(double) * s.den.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int restore(mp_ctx* c, const Saved& s) {
+    // both SoA buffers have capacity c->cap >= s.n_slots (steps never add particles)
+    HIP_TRY(c, copy_soa(c->soa[c->cur], s.soa, s.n_slots, c->stream));
+    c->n_slots = s.n_slots;
+    c->next_pid = s.next_pid;
+    c->since_sort = s.since_sort;
+    HIP_TRY(c, hipMemcpyAsync(c->ctr, s.ctr, sizeof(s.ctr), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->den_out, s.den.data(), sizeof(double) * s.den.size(), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+struct Trial {
+    mp_ctx* c;
+    const Saved* saved;
+    double qm, dt, spwt;
+    int32_t mode;
+    int32_t min_steps;
+    hipEvent_t ev[2];
+    std::vector<mp_tune_trial> log;
+
+    // Time per step (ms) of options o, by the protocol in the file comment.
+    int run(const mp_options& o, double* ms_per_step) {
+        int rc = restore(c, *saved);
+        if (rc) return rc;
+        c->opt = o;
+        rc = mp_sort_now(c);
+        if (rc) return rc;
+        const int interval = o.sort_interval > 0 ? o.sort_interval : min_steps;
+        const int timed = std::max(interval, min_steps);
+        for (int k = 0; k < interval; k++)
+            if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+        for (int k = 0; k < timed; k++)
+            if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+        HIP_TRY(c, hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+        *ms_per_step = (double)ms / timed;
+        log.push_back(mp_tune_trial{o, *ms_per_step});
+        return MP_OK;
+    }
+};
+
+bool valid(const mp_options& o) {
+    if (o.threads_per_particle == 4 && o.particles_per_thread != 1) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode, int32_t min_steps,
+                           mp_options* best, double* best_ms_per_step, mp_tune_trial* trials,
+                           int32_t max_trials, int32_t* n_trials) {
+    if (!c) return MP_ERR_ARG;
+    if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
+        return set_err(c, MP_ERR_ARG, "mp_autotune: unknown scatter mode");
+    if (min_steps < 1 || min_steps > 1000) return set_err(c, MP_ERR_ARG, "mp_autotune: min_steps must be 1..1000");
+    if (max_trials < 0 || (max_trials > 0 && !trials)) return set_err(c, MP_ERR_ARG, "mp_autotune: bad trial buffer");
+    if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_autotune: no field uploaded");
+    if (c->n_slots == 0) return set_err(c, MP_ERR_STATE, "mp_autotune: no resident particles");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+
+    const mp_options start = c->opt;
+    const bool timing = c->timing;
+    c->timing = false;
+    Saved saved;
+    int rc = save(c, saved);
+    Trial tr{c, &saved, q_over_m, dt, spwt, mode, min_steps, {nullptr, nullptr}, {}};
+    if (!rc && (hipEventCreate(&tr.ev[0]) != hipSuccess || hipEventCreate(&tr.ev[1]) != hipSuccess))
+        rc = set_err(c, MP_ERR_HIP, "mp_autotune: hipEventCreate failed");
+
+    mp_options cur = start;
+    double cur_ms = 0.0;
+    if (!rc) rc = tr.run(cur, &cur_ms);
+    // one knob at a time: keep a candidate only if it is faster than the incumbent
+    auto sweep = [&](auto set, std::initializer_list<int> values) {
+        for (int v : values) {
+            if (rc) return;
+            mp_options o = cur;
+            set(o, v);
+            if (!valid(o) || memcmp(&o, &cur, sizeof(o)) == 0) continue;
+            double ms = 0.0;
+            rc = tr.run(o, &ms);
+            if (!rc && ms < cur_ms) {
+                cur = o;
+                cur_ms = ms;
+            }
+        }
+    };
+    sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
+    sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
+    if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
+    sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
+    sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
+    sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
+    sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
+
+    // leave the run as it was, with the chosen options
+    if (!rc) rc = restore(c, saved);
+    c->opt = rc ? start : cur;
+    c->timing = timing;
+    for (hipEvent_t e : tr.ev)
+        if (e) (void)hipEventDestroy(e);
+    dfree(saved.mem);
+    if (rc) return rc;
+    if (best) *best = cur;
+    if (best_ms_per_step) *best_ms_per_step = cur_ms;
+    const int32_t nt = (int32_t)tr.log.size();
+    for (int32_t k = 0; k < std::min(nt, max_trials); k++) trials[k] = tr.log[(size_t)k];
+    if (n_trials) *n_trials = nt;
+    return MP_OK;
+}

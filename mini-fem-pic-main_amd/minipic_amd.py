@@ -49,6 +49,10 @@ class MpOptions(C.Structure):
                 ("walk_cap", C.c_int32), ("scatter_agg", C.c_int32), ("sort_shift", C.c_int32)]
 
 
+class MpTuneTrial(C.Structure):
+    _fields_ = [("opt", MpOptions), ("ms_per_step", C.c_double)]
+
+
 class MpCounters(C.Structure):
     _fields_ = [("particle_steps", C.c_int64), ("lost", C.c_int64), ("crossed", C.c_int64),
                 ("moves", C.c_int64), ("cap_hits", C.c_int64), ("reflections", C.c_int64)]
@@ -84,6 +88,8 @@ SIGNATURES = {
     "mp_get_timing": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64)]),
     "mp_adjoint": (C.c_int, [_P, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32, C.POINTER(_D), _P,
                              C.POINTER(_D)]),
+    "mp_autotune": (C.c_int, [_P, _D, _D, _D, _I32, _I32, C.POINTER(MpOptions), C.POINTER(_D),
+                              C.POINTER(MpTuneTrial), _I32, C.POINTER(_I32)]),
     "mp_batch_create": (C.c_int, [_P, _I32, C.POINTER(_P)]),
     "mp_batch_destroy": (None, [_P]),
     "mp_batch_set_run": (C.c_int, [_P, _I32, _P, _P, _P, _I64, _D, _D, _D]),
@@ -415,6 +421,18 @@ class Mover:
                                      steps, _ptr(tgt), _ptr(dd), checkpoint_every, C.byref(loss), _ptr(g),
                                      C.byref(dd_out)))
         return float(loss.value), g, float(dd_out.value)
+
+    def autotune(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC, min_steps=10, max_trials=64):
+        """mp_autotune: (chosen options, their ms per step, [(options, ms per step)] of every
+        timed candidate).  The chosen options are set; the particle state is left unchanged."""
+        best, ms = MpOptions(), C.c_double()
+        trials = (MpTuneTrial * max_trials)()
+        nt = C.c_int32()
+        self._check(lib().mp_autotune(self._h, q_over_m, dt, spwt, mode, min_steps, C.byref(best), C.byref(ms),
+                                      trials, max_trials, C.byref(nt)))
+        as_dict = lambda o: {k: int(getattr(o, k)) for k, _ in MpOptions._fields_}  # noqa: E731
+        log = [(as_dict(trials[k].opt), float(trials[k].ms_per_step)) for k in range(min(nt.value, max_trials))]
+        return as_dict(best), float(ms.value), log
 
     def set_timing(self, enable=True):
         self._check(lib().mp_set_timing(self._h, int(bool(enable))))

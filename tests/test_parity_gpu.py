@@ -217,7 +217,8 @@ def test_cpp_dropin_program(mp):
     assert r.stdout.count(" ok") == 2
 
 
-@pytest.mark.parametrize("agg", [0, 1, 2, 3], ids=["plain", "wave-by-tet", "lds-by-node", "persistent"])
+@pytest.mark.parametrize("agg", [0, 1, 2, 3, 4], ids=["plain", "wave-by-tet", "lds-by-node", "persistent",
+                                                     "lds-by-node-per-wave"])
 def test_atomic_aggregation_modes(mp, orc, agg):
     """Every atomic-scatter aggregation gives the oracle's density to 1e-12 relative."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=13, n=80000)
@@ -233,6 +234,59 @@ def test_atomic_aggregation_modes(mp, orc, agg):
             check_den(mv.density(), ref, mp.SCATTER_ATOMIC, mp)
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+
+
+@pytest.mark.parametrize("ppt,agg,block", [(2, 2, 256), (4, 4, 128), (2, 4, 64), (4, 0, 256)])
+def test_particles_per_thread_matches_oracle(mp, orc, ppt, agg, block):
+    """Several particles per thread (and per-wave aggregation, smaller blocks) change only the
+    schedule: cells, positions, velocities bit-exact, both density modes to their bar."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=15, n=70001)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(particles_per_thread=ppt, scatter_agg=agg, block_size=block, sort_interval=2)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(4):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density(), ref, mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+@pytest.mark.parametrize("agg,block", [(0, 256), (2, 64), (2, 256)])
+def test_four_threads_per_particle_matches_oracle(mp, orc, wall, agg, block):
+    """One lane per tet vertex (quad DPP exchange of the barycentrics): same decisions, same
+    bits as the serial walk; atomic and deterministic steps alternate; a walk cap of 3 is hit."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=19 + wall, n=50003)
+    dt = 9e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=3)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(threads_per_particle=4, scatter_agg=agg, block_size=block, sort_interval=2,
+                       wall_mode=wall, walk_cap=3)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(4):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density(), ref, mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+        assert c["cap_hits"] > 0
+        if wall == 1:
+            assert c["reflections"] > 0
 
 
 @pytest.mark.parametrize("shift", [3, 7])
@@ -343,3 +397,34 @@ def test_batch_sequential_equals_batched(mp):
         np.testing.assert_allclose(da, db, rtol=ATOMIC_RTOL, atol=0)
         for x, y in zip(pa, pb):
             assert x.tobytes() == y.tobytes()
+
+
+def test_autotune_leaves_state_and_keeps_parity(mp, orc):
+    """mp_autotune times candidates from a saved state and restores it: particles, density and
+    counters are unchanged, the chosen options are valid, and later steps still match the
+    oracle bit for bit."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=23, n=120000)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for _ in range(2):
+            mv.step(QM, dt, SPWT, mp.SCATTER_DETERMINISTIC)
+            run.step(QM, dt, SPWT)
+        before = mv.get_particles(), mv.density(), mv.counters()
+        best, ms, trials = mv.autotune(QM, dt, SPWT, mp.SCATTER_ATOMIC, min_steps=3)
+        after = mv.get_particles(), mv.density(), mv.counters()
+        for a, b in zip(before[0], after[0]):
+            assert a.tobytes() == b.tobytes()
+        assert before[1].tobytes() == after[1].tobytes() and before[2] == after[2]
+        assert mv.options == best
+        assert best["threads_per_particle"] in (1, 4) and best["particles_per_thread"] in (1, 2, 4)
+        assert best["block_size"] in (64, 128, 256) and 0 <= best["scatter_agg"] <= 4
+        assert len(trials) > 20 and ms == min(t for _, t in trials) and ms > 0
+        for k in range(3):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
