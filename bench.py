@@ -211,6 +211,59 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
                         f"no drift, reflecting walls, sort every 8 steps (shift {shift}), dt {dt:g} s"}
 
 
+def run_coupled(steps=200):
+    """mini-pic's coupled loop on a 12^3 six-tet box of side 0.1 m (10,368 tets, 2,197 nodes):
+    O+ ions injected at 7 km/s through the x = 0 face (n = 1e10 m^-3, spwt 200: 1750 a step),
+    Boltzmann electrons at Te = 2 eV, dt = 5e-7 s; each step InjectIons (host) -> MoveParticles
+    (GPU drop-in, deterministic scatter) -> FE Poisson-Boltzmann (host, Newton + Gauss-Seidel,
+    F1 cleared per solve) -> updateEf.  Also runs the reference's own solver setting (F1
+    accumulating) until its Newton iteration stops converging, as the reference's would."""
+    import tempfile
+
+    import coupled as cp
+
+    with tempfile.TemporaryDirectory() as d:
+        mesh = cp.box_case(12, 0.1, d)
+    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                      dt=5e-7, max_iter=steps)
+    out = {}
+    gpu = cp.GpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
+    try:
+        run = cp.CoupledRun(mesh, p, move=gpu, fe_flags=mp.FE_RESET_F1)
+        t_inj = t_move = t_fe = 0.0
+        psteps = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            run.inject()
+            b = time.perf_counter()
+            psteps += len(run.parts)
+            run.move_step()
+            c = time.perf_counter()
+            run.solve()
+            t_inj, t_move, t_fe = t_inj + b - a, t_move + c - b, t_fe + time.perf_counter() - c
+        wall = time.perf_counter() - t0
+        out = {"steps": steps, "particles_final": len(run.parts), "ms_per_step": 1e3 * wall / steps,
+               "inject_ms_per_step": 1e3 * t_inj / steps, "move_ms_per_step": 1e3 * t_move / steps,
+               "fe_ms_per_step": 1e3 * t_fe / steps, "move_particle_steps_per_s": psteps / t_move}
+        run = cp.CoupledRun(mesh, p, move=gpu)
+        fail = None
+        for ts in range(steps):
+            try:
+                run.step()
+            except mp.MpError:
+                fail = ts
+                break
+        out["reference_solver_newton_fails_at_step"] = fail
+    finally:
+        gpu.close()
+    log(f"coupled: {out['ms_per_step']:.2f} ms/step (move {out['move_ms_per_step']:.2f}, FE {out['fe_ms_per_step']:.2f}),"
+        f" {out['particles_final']} ions at the end; reference-F1 Newton fails at step {out['reference_solver_newton_fails_at_step']}")
+    out["workload"] = ("12^3 six-tet box, side 0.1 m, O+ beam 7 km/s from x = 0, n 1e10 m^-3, Te 2 eV, dt 5e-7 s, "
+                       "GPU drop-in MoveParticles (deterministic) + host FE Newton/Gauss-Seidel, F1 cleared per solve")
+    return out
+
+
 def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
     """Adjoint: 2M particles on a 30^3 six-tet box (side 1 m), frozen sinusoidal field, K = 50
     steps, checkpoint every 10; L = |den_K - den_target|^2 with the target from a forward run
@@ -289,6 +342,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
+    ap.add_argument("--coupled-steps", type=int, default=200, help="steps of the coupled PIC loop (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
@@ -323,6 +377,8 @@ def main():
         extra["adjoint"] = run_adjoint(args.adjoint_steps)
     if args.contention_steps > 0:
         extra["contention"] = run_contention(args.contention_steps, args.contention_particles)
+    if args.coupled_steps > 0:
+        extra["coupled"] = run_coupled(args.coupled_steps)
     cpu = None
     if args.cpu_seconds > 0:
         cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)

@@ -11,6 +11,8 @@
 // Compiled with -ffp-contract=off: no fused multiply-adds anywhere.
 #include "mp_mesh.h"
 
+#include "fe_geom.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -453,50 +455,15 @@ int mp_field_from_potential(const mp_mesh* m, const double* phi, double* ef) {
     if (!m || !phi || !ef) return fail("mp_field_from_potential: null argument");
     const int64_t T = (int64_t)m->tets.size();
     // computeNX (src/FESolver.cpp:207-246), inverse (:249-287), updateEf (:622-636)
-    static const double na_xi[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
     parallel_for(T, [&](int64_t e) {
         const mp_tetra& tet = m->tets[e];
-        double x[4][3];
-        for (int a = 0; a < 4; a++)
+        double x[4][3], NX[4][3], u[4];
+        for (int a = 0; a < 4; a++) {
             for (int d = 0; d < 3; d++) x[a][d] = m->pos[3 * (size_t)tet.con[a] + d];
-        double x_xi[3][3];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                x_xi[i][j] = 0;
-                for (int a = 0; a < 4; a++) x_xi[i][j] += na_xi[a][j] * x[a][i];
-            }
-        double a = x_xi[0][0], b = x_xi[0][1], c = x_xi[0][2];
-        double d = x_xi[1][0], ee = x_xi[1][1], f = x_xi[1][2];
-        double g = x_xi[2][0], h = x_xi[2][1], i = x_xi[2][2];
-        double V[3][3];
-        V[0][0] = (ee * i - f * h);
-        V[1][0] = -(d * i - f * g);
-        V[2][0] = (d * h - ee * g);
-        V[0][1] = -(b * i - c * h);
-        V[1][1] = (a * i - c * g);
-        V[2][1] = -(a * h - b * g);
-        V[0][2] = (b * f - c * ee);
-        V[1][2] = -(a * f - c * d);
-        V[2][2] = (a * ee - b * d);
-        double det = a * V[0][0] + b * V[1][0] + c * V[2][0];
-        double Vmax = 0;
-        for (int r = 0; r < 3; r++)
-            for (int s = 0; s < 3; s++) Vmax = std::fabs(V[r][s]) > Vmax ? std::fabs(V[r][s]) : Vmax;
-        double idet = 0;
-        if (!(std::fabs(Vmax) / std::fabs(det) > 1e12)) idet = 1 / det;
-        for (int r = 0; r < 3; r++)
-            for (int s = 0; s < 3; s++) V[r][s] *= idet;
-        double out[3] = {0, 0, 0};
-        for (int q = 0; q < 4; q++) {
-            double nx[3];
-            for (int dd = 0; dd < 3; dd++) {
-                nx[dd] = 0;
-                for (int k = 0; k < 3; k++) nx[dd] += na_xi[q][k] * V[k][dd];
-            }
-            double u = phi[tet.con[q]];
-            for (int dd = 0; dd < 3; dd++) out[dd] -= nx[dd] * u;
+            u[a] = phi[tet.con[a]];
         }
-        for (int dd = 0; dd < 3; dd++) ef[3 * e + dd] = out[dd];
+        mphost::element_nx(x, NX, nullptr);
+        mphost::element_ef(NX, u, ef + 3 * e);
     });
     return MP_OK;
 }

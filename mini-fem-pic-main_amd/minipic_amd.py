@@ -53,6 +53,11 @@ class MpTuneTrial(C.Structure):
     _fields_ = [("opt", MpOptions), ("ms_per_step", C.c_double)]
 
 
+class MpFeStats(C.Structure):
+    _fields_ = [("nr_iterations", C.c_int32), ("gs_sweeps", C.c_int32), ("gs_unconverged", C.c_int32),
+                ("zero_diagonal", C.c_int32), ("nr_l2", C.c_double)]
+
+
 class MpCounters(C.Structure):
     _fields_ = [("particle_steps", C.c_int64), ("lost", C.c_int64), ("crossed", C.c_int64),
                 ("moves", C.c_int64), ("cap_hits", C.c_int64), ("reflections", C.c_int64)]
@@ -113,6 +118,21 @@ SIGNATURES = {
     "mp_mesh_node_volume": (_P, [_P]),
     "mp_mesh_node_type": (_P, [_P]),
     "mp_mesh_last_error": (C.c_char_p, []),
+    # mp_fem.h
+    "mp_fe_create": (C.c_int, [_P, _D, _D, _D, _D, _I32, C.POINTER(_P)]),
+    "mp_fe_free": (None, [_P]),
+    "mp_fe_num_unknowns": (_I32, [_P]),
+    "mp_fe_compute_phi": (C.c_int, [_P, _P, _I32, C.POINTER(MpFeStats)]),
+    "mp_fe_update_ef": (C.c_int, [_P, _P]),
+    "mp_fe_get_phi": (C.c_int, [_P, _P]),
+    "mp_fe_get_stiffness": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P]),
+    "mp_fe_last_error": (C.c_char_p, []),
+    # mp_inject.h
+    "mp_rng_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
+    "mp_rng_free": (None, [_P]),
+    "mp_rng_uniform": (C.c_int, [_P, _P, _I64]),
+    "mp_inject_count": (C.c_int, [_P, _D, _D, _D, _D, _D, C.POINTER(_I64)]),
+    "mp_inject_ions": (C.c_int, [_P, _P, _D, _D, _D, _D, _D, _P, C.POINTER(_D), _P, _I64, C.POINTER(_I64)]),
     "mp_mesh_num_inlet_faces": (_I32, [_P]),
     "mp_mesh_inlet_faces": (_P, [_P]),
     "mp_mesh_locate_box": (C.c_int, [_P, _P, _I64, _P]),
@@ -275,6 +295,107 @@ def synth_mixture(seed, n, lo, hi, weights, centres, sigma_pos, sigma_vel, drift
     if rc != MP_OK:
         raise MpError(rc, lib().mp_mesh_last_error().decode())
     return pos, vel
+
+
+# ---------------------------------------------------------------------------------------
+FE_NONLINEAR, FE_GAUSS_SEIDEL, FE_DIRECT = 0, 1, 2
+FE_RESET_F1 = 1
+KB = 8.617333262e-5   # src/FESolver.h:21, eV/K
+EPS0 = 8.8541878e-12  # src/FESolver.h:18
+
+
+class FESolver:
+    """Host FE Poisson-Boltzmann solver (FESolver, src/FESolver.cpp) over a Mesh's node types."""
+
+    def __init__(self, mesh, n0, phi0, kTe, wall_potential=0.0, flags=0):
+        h = C.c_void_p()
+        rc = lib().mp_fe_create(mesh._h, n0, phi0, kTe, wall_potential, flags, C.byref(h))
+        if rc != MP_OK:
+            raise MpError(rc, lib().mp_fe_last_error().decode())
+        self._h = h
+        self.mesh = mesh
+        self.neq = int(lib().mp_fe_num_unknowns(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.mp_fe_free(self._h)
+            self._h = C.c_void_p()
+
+    def compute_phi(self, ion_den, method=FE_NONLINEAR):
+        den = _f64(ion_den)
+        assert den.shape == (self.mesh.n_nodes,)
+        st = MpFeStats()
+        rc = lib().mp_fe_compute_phi(self._h, _ptr(den), method, C.byref(st))
+        if rc != MP_OK:
+            raise MpError(rc, lib().mp_fe_last_error().decode())
+        return {k: (float if k == "nr_l2" else int)(getattr(st, k)) for k, _ in MpFeStats._fields_}
+
+    def update_ef(self):
+        ef = np.empty((self.mesh.n_tets, 3), np.float64)
+        rc = lib().mp_fe_update_ef(self._h, _ptr(ef))
+        if rc != MP_OK:
+            raise MpError(rc, lib().mp_fe_last_error().decode())
+        return ef
+
+    @property
+    def phi(self):
+        uh = np.empty(self.mesh.n_nodes, np.float64)
+        rc = lib().mp_fe_get_phi(self._h, _ptr(uh))
+        if rc != MP_OK:
+            raise MpError(rc, lib().mp_fe_last_error().decode())
+        return uh
+
+    def stiffness(self):
+        """K over the unknowns as (row_ptr, col, val)."""
+        nnz = C.c_int64()
+        lib().mp_fe_get_stiffness(self._h, C.byref(nnz), None, None, None)
+        rp = np.empty(self.neq + 1, np.int64)
+        ci = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float64)
+        rc = lib().mp_fe_get_stiffness(self._h, C.byref(nnz), _ptr(rp), _ptr(ci), _ptr(val))
+        if rc != MP_OK:
+            raise MpError(rc, lib().mp_fe_last_error().decode())
+        return rp, ci, val
+
+
+class Rng:
+    """rnd() of src/maths.cpp:17-19: std::mt19937(seed) + uniform_real_distribution<double>(0, 1)."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        rc = lib().mp_rng_create(seed, C.byref(h))
+        if rc != MP_OK:
+            raise MpError(rc, "mp_rng_create failed")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.mp_rng_free(self._h)
+            self._h = C.c_void_p()
+
+    def uniform(self, n):
+        out = np.empty(n, np.float64)
+        rc = lib().mp_rng_uniform(self._h, _ptr(out), n)
+        if rc != MP_OK:
+            raise MpError(rc, "mp_rng_uniform failed")
+        return out
+
+
+def inject_ions(mesh, rng, plasma_den, ion_velocity, dt, spwt, q_over_m, ef, rem):
+    """InjectIons (src/mini-pic.cpp:208-265): (new Particle records, carried fraction)."""
+    n = C.c_int64()
+    rc = lib().mp_inject_count(mesh._h, plasma_den, ion_velocity, dt, spwt, rem, C.byref(n))
+    if rc != MP_OK:
+        raise MpError(rc, "mp_inject_count failed")
+    out = np.zeros(max(n.value, 1), PARTICLE_DTYPE)
+    ef = _f64(ef, (-1, 3))
+    r = C.c_double(rem)
+    got = C.c_int64()
+    rc = lib().mp_inject_ions(mesh._h, rng._h, plasma_den, ion_velocity, dt, spwt, q_over_m, _ptr(ef), C.byref(r),
+                              _ptr(out), len(out), C.byref(got))
+    if rc != MP_OK:
+        raise MpError(rc, "mp_inject_ions failed")
+    return out[:got.value], float(r.value)
 
 
 # ---------------------------------------------------------------------------------------

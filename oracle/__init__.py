@@ -3,7 +3,8 @@
 Only tests/, the smoke check and the CPU-baseline leg of bench.py import this package:
 it is the checker, never the thing measured or shipped.
 
-  liboracle.so      plain-C serial restatement (mp_oracle.c), -ffp-contract=off
+  liboracle.so      plain-C serial restatement of the mover (mp_oracle.c) and of the
+                    reference's dense FE solver (fe_dense.c), -ffp-contract=off
   _ref/libref.so    the reference's own src/meshes.cpp, maths.cpp, trace.cpp compiled where
                     they lie, plus ref_harness.cpp (built only where the reference exists)
 """
@@ -34,6 +35,11 @@ _SIG = {
     "or_normalise": (None, [_P, _P, _I32, _D]),
     "or_adjoint": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32,
                              _P, _P, _P]),
+    "or_fe_create": (_P, [_P, _P, _I32, _P, _I32, _D, _D, _D, _D, C.c_int]),
+    "or_fe_free": (None, [_P]),
+    "or_fe_compute_phi": (C.c_int, [_P, _P, C.c_int]),
+    "or_fe_get": (None, [_P, _P, _P]),
+    "or_fe_neq": (_I32, [_P]),
 }
 _REF_SIG = {
     "ref_load_volume": (C.c_int, [C.c_char_p]),
@@ -145,6 +151,52 @@ def compact(pos, vel, cell, pid=None):
 def compact_aos(parts):
     k = lib().or_compact_aos(_ptr(parts), len(parts))
     return parts[:k]
+
+
+class DenseFE:
+    """FESolver with the reference's dense K and J (fe_dense.c): the checker of the library's
+    compressed-row solver.  node_type as in src/meshes.h:22; method 0 NonLinear, 1 GaussSeidel."""
+
+    def __init__(self, node_pos, node_type, tets, n0, phi0, kTe, wall_potential=0.0, reset_f1=False):
+        self.pos = np.ascontiguousarray(node_pos, np.float64)
+        self.type = np.ascontiguousarray(node_type, np.int32)
+        self.con = np.ascontiguousarray(np.asarray(tets)["con"], np.int32)
+        self.nn, self.ne = len(self.type), len(self.con)
+        self._h = lib().or_fe_create(_ptr(self.pos), _ptr(self.type), self.nn, _ptr(self.con), self.ne, n0, phi0,
+                                     kTe, wall_potential, int(bool(reset_f1)))
+        self.neq = int(lib().or_fe_neq(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_fe_free(self._h)
+            self._h = None
+
+    def compute_phi(self, ion_den, method=0):
+        den = np.ascontiguousarray(ion_den, np.float64)
+        return int(lib().or_fe_compute_phi(self._h, _ptr(den), method))
+
+    def phi_ef(self):
+        uh = np.empty(self.nn)
+        ef = np.empty((self.ne, 3))
+        lib().or_fe_get(self._h, _ptr(uh), _ptr(ef))
+        return uh, ef
+
+
+class AosMover:
+    """MoveParticles (src/mini-pic.cpp:268-320) serially on the host: the reference kernel body
+    over the Particle records, stable erase of the lost ones, den = raw * (spwt / node volume).
+    Same call shape as the GPU mover of the coupled loop: (parts, ef) -> (kept parts, den)."""
+
+    def __init__(self, tets, node_volume, qm, dt, spwt):
+        self.tets = np.ascontiguousarray(tets, dtype=TET_DTYPE)
+        self.vol = np.ascontiguousarray(node_volume, dtype=np.float64)
+        self.qm, self.dt, self.spwt = qm, dt, spwt
+
+    def __call__(self, parts, ef):
+        parts = np.ascontiguousarray(parts)
+        raw, _ = move_aos(self.tets, ef, parts, self.qm, self.dt, np.zeros(len(self.vol)))
+        parts = compact_aos(parts).copy()
+        return parts, normalise(raw, self.vol, self.spwt)
 
 
 class Run:

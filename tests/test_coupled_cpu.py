@@ -1,0 +1,167 @@
+"""The host side of mini-pic's coupled loop (no GPU): the random stream, InjectIons, the FE
+Poisson-Boltzmann solver and the parameter file, each pinned by the reference's draws, closed
+forms or the dense restatement of FESolver."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+QE, AMU = 1.602e-19, 1.660538921e-27
+QM = QE / (16 * AMU)
+
+
+@pytest.fixture(scope="module")
+def cp():
+    import coupled
+
+    return coupled
+
+
+def test_rng_draws_match_reference(mp):
+    """rnd() of src/maths.cpp:17-19: the draws the reference's own maths.cpp produced."""
+    want = np.load(os.path.join(GOLDEN, "rnd_mt19937.npz"))["draws"]
+    assert mp.Rng(0).uniform(len(want)).tobytes() == want.tobytes()
+
+
+def test_inject_ions_known_answer(mp, cp, tmp_path):
+    """InjectIons on the 8 inlet triangles of a 2^3 box, recomputed from the reference's draws:
+    counts with the carried fraction, folded triangle sampling, the cold beam and the half-step
+    velocity rewind in the start tet's field."""
+    mesh = cp.box_case(2, 0.1, str(tmp_path))
+    faces = mesh.inlet_faces
+    assert len(faces) == 8 and np.all(faces["normal"] == [1.0, 0.0, 0.0])
+    assert np.allclose(faces["area"], 0.05 ** 2 / 2, rtol=1e-15)
+    ef = np.random.default_rng(3).normal(0.0, 50.0, (mesh.n_tets, 3))
+    plasma_den, v0, dt, spwt = 1e8, 7000.0, 5e-7, 200.0  # 2.1875 macroparticles per face
+    draws = np.load(os.path.join(GOLDEN, "rnd_mt19937.npz"))["draws"]
+    parts, rem = mp.inject_ions(mesh, mp.Rng(0), plasma_den, v0, dt, spwt, QM, ef, 0.25)
+    # the same arithmetic, face by face, from the recorded draws
+    r, k, want_pos, want_vel, want_cell = 0.25, 0, [], [], []
+    npos = mesh.node_pos
+    for f in faces:
+        fnum = plasma_den * v0 * f["area"] * dt / spwt + r
+        num = int(fnum)
+        r = fnum - num
+        for _ in range(num):
+            a, b = draws[k], draws[k + 1]
+            k += 2
+            if a + b > 1:
+                a, b = 1 - a, 1 - b
+            want_pos.append(a * f["u"] + b * f["v"] + npos[f["con"][0]])
+            want_vel.append(f["normal"] * v0 - QM * ef[f["cell_con"]] * (0.5 * dt))
+            want_cell.append(f["cell_con"])
+    assert 0 < len(parts) == len(want_pos) and k <= len(draws)
+    assert rem == r
+    assert parts["pos"].tobytes() == np.array(want_pos).tobytes()
+    assert parts["vel"].tobytes() == np.array(want_vel).tobytes()
+    assert np.array_equal(parts["cell_index"], want_cell)
+    # every sample lies on the x = 0 face, in its face's triangle, inside the box
+    assert np.all(parts["pos"][:, 0] == 0.0) and np.all((parts["pos"] >= 0) & (parts["pos"] <= 0.1))
+
+
+def test_fe_stiffness_closed_form(mp, cp, tmp_path):
+    """K_PQ = sum_e 8 detJ_e (grad N_a . grad N_b) (2x2x2 unit weights over the element), with the
+    gradients taken from the tet tables (grad lc_i = (-beta_i, gamma_i, -delta_i) / 6V) and
+    detJ from numpy: an independent evaluation.  K is exactly symmetric."""
+    mesh = cp.box_case(4, 0.1, str(tmp_path), wall=True)
+    fe = mp.FESolver(mesh, n0=1e10, phi0=0.0, kTe=2.0, wall_potential=3.0)
+    rp, ci, val = fe.stiffness()
+    K = np.zeros((fe.neq, fe.neq))
+    for i in range(fe.neq):
+        K[i, ci[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    assert K.tobytes() == K.T.copy().tobytes()
+    t = mesh.tets
+    types = mesh.node_type
+    ID = np.cumsum((types == 0) | (types == 1)) - 1
+    ID[(types != 0) & (types != 1)] = -1
+    pos = mesh.node_pos
+    Kc = np.zeros_like(K)
+    for e in range(mesh.n_tets):
+        con = t["con"][e]
+        grad = np.stack([-t["beta"][e], t["gamma"][e], -t["delta"][e]], axis=1) / (6 * t["volume"][e])
+        x = pos[con]
+        detJ = np.linalg.det((x[:3] - x[3]).T)
+        ke = 8 * detJ * grad @ grad.T
+        for a in range(4):
+            for b in range(4):
+                P, Q = ID[con[a]], ID[con[b]]
+                if P >= 0 and Q >= 0:
+                    Kc[P, Q] += ke[a, b]
+    np.testing.assert_allclose(K, Kc, rtol=1e-11, atol=1e-13 * np.abs(Kc).max())
+
+
+@pytest.mark.parametrize("method", [2, 0], ids=["direct", "nonlinear"])
+def test_fe_laplace_reproduces_linear_potential(mp, cp, tmp_path, method):
+    """No space charge (n0 = 0, no ions), phi = 0 on the x = 0 inlet and -V on the x = L wall,
+    the other faces open: the exact solution phi = -V x / L is linear, so P1 elements reproduce
+    it; E is uniform (V / L, 0, 0).  The direct solve holds it to rounding, Gauss-Seidel to its
+    1e-4 residual tolerance."""
+    V, L = 7.0, 0.1
+    mesh = cp.box_case(5, L, str(tmp_path), wall=True)
+    fe = mp.FESolver(mesh, n0=0.0, phi0=0.0, kTe=2.0, wall_potential=V)
+    fe.compute_phi(np.zeros(mesh.n_nodes), method)
+    exact = -V * mesh.node_pos[:, 0] / L
+    ef = fe.update_ef()
+    tol = 1e-9 if method == 2 else 2e-2
+    np.testing.assert_allclose(fe.phi, exact, atol=tol * V)
+    np.testing.assert_allclose(ef, np.tile([V / L, 0.0, 0.0], (mesh.n_tets, 1)), atol=tol * 50 * V / L)
+
+
+@pytest.mark.parametrize("reset", [False, True], ids=["reference-F1", "reset-F1"])
+@pytest.mark.parametrize("method", [0, 1], ids=["nonlinear", "gauss-seidel"])
+def test_fe_matches_dense_restatement(mp, orc, cp, tmp_path, method, reset):
+    """The compressed-row solver against the reference's dense loops (oracle), bit for bit over
+    successive solves with a Boltzmann electron term and a fixed wall: phi and E, and the same
+    Newton failure when the reference's accumulating F1 stops the iteration converging."""
+    mesh = cp.box_case(5, 0.1, str(tmp_path), wall=True)
+    args = dict(n0=1e10, phi0=0.0, kTe=2.0, wall_potential=5.0)
+    fe = mp.FESolver(mesh, flags=mp.FE_RESET_F1 if reset else 0, **args)
+    ref = orc.DenseFE(mesh.node_pos, mesh.node_type, mesh.tets, reset_f1=reset, **args)
+    assert fe.neq == ref.neq
+    rng = np.random.default_rng(11)
+    for k in range(6):
+        den = rng.uniform(0.5e10, 1.5e10, mesh.n_nodes)
+        rc = ref.compute_phi(den, method)
+        if rc != 0:
+            with pytest.raises(mp.MpError, match="NR failed to converge"):
+                fe.compute_phi(den, method)
+            assert not reset
+            return
+        fe.compute_phi(den, method)
+        uh, ef = ref.phi_ef()
+        assert fe.phi.tobytes() == uh.tobytes(), f"solve {k}: phi differs"
+        assert fe.update_ef().tobytes() == ef.tobytes(), f"solve {k}: E differs"
+
+
+def test_parameter_file(cp):
+    """Parameters::Parameters on the reference's coarse example file (its species spelling
+    'Duterium' is what the reference's own pattern accepts)."""
+    p = cp.Parameters.from_file(os.path.join(GOLDEN, "coarse_system.param"))
+    assert (p.plasma_den, p.ion_velocity, p.electron_temperature) == (1e16, 1e8, 2e8)
+    assert (p.wall_potential, p.max_iter, p.dt) == (10000.0, 250, 1e-12)
+    assert p.plasma_species == "Deuterium" and p.fesolver_method == 2 and not p.invert_normals
+    assert os.path.basename(p.mesh_files["inlet_mesh"]) == "inlet.dat"
+
+
+def test_coupled_loop_on_host(mp, orc, cp, tmp_path):
+    """The 12^3 box loop (ions at 7 km/s from the x = 0 face, n = 1e10, Te = 2 eV, dt = 5e-7 s)
+    with the serial mover: 1750 ions a step (plasma_den v A dt / spwt = 1750 with the carried
+    fraction), none leave before crossing the 0.1 m box (~28 steps), the Newton solve converges
+    with F1 reset; with the reference's accumulating F1 it stops converging within 60 steps,
+    as the reference would (it exits there)."""
+    mesh = cp.box_case(12, 0.1, str(tmp_path))
+    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                      dt=5e-7, max_iter=200)
+    mover = orc.AosMover(mesh.tets, mesh.node_volume, QM, p.dt, 2e2)
+    run = cp.CoupledRun(mesh, p, move=mover, fe_flags=mp.FE_RESET_F1)
+    recs = [run.step() for _ in range(40)]
+    added = np.array([r["added"] for r in recs])
+    assert abs(added.sum() - 40 * 1750) <= 1 and set(added) <= {1749, 1750, 1751}
+    assert all(r["removed"] == 0 for r in recs[:20]) and recs[-1]["removed"] > 0
+    assert all(r["nr_iterations"] <= 3 for r in recs)
+    run = cp.CoupledRun(mesh, p, move=mover)
+    with pytest.raises(mp.MpError, match="NR failed to converge"):
+        for _ in range(60):
+            run.step()
