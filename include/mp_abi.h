@@ -74,6 +74,10 @@ typedef struct mp_options {
                                       pipelined workgroups, LDS hash kept across chunks,
                                       4 per wave by node (LDS hash, no workgroup barrier) */
     int32_t sort_shift;            /* cell sort key = cell >> sort_shift (0: exact cell) */
+    int32_t stage_tets;            /* 1: each workgroup loads its distinct start tets into
+                                      LDS once (TPP=1, PPT=1, aggregation 0 or 2, or the
+                                      deterministic scatter); 0 (default): every lane
+                                      gathers its own */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */

@@ -158,9 +158,10 @@ __device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node
     }
 }
 
-__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                            bool ok, int32_t fc, double l0, double l1, double l2,
-                                            double l3, int32_t base = 0) {
+// The same over the final tet's node ids (already in registers).
+__device__ __forceinline__ void scatter_lds_nodes(double* __restrict__ den, bool ok, int32_t n0, int32_t n1,
+                                                  int32_t n2, int32_t n3, double l0, double l1, double l2,
+                                                  double l3) {
     __shared__ int32_t hk[kHashSlots];
     __shared__ double hv[kHashSlots];
     for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
@@ -169,9 +170,6 @@ __device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, dou
     }
     __syncthreads();
     if (ok) {
-        const DevTet* t = tets + fc;
-        const int32_t n0 = base + t->con[0], n1 = base + t->con[1], n2 = base + t->con[2],
-                      n3 = base + t->con[3];
         lds_insert(hk, hv, n0, l0);
         lds_insert(hk, hv, n1, l1);
         lds_insert(hk, hv, n2, l2);
@@ -182,6 +180,20 @@ __device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, dou
         const int32_t node = hk[s];
         if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
     }
+}
+
+__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                            bool ok, int32_t fc, double l0, double l1, double l2,
+                                            double l3, int32_t base = 0) {
+    int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+    if (ok) {
+        const DevTet* t = tets + fc;
+        n0 = base + t->con[0];
+        n1 = base + t->con[1];
+        n2 = base + t->con[2];
+        n3 = base + t->con[3];
+    }
+    scatter_lds_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
 }
 
 // Atomic scatter aggregated per wave by node: like scatter_lds, in a wave-private LDS table of
@@ -246,18 +258,25 @@ __device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets
 // record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
 // (pre-filled).  A stable sort on the node bits then leaves each node's weights in
 // ascending pid: the order of the serial MoveParticles fold.
-__device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
-                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
-                                            double l1, double l2, double l3, uint32_t base = 0) {
+__device__ __forceinline__ void put_records_nodes(uint32_t* __restrict__ keys, double* __restrict__ vals,
+                                                  int64_t pid, uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3,
+                                                  double l0, double l1, double l2, double l3) {
     const int64_t r = 4 * pid;
-    keys[r + 0] = base + (uint32_t)tf->con[0];
-    keys[r + 1] = base + (uint32_t)tf->con[1];
-    keys[r + 2] = base + (uint32_t)tf->con[2];
-    keys[r + 3] = base + (uint32_t)tf->con[3];
+    keys[r + 0] = n0;
+    keys[r + 1] = n1;
+    keys[r + 2] = n2;
+    keys[r + 3] = n3;
     vals[r + 0] = l0;
     vals[r + 1] = l1;
     vals[r + 2] = l2;
     vals[r + 3] = l3;
+}
+
+__device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
+                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
+                                            double l1, double l2, double l3, uint32_t base = 0) {
+    put_records_nodes(keys, vals, pid, base + (uint32_t)tf->con[0], base + (uint32_t)tf->con[1],
+                      base + (uint32_t)tf->con[2], base + (uint32_t)tf->con[3], l0, l1, l2, l3);
 }
 
 }  // namespace mpd

@@ -105,6 +105,7 @@ mp_options default_options() {
     o.walk_cap = 4096;
     o.scatter_agg = 2;
     o.sort_shift = 0;
+    o.stage_tets = 0;
     return o;
 }
 
@@ -226,7 +227,7 @@ bool good_options(const mp_options& o) {
            (o.threads_per_particle == 1 || (o.threads_per_particle == 4 && o.particles_per_thread == 1)) &&
            (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4) &&
            o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
-           o.sort_shift <= 16;
+           o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1);
 }
 
 }  // namespace
@@ -487,6 +488,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.block = c->opt.block_size;
     a.ppt = c->opt.particles_per_thread;
     a.tpp = c->opt.threads_per_particle;
+    a.stage = c->opt.stage_tets != 0;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));

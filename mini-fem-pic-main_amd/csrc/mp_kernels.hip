@@ -117,6 +117,114 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
         tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
+// ---- start tets staged in LDS -----------------------------------------------------------
+// The push is bound by the vector-memory address/data path, not by HBM bytes: every lane
+// gathers its start tet's 192 B record with 12 16-byte loads, and after the cell sort ~15
+// lanes of a workgroup ask for the same record.  Here the workgroup first collects its
+// distinct start cells in an LDS hash (kStageSlots entries), loads each record once with
+// consecutive lanes reading consecutive 16-byte chunks, and the particles read their start
+// tet from LDS.  Cells that find no free entry (more than kStageSlots distinct cells in the
+// workgroup) read it from global memory as before; walk steps into neighbours always do.
+// The particle state loads are issued before the staging barriers so both round trips
+// overlap.  AGG as in k_push_soa (0, 2 or 4).
+constexpr int kStageSlots = 64;
+constexpr int kTetChunks = (int)(sizeof(DevTet) / 16);
+
+template <int WALL, int DET, int AGG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
+k_push_staged(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
+              double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
+              int32_t* __restrict__ part, int cap) {
+    __shared__ DevTet stage[kStageSlots];
+    __shared__ int32_t skey[kStageSlots];
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    for (int q = threadIdx.x; q < kStageSlots; q += blockDim.x) skey[q] = -1;
+    const int32_t c0 = i < n ? s.cell[i] : -1;
+    double vx = 0.0, vy = 0.0, vz = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    if (c0 >= 0) {
+        vx = ld(s.vx + i); vy = ld(s.vy + i); vz = ld(s.vz + i);
+        x = ld(s.x + i); y = ld(s.y + i); z = ld(s.z + i);
+    }
+    __syncthreads();
+    int slot = -1;
+    if (c0 >= 0) {
+        uint32_t h = ((uint32_t)c0 * 0x9E3779B1u) >> (32 - 6);
+        for (int p = 0; p < kStageSlots; p++) {
+            const int32_t prev = atomicCAS(&skey[h], -1, c0);
+            if (prev == -1 || prev == c0) {
+                slot = (int)h;
+                break;
+            }
+            h = (h + 1) & (kStageSlots - 1);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kStageSlots * kTetChunks; q += blockDim.x) {
+        const int sl = q / kTetChunks, k = q - sl * kTetChunks;
+        const int32_t key = skey[sl];
+        if (key >= 0) reinterpret_cast<uint4*>(stage + sl)[k] = reinterpret_cast<const uint4*>(tets + key)[k];
+    }
+    __syncthreads();
+
+    bool act = false, lost = false, crossed = false, capped = false, ok = false;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+    int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+    WalkStats st{0, 0};
+    if (c0 >= 0) {
+        act = true;
+        StartTet g0;
+        double ex, ey, ez;
+        if (slot >= 0) {
+            const DevTet* t = stage + slot;
+            load_start(t, g0);
+            ex = t->ef[0]; ey = t->ef[1]; ez = t->ef[2];
+        } else {
+            const DevTet* t = tets + c0;
+            load_start(t, g0);
+            ex = t->ef[0]; ey = t->ef[1]; ez = t->ef[2];
+        }
+        vx = vx + (qm * ex) * dt;
+        vy = vy + (qm * ey) * dt;
+        vz = vz + (qm * ez) * dt;
+        x = x + vx * dt;
+        y = y + vy * dt;
+        z = z + vz * dt;
+        int32_t c = c0;
+        const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+        crossed = (st.moves - st.refl) > 0;
+        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+        if (r == kInside) {
+            s.cell[i] = c;
+            ok = true;
+            if (c == c0 && slot >= 0) {
+                const DevTet* t = stage + slot;
+                n0 = t->con[0]; n1 = t->con[1]; n2 = t->con[2]; n3 = t->con[3];
+            } else {
+                const DevTet* t = tets + c;
+                n0 = t->con[0]; n1 = t->con[1]; n2 = t->con[2]; n3 = t->con[3];
+            }
+            if (DET) put_records_nodes(keys, vals, s.pid[i], (uint32_t)n0, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3,
+                                       l0, l1, l2, l3);
+        } else {
+            s.cell[i] = -1;
+            lost = true;
+            capped = (r == kCapped);
+        }
+    }
+    if (!DET) {
+        if (AGG == 2) {
+            scatter_lds_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
+        } else if (ok) {
+            unsafeAtomicAdd(den + n0, l0);
+            unsafeAtomicAdd(den + n1, l1);
+            unsafeAtomicAdd(den + n2, l2);
+            unsafeAtomicAdd(den + n3, l3);
+        }
+    }
+    tally(part, act, lost, crossed, st.moves, capped, st.refl);
+}
+
 // ---- four threads per particle ------------------------------------------------------------
 // Lane v (0..3) of a quad owns vertex v of the current tet: it gathers that vertex's 32 B row
 // (alpha, beta, gamma, delta) instead of all 128 B, computes lc_v, and the quad exchanges the
@@ -764,6 +872,27 @@ static hipError_t launch_push_persist(const PushArgs& a, hipStream_t st) {
     return launch_tally_finish(a.part, g, a.ctr, st);
 }
 
+static hipError_t launch_push_staged(const PushArgs& a, hipStream_t st) {
+    const int bs = a.block;
+    const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
+#define MP_STAGED(W, D, G)                                                                              \
+    hipLaunchKernelGGL((k_push_staged<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
+                       a.den, a.keys, a.vals, a.part, a.cap)
+    if (a.wall == 0) {
+        if (a.det) MP_STAGED(0, 1, 0);
+        else if (a.agg == 0) MP_STAGED(0, 0, 0);
+        else MP_STAGED(0, 0, 2);
+    } else {
+        if (a.det) MP_STAGED(1, 1, 0);
+        else if (a.agg == 0) MP_STAGED(1, 0, 0);
+        else MP_STAGED(1, 0, 2);
+    }
+#undef MP_STAGED
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
 static hipError_t launch_push_tpp4(const PushArgs& a, hipStream_t st) {
     const int bs = a.block;
     const unsigned g = (grid_for(4 * a.n, bs) + kXcds - 1) / kXcds * kXcds;
@@ -791,6 +920,7 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
     const int ppt = a.ppt > 0 ? a.ppt : 1;
+    if (a.stage && ppt == 1 && (a.det || a.agg == 0 || a.agg == 2)) return launch_push_staged(a, st);
     const unsigned g = (grid_for(a.n, (int64_t)bs * ppt) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                                  \
     do {                                                                                                  \

@@ -36,6 +36,7 @@ struct PushArgs {
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
     int tpp;  // threads per particle: 1, or 4 (one lane per tet vertex)
+    bool stage;  // stage each workgroup's distinct start tets in LDS
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);

@@ -260,6 +260,32 @@ def test_particles_per_thread_matches_oracle(mp, orc, ppt, agg, block):
             assert c[key] == run.stats[key], key
 
 
+@pytest.mark.parametrize("stage", [0, 1], ids=["per-lane-gather", "lds-staged"])
+@pytest.mark.parametrize("sort_interval", [0, 3], ids=["unsorted", "sorted"])
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_start_tet_staging_matches_oracle(mp, orc, stage, sort_interval, wall):
+    """Start tets staged in LDS per workgroup or gathered per lane, on cell-sorted particles
+    and on unsorted ones (more distinct cells per workgroup than the staging table holds, so
+    some lanes take the global fallback): same bits, same counters, both density modes."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=27 + wall, n=90001)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(stage_tets=stage, sort_interval=sort_interval, wall_mode=wall)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(4):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+
+
 @pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
 @pytest.mark.parametrize("agg,block", [(0, 256), (2, 64), (2, 256)])
 def test_four_threads_per_particle_matches_oracle(mp, orc, wall, agg, block):
