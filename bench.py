@@ -303,7 +303,11 @@ def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
                         "timings include the particle upload (best of 3)"}
 
 
-def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30):
+def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30, sample=2_000_000):
+    """The oracle's restatement of the reference kernel body on the host: OpenMP over all
+    threads (atomic density adds) on the full workload until the budget is spent; one thread
+    on a sample; and one thread running the whole MoveParticles over 88 B Particle records
+    (copy in, kernel body, stable erase of the lost, normalisation, copy out) on the sample."""
     import oracle
 
     tets = mesh.tets
@@ -319,7 +323,28 @@ def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30):
         if time.perf_counter() - t0 > budget_s:
             break
     wall = time.perf_counter() - t0
-    return {"value": done / wall, "steps": steps, "threads": threads, "wall_s": wall}
+    out = {"value": done / wall, "steps": steps, "threads": threads, "wall_s": wall}
+    # one thread, resident arrays
+    p, v, c = pos[:sample].copy(), vel[:sample].copy(), cell[:sample].copy()
+    t0 = time.perf_counter()
+    n1 = 0
+    for _ in range(2):
+        n1 += int((c >= 0).sum())
+        oracle.move(tets, ef, p, v, c, QM, DT, mesh.n_nodes)
+    out["single_thread_value"] = n1 / (time.perf_counter() - t0)
+    # one thread, the whole MoveParticles over Particle records
+    mover = oracle.AosMover(tets, mesh.node_volume, QM, DT, SPWT)
+    parts = mp.particles_from_arrays(pos[:sample], vel[:sample], cell[:sample])
+    t0 = time.perf_counter()
+    n2 = 0
+    for _ in range(2):
+        n2 += len(parts)
+        dev = parts.copy()                     # the host -> device copy
+        dev, den = mover(dev, ef)
+        parts = dev.copy()                     # the device -> host copy
+    out["full_moveparticles_value"] = n2 / (time.perf_counter() - t0)
+    out["sample"] = sample
+    return out
 
 
 def main():
@@ -386,7 +411,11 @@ def main():
         cpu = {"value": cb["value"], "unit": "particle-steps/s", "cores": cb["threads"], "kind": "port",
                "sample": f"the same 20M particles and mesh, {cb['steps']} steps from the same initial state "
                          f"({cb['wall_s']:.1f} s), OpenMP port of the reference kernel body with atomic "
-                         f"density adds on {cb['threads']} threads of {cpu_model()}"}
+                         f"density adds on {cb['threads']} threads of {cpu_model()}",
+               "single_thread_value": cb["single_thread_value"],
+               "full_moveparticles_1thread_value": cb["full_moveparticles_value"],
+               "single_thread_sample": f"first {cb['sample']} particles, 2 steps; the full MoveParticles line adds "
+                                       "the 2 x 88 B record copies, the stable erase and the normalisation"}
         log(f"cpu baseline: {cb['value'] / 1e6:.2f} M particle-steps/s on {cb['threads']} threads")
     c = r["counters"]
     out = {
