@@ -132,13 +132,14 @@ __device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const
     const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
                      (l3 < 0 || l3 > 1.0);
     if (!out) return kInside;
+    if (cap <= 0) return kCapped;  // a caller's way to evaluate the start tet only
     int mi = 0;
     double ml = l0;
     if (l1 < ml) { ml = l1; mi = 1; }
     if (l2 < ml) { ml = l2; mi = 2; }
     if (l3 < ml) { ml = l3; mi = 3; }
     const int32_t nb = mi == 0 ? s.n0 : mi == 1 ? s.n1 : mi == 2 ? s.n2 : s.n3;
-    if (nb >= 0) {  // cap >= 1, so the first move is never capped
+    if (nb >= 0) {  // cap >= 1 here, so the first move is never capped
         st.moves = 1;
         c = nb;
         return walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);

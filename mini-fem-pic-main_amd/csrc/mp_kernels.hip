@@ -61,28 +61,35 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
         int32_t fc = -1;
         double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
         if (i < n) {
-            int32_t c = s.cell[i];
-            if (c >= 0) {
+            // The particle's state is loaded with its cell, not after it: only the start tet
+            // depends on the cell.  Nothing branches on the cell before the walk (a lost slot,
+            // cell < 0, evaluates tet 0 with a cap of 0 and its results are dropped), so the
+            // compiler cannot sink the state or start-tet loads behind a wait.
+            const int32_t c_in = s.cell[i];
+            double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
+            const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+            // keep the six state loads issued ahead of the cell's first use (the scheduler
+            // would otherwise place them after the wait for the cell)
+            __builtin_amdgcn_sched_barrier(0);
+            const bool live = c_in >= 0;
+            int32_t c = live ? c_in : 0;
+            const DevTet* t = tets + c;
+            StartTet g0;
+            load_start(t, g0);
+            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+            vx = vx + (qm * ex) * dt;
+            vy = vy + (qm * ey) * dt;
+            vz = vz + (qm * ez) * dt;
+            double x = x0 + vx * dt;
+            double y = y0 + vy * dt;
+            double z = z0 + vz * dt;
+            WalkStats st{0, 0};
+            const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+            if (live) {
                 n_act++;
-                const DevTet* t = tets + c;
-                StartTet g0;
-                load_start(t, g0);
-                const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-                double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
-                const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
-                vx = vx + (qm * ex) * dt;
-                vy = vy + (qm * ey) * dt;
-                vz = vz + (qm * ez) * dt;
-                double x = x0 + vx * dt;
-                double y = y0 + vy * dt;
-                double z = z0 + vz * dt;
-                WalkStats st{0, 0};
-                const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
                 n_cross += (st.moves - st.refl) > 0;
                 n_moves += st.moves;
                 n_refl += st.refl;
-                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
                 if (r == kInside) {
                     s.cell[i] = c;
                     ok = true;
@@ -93,6 +100,8 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                     n_lost++;
                     n_cap += (r == kCapped);
                 }
+                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
         }
         if (!DET) {
