@@ -237,15 +237,31 @@ def run_coupled(steps=200):
             a = time.perf_counter()
             run.inject()
             b = time.perf_counter()
-            psteps += len(run.parts)
+            psteps += run.n_parts
             run.move_step()
             c = time.perf_counter()
             run.solve()
             t_inj, t_move, t_fe = t_inj + b - a, t_move + c - b, t_fe + time.perf_counter() - c
         wall = time.perf_counter() - t0
-        out = {"steps": steps, "particles_final": len(run.parts), "ms_per_step": 1e3 * wall / steps,
+        out = {"steps": steps, "particles_final": run.n_parts, "ms_per_step": 1e3 * wall / steps,
                "inject_ms_per_step": 1e3 * t_inj / steps, "move_ms_per_step": 1e3 * t_move / steps,
                "fe_ms_per_step": 1e3 * t_fe / steps, "move_particle_steps_per_s": psteps / t_move}
+        # the same loop with the ions resident on the device
+        res = cp.ResidentGpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
+        try:
+            rrun = cp.CoupledRun(mesh, p, move=res, fe_flags=mp.FE_RESET_F1)
+            t_move_r = 0.0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                rrun.inject()
+                c = time.perf_counter()
+                rrun.move_step()
+                t_move_r += time.perf_counter() - c
+                rrun.solve()
+            out["resident_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+            out["resident_move_ms_per_step"] = 1e3 * t_move_r / steps
+        finally:
+            res.close()
         run = cp.CoupledRun(mesh, p, move=gpu)
         fail = None
         for ts in range(steps):
@@ -258,6 +274,7 @@ def run_coupled(steps=200):
     finally:
         gpu.close()
     log(f"coupled: {out['ms_per_step']:.2f} ms/step (move {out['move_ms_per_step']:.2f}, FE {out['fe_ms_per_step']:.2f}),"
+        f" resident {out['resident_ms_per_step']:.2f} ms/step (move {out['resident_move_ms_per_step']:.2f}),"
         f" {out['particles_final']} ions at the end; reference-F1 Newton fails at step {out['reference_solver_newton_fails_at_step']}")
     out["workload"] = ("12^3 six-tet box, side 0.1 m, O+ beam 7 km/s from x = 0, n 1e10 m^-3, Te 2 eV, dt 5e-7 s, "
                        "GPU drop-in MoveParticles (deterministic) + host FE Newton/Gauss-Seidel, F1 cleared per solve")

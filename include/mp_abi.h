@@ -123,7 +123,9 @@ int  mp_step_aos(mp_ctx* ctx, mp_particle* parts, size_t* n_inout, double q_over
  * are 0..n-1 in array order; pid order is the order a serial MoveParticles sees. */
 int  mp_set_particles(mp_ctx* ctx, const double* pos, const double* vel,
                       const int32_t* cell, int64_t n);
-/* Append particles (injection); they get the next pids. */
+/* Append particles (injection); they get the next pids.  A batch larger than an eighth of
+ * the resident set is cell-sorted in before the next step; smaller ones join the next
+ * periodic sort. */
 int  mp_add_particles(mp_ctx* ctx, const double* pos, const double* vel,
                       const int32_t* cell, int64_t n);
 /* One MoveParticles step on the resident set: push + walk + scatter, then the nodal

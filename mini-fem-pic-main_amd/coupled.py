@@ -134,8 +134,39 @@ class GpuMover:
         self.mv.close()
 
 
+class ResidentGpuMover:
+    """MoveParticles with the ions kept in HBM between steps: each step only the newly
+    injected ions cross PCIe (mp_add_particles, which gives them the next pids, so the
+    deterministic density keeps the serial order), the field is uploaded and the density
+    read back."""
+
+    resident = True
+
+    def __init__(self, mesh, q_over_m, dt, spwt, mode=mp.SCATTER_DETERMINISTIC, sort_interval=10):
+        self.mv = mp.Mover.from_mesh(mesh)
+        self.mv.set_options(sort_interval=sort_interval)
+        self.mv.set_particles(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32))
+        self.q_over_m, self.dt, self.spwt, self.mode = q_over_m, dt, spwt, mode
+
+    def __call__(self, new_parts, ef):
+        if len(new_parts):
+            self.mv.add_particles(new_parts["pos"], new_parts["vel"], new_parts["cell_index"])
+        self.mv.upload_field(ef)
+        self.mv.step(self.q_over_m, self.dt, self.spwt, self.mode)
+        return self.mv.num_active(), self.mv.density()
+
+    def particles(self):
+        """(pos, vel, cell, pid) of the ions in the domain, ascending pid."""
+        return self.mv.get_particles()
+
+    def close(self):
+        self.mv.close()
+
+
 class CoupledRun:
-    """The main loop's state: particles, carried injection fraction, field and solver."""
+    """The main loop's state: particles, carried injection fraction, field and solver.  With a
+    resident mover the particles live on the device and `parts` holds only the ions injected
+    this step."""
 
     def __init__(self, mesh, params, move=None, mode=mp.SCATTER_DETERMINISTIC, seed=0, spwt=2e2,
                  charge=mp.QE, mass=16 * mp.AMU, fe_flags=0):
@@ -146,7 +177,9 @@ class CoupledRun:
                               wall_potential=params.wall_potential, flags=fe_flags)
         self.rng = mp.Rng(seed)
         self.move = move if move is not None else GpuMover(mesh, self.q_over_m, params.dt, spwt, mode)
+        self.resident = getattr(self.move, "resident", False)
         self.parts = np.zeros(0, mp.PARTICLE_DTYPE)
+        self.n_parts = 0
         self.rem = 0.0
         self.ef = np.zeros((mesh.n_tets, 3))  # zero until the first solve
         self.den = np.zeros(mesh.n_nodes)
@@ -156,13 +189,21 @@ class CoupledRun:
         p = self.params
         new, self.rem = mp.inject_ions(self.mesh, self.rng, p.plasma_den, p.ion_velocity, p.dt, self.spwt,
                                        self.q_over_m, self.ef, self.rem)
-        self.parts = np.concatenate([self.parts, new]) if len(self.parts) else new.copy()
+        if self.resident:
+            self.parts = new
+        else:
+            self.parts = np.concatenate([self.parts, new]) if len(self.parts) else new.copy()
+        self.n_parts += len(new)
         return len(new)
 
     def move_step(self):
-        old = len(self.parts)
+        old = self.n_parts
         t0 = time.perf_counter()
-        self.parts, self.den = self.move(self.parts, self.ef)
+        if self.resident:
+            self.n_parts, self.den = self.move(self.parts, self.ef)
+        else:
+            self.parts, self.den = self.move(self.parts, self.ef)
+            self.n_parts = len(self.parts)
         return old, time.perf_counter() - t0
 
     def solve(self, den=None):
@@ -176,7 +217,7 @@ class CoupledRun:
         old, duration = self.move_step()
         st = self.solve()
         phi = self.fe.phi
-        rec = {"ts": self.ts, "np": len(self.parts), "added": added, "removed": old - len(self.parts),
+        rec = {"ts": self.ts, "np": self.n_parts, "added": added, "removed": old - self.n_parts,
                "max_den": float(max(self.den.max(initial=0.0), 0.0)), "max_phi": float(np.abs(phi).max()),
                "move_s": duration, "nr_iterations": st["nr_iterations"], "gs_sweeps": st["gs_sweeps"]}
         self.ts += 1

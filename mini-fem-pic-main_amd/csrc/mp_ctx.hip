@@ -307,7 +307,8 @@ void mp_destroy(mp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
-                    c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally};
+                    c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
+                    c->inj_tmp};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -430,9 +431,19 @@ int mp_add_particles(mp_ctx* c, const double* pos, const double* vel, const int3
     int rc = reserve_soa(c, c->n_slots + n, true);
     if (rc) return rc;
     if (n > 0) {
-        char* tmp = nullptr;
+        // staging buffer for the host arrays, kept for the next injection
         const size_t bp = sizeof(double) * 3 * (size_t)n, bc = sizeof(int32_t) * (size_t)n;
-        HIP_TRY(c, hipMalloc((void**)&tmp, 2 * bp + bc));
+        if (2 * bp + bc > c->inj_bytes) {
+            dfree(c->inj_tmp);
+            c->inj_tmp = nullptr;
+            c->inj_bytes = 0;
+            if (hipMalloc(&c->inj_tmp, 2 * (2 * bp + bc)) != hipSuccess) {
+                (void)hipGetLastError();
+                return set_err(c, MP_ERR_ALLOC, "mp_add_particles: staging buffer allocation failed");
+            }
+            c->inj_bytes = 2 * (2 * bp + bc);
+        }
+        char* tmp = (char*)c->inj_tmp;
         hipError_t e = hipMemcpyAsync(tmp, pos, bp, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(tmp + bp, vel, bp, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(tmp + 2 * bp, cell, bc, hipMemcpyHostToDevice, c->stream);
@@ -440,12 +451,13 @@ int mp_add_particles(mp_ctx* c, const double* pos, const double* vel, const int3
             e = mpd::launch_unpack((double*)tmp, (double*)(tmp + bp), (int32_t*)(tmp + 2 * bp), n, c->next_pid,
                                    c->soa[c->cur], c->n_slots, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        dfree(tmp);
         if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_add_particles: ") + hipGetErrorString(e));
     }
     c->n_slots += n;
     c->next_pid += n;
-    c->since_sort = c->opt.sort_interval;  // sort before the next step
+    // a large batch (over an eighth of the set) is sorted in before the next step; small
+    // injections join the next periodic sort
+    if (8 * n > c->n_slots) c->since_sort = c->opt.sort_interval;
     return MP_OK;
 }
 

@@ -47,6 +47,9 @@ struct mp_ctx {
     int32_t* offs = nullptr;
     int64_t aos_cap = 0;
     int32_t* pinned_i32 = nullptr;
+    // staging of injected particles (mp_add_particles)
+    void* inj_tmp = nullptr;
+    size_t inj_bytes = 0;
     // per-block tallies of the push kernels
     int32_t* tally = nullptr;
     int64_t tally_n = 0;

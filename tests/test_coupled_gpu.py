@@ -39,7 +39,7 @@ def _lockstep(run, cpu, steps, done):
         before = run.parts.copy()
         run.move_step()
         ref_parts, ref_den = cpu(before, run.ef)
-        assert len(run.parts) == len(ref_parts), f"step {ts}: particle count"
+        assert run.n_parts == len(ref_parts), f"step {ts}: particle count"
         assert run.parts.tobytes() == ref_parts.tobytes(), f"step {ts}: particles differ"
         assert run.den.tobytes() == ref_den.tobytes(), f"step {ts}: density differs"
         run.solve()
@@ -53,7 +53,7 @@ def test_coupled_loop_200_steps_bit_exact(mp, orc, cp, tmp_path):
     try:
         done = []
         _lockstep(run, cpu, 200, done)
-        assert len(done) == 200 and 45000 < len(run.parts) < 52000
+        assert len(done) == 200 and 45000 < run.n_parts < 52000
     finally:
         gpu.close()
 
@@ -68,5 +68,34 @@ def test_coupled_loop_reference_solver_fails_alike(mp, orc, cp, tmp_path):
         with pytest.raises(mp.MpError, match="NR failed to converge"):
             _lockstep(run, cpu, 60, done)
         assert 10 < len(done) < 60
+    finally:
+        gpu.close()
+
+
+def test_coupled_loop_resident_mover_200_steps(mp, orc, cp, tmp_path):
+    """The same loop with the ions resident in HBM (only the injected ions cross PCIe, cell
+    sort every 10 steps): density bit-exact every step, and the ions themselves (ascending pid
+    = the serial vector's order) bit-exact every 25 steps."""
+    mesh = cp.box_case(12, 0.1, str(tmp_path))
+    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                      dt=5e-7, max_iter=200)
+    gpu = cp.ResidentGpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC, sort_interval=10)
+    run = cp.CoupledRun(mesh, p, move=gpu, fe_flags=mp.FE_RESET_F1)
+    cpu = orc.AosMover(mesh.tets, mesh.node_volume, QM, p.dt, 2e2)
+    ref = np.zeros(0, mp.PARTICLE_DTYPE)
+    try:
+        for ts in range(200):
+            run.inject()
+            ref = np.concatenate([ref, run.parts])
+            run.move_step()
+            ref, ref_den = cpu(ref, run.ef)
+            assert run.n_parts == len(ref), f"step {ts}: particle count"
+            assert run.den.tobytes() == ref_den.tobytes(), f"step {ts}: density differs"
+            if ts % 25 == 24:
+                gp, gv, gc, gid = gpu.particles()
+                assert np.array_equal(gc, ref["cell_index"]), f"step {ts}: cells differ"
+                assert gp.tobytes() == ref["pos"].tobytes() and gv.tobytes() == ref["vel"].tobytes()
+            run.solve()
+        assert 45000 < run.n_parts < 52000
     finally:
         gpu.close()
