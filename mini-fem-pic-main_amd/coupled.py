@@ -169,8 +169,9 @@ class CoupledRun:
     this step."""
 
     def __init__(self, mesh, params, move=None, mode=mp.SCATTER_DETERMINISTIC, seed=0, spwt=2e2,
-                 charge=mp.QE, mass=16 * mp.AMU, fe_flags=0):
+                 charge=mp.QE, mass=16 * mp.AMU, fe_flags=0, output_dir=None):
         self.mesh, self.params = mesh, params
+        self.output_dir = output_dir
         self.spwt = spwt
         self.q_over_m = charge / mass
         self.fe = mp.FESolver(mesh, n0=params.plasma_den, phi0=0.0, kTe=mp.KB * params.electron_temperature,
@@ -220,5 +221,21 @@ class CoupledRun:
         rec = {"ts": self.ts, "np": self.n_parts, "added": added, "removed": old - self.n_parts,
                "max_den": float(max(self.den.max(initial=0.0), 0.0)), "max_phi": float(np.abs(phi).max()),
                "move_s": duration, "nr_iterations": st["nr_iterations"], "gs_sweeps": st["gs_sweeps"]}
+        if self.output_dir and (self.ts + 1) % 10 == 0:
+            self.write_mesh(self.ts)
         self.ts += 1
         return rec
+
+    def write_mesh(self, ts):
+        """OutputMesh(ts, ...): mesh_{ts+1:04d}.vtu in the output directory."""
+        path = os.path.join(self.output_dir, f"mesh_{ts + 1:04d}.vtu")
+        mp.write_mesh_vtu(self.mesh, path, self.fe.phi, self.ef, self.den)
+        return path
+
+    def finish(self):
+        """The end of main (src/mini-pic.cpp:160-164): the last mesh file and particles.vtp."""
+        if not self.output_dir:
+            return
+        self.write_mesh(self.ts)
+        pos = self.move.particles()[0] if self.resident else self.parts
+        mp.write_particles_vtp(os.path.join(self.output_dir, "particles.vtp"), pos)

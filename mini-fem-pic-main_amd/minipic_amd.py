@@ -128,6 +128,10 @@ SIGNATURES = {
     "mp_fe_get_phi": (C.c_int, [_P, _P]),
     "mp_fe_get_stiffness": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P]),
     "mp_fe_last_error": (C.c_char_p, []),
+    # mp_output.h
+    "mp_write_mesh_vtu": (C.c_int, [_P, C.c_char_p, _P, _P, _P]),
+    "mp_write_particles_vtp": (C.c_int, [C.c_char_p, _P, _I64, _I64]),
+    "mp_output_last_error": (C.c_char_p, []),
     # mp_inject.h
     "mp_rng_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
     "mp_rng_free": (None, [_P]),
@@ -380,6 +384,25 @@ class Rng:
         if rc != MP_OK:
             raise MpError(rc, "mp_rng_uniform failed")
         return out
+
+
+def write_mesh_vtu(mesh, path, phi, ef, ion_den):
+    """OutputMesh (src/meshes.cpp:360-449) to path."""
+    phi, ef, den = _f64(phi), _f64(ef, (-1, 3)), _f64(ion_den)
+    rc = lib().mp_write_mesh_vtu(mesh._h, str(path).encode(), _ptr(phi), _ptr(ef), _ptr(den))
+    if rc != MP_OK:
+        raise MpError(rc, lib().mp_output_last_error().decode())
+
+
+def write_particles_vtp(path, pos):
+    """OutputParticles (src/particles.cpp:20-45): pos is [n][3] or Particle records."""
+    if pos.dtype == PARTICLE_DTYPE:
+        arr, stride = np.ascontiguousarray(pos), PARTICLE_DTYPE.itemsize
+    else:
+        arr, stride = _f64(pos, (-1, 3)), 24
+    rc = lib().mp_write_particles_vtp(str(path).encode(), _ptr(arr), len(arr), stride)
+    if rc != MP_OK:
+        raise MpError(rc, lib().mp_output_last_error().decode())
 
 
 def inject_ions(mesh, rng, plasma_den, ion_velocity, dt, spwt, q_over_m, ef, rem):

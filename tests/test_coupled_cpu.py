@@ -165,3 +165,42 @@ def test_coupled_loop_on_host(mp, orc, cp, tmp_path):
     with pytest.raises(mp.MpError, match="NR failed to converge"):
         for _ in range(60):
             run.step()
+
+
+def test_vtk_output_layout(mp, orc, cp, tmp_path):
+    """OutputMesh / OutputParticles files of a short coupled run: the reference's VTK layout
+    (ASCII UnstructuredGrid with node_index, node_type, phi, ion_den, ef, cell_volume; PolyData
+    of positions), every number written as the C++ stream writes a double by default
+    (printf %g: six significant digits)."""
+    import xml.etree.ElementTree as ET
+
+    mesh = cp.box_case(3, 0.1, str(tmp_path))
+    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                      dt=5e-7, max_iter=10)
+    out = tmp_path / "out"
+    out.mkdir()
+    run = cp.CoupledRun(mesh, p, move=orc.AosMover(mesh.tets, mesh.node_volume, QM, p.dt, 2e2),
+                        fe_flags=mp.FE_RESET_F1, output_dir=str(out))
+    for _ in range(10):
+        run.step()
+    run.finish()
+    assert sorted(os.listdir(out)) == ["mesh_0010.vtu", "mesh_0011.vtu", "particles.vtp"]
+    text = (out / "mesh_0011.vtu").read_text()
+    root = ET.fromstring(text)
+    piece = root.find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfPoints")) == mesh.n_nodes and int(piece.get("NumberOfCells")) == mesh.n_tets
+    arrays = {a.get("Name"): a.text.split() for a in root.iter("DataArray")}
+    assert arrays["connectivity"] == [str(v) for v in mesh.tets["con"].ravel()]
+    assert arrays["offsets"] == [str(4 * (e + 1)) for e in range(mesh.n_tets)]
+    assert arrays["types"] == ["10"] * mesh.n_tets
+    assert arrays["node_index"] == [str(n) for n in range(mesh.n_nodes)]
+    assert arrays["node_type"] == [str(t) for t in mesh.node_type]
+    assert arrays["phi"] == ["%g" % v for v in run.fe.phi]
+    assert arrays["ion_den"] == ["%g" % v for v in run.den]
+    assert arrays["ef"] == ["%g" % v for v in run.ef.ravel()]
+    assert arrays["cell_volume"] == ["%g" % v for v in mesh.tets["volume"]]
+    pts = root.find("UnstructuredGrid/Piece/Points/DataArray").text.split()
+    assert pts == ["%g" % v for v in mesh.node_pos.ravel()]
+    proot = ET.fromstring((out / "particles.vtp").read_text())
+    assert int(proot.find("PolyData/Piece").get("NumberOfPoints")) == len(run.parts)
+    assert proot.find("PolyData/Piece/Points/DataArray").text.split() == ["%g" % v for v in run.parts["pos"].ravel()]
