@@ -117,6 +117,62 @@ def box_case(n_hex, side, workdir, wall=False):
     return mesh
 
 
+class Trace:
+    """Per-phase wall-time profile in the reference's trace.csv format (src/trace.cpp:36-78):
+    function_name, total_time, exclusive_time, exclusive_proportion, number_of_calls, then the
+    traceEnter / traceExit / traceTotalTime rows; numbers as the C++ stream writes them."""
+
+    def __init__(self):
+        self.start = time.perf_counter()
+        self.funcs = {}
+        self.stack = []
+        self.entry_time = self.exit_time = 0.0
+
+    def enter(self, name):
+        t = time.perf_counter()
+        f = self.funcs.setdefault(name, {"calls": 0, "total": 0.0, "child": 0.0, "start": 0.0})
+        f["calls"] += 1
+        f["start"] = t
+        self.stack.append(name)
+        self.entry_time += time.perf_counter() - t
+
+    def exit(self, name):
+        t = time.perf_counter()
+        f = self.funcs[name]
+        dt = t - f["start"]
+        f["total"] += dt
+        f["start"] = 0.0
+        self.stack.pop()
+        if self.stack:
+            self.funcs[self.stack[-1]]["child"] += dt
+        self.exit_time += time.perf_counter() - t
+
+    def phase(self, name):
+        tr = self
+
+        class _Phase:
+            def __enter__(self):
+                tr.enter(name)
+
+            def __exit__(self, *exc):
+                tr.exit(name)
+
+        return _Phase()
+
+    def write_profile(self, path):
+        calc = time.perf_counter() - self.start
+        g = lambda v: "%g" % v  # noqa: E731
+        with open(path, "w") as f:
+            f.write("function_name,total_time,exclusive_time,exclusive_proportion,number_of_calls\n")
+            for name, e in self.funcs.items():
+                if not e["start"]:
+                    excl = e["total"] - e["child"]
+                    f.write(f"{name},{g(e['total'])},{g(excl)},{g(excl / calc)},{e['calls']}\n")
+            f.write(f"traceEnter,{g(self.entry_time)},{g(self.entry_time)},{g(self.entry_time / calc)},1\n")
+            f.write(f"traceExit,{g(self.exit_time)},{g(self.exit_time)},{g(self.entry_time / calc)},1\n")
+            f.write(f"traceTotalTime,{g(calc)},0,0,1\n")
+
+
 class GpuMover:
     """MoveParticles on the MI355X (drop-in mp_step_aos: field copied every step, survivors
     compacted in order, den normalised)."""
@@ -172,6 +228,7 @@ class CoupledRun:
                  charge=mp.QE, mass=16 * mp.AMU, fe_flags=0, output_dir=None):
         self.mesh, self.params = mesh, params
         self.output_dir = output_dir
+        self.trace = Trace()
         self.spwt = spwt
         self.q_over_m = charge / mass
         self.fe = mp.FESolver(mesh, n0=params.plasma_den, phi0=0.0, kTe=mp.KB * params.electron_temperature,
@@ -214,15 +271,21 @@ class CoupledRun:
 
     def step(self):
         """One time step; returns the reference's per-step report as a dict."""
-        added = self.inject()
-        old, duration = self.move_step()
-        st = self.solve()
+        with self.trace.phase("InjectIons"):
+            added = self.inject()
+        with self.trace.phase("MoveParticles"):
+            old, duration = self.move_step()
+        with self.trace.phase("computePhi"):
+            st = self.fe.compute_phi(self.den, self.params.fesolver_method)
+        with self.trace.phase("updateEf"):
+            self.ef = self.fe.update_ef()
         phi = self.fe.phi
         rec = {"ts": self.ts, "np": self.n_parts, "added": added, "removed": old - self.n_parts,
                "max_den": float(max(self.den.max(initial=0.0), 0.0)), "max_phi": float(np.abs(phi).max()),
                "move_s": duration, "nr_iterations": st["nr_iterations"], "gs_sweeps": st["gs_sweeps"]}
         if self.output_dir and (self.ts + 1) % 10 == 0:
-            self.write_mesh(self.ts)
+            with self.trace.phase("OutputMesh"):
+                self.write_mesh(self.ts)
         self.ts += 1
         return rec
 
@@ -233,9 +296,13 @@ class CoupledRun:
         return path
 
     def finish(self):
-        """The end of main (src/mini-pic.cpp:160-164): the last mesh file and particles.vtp."""
+        """The end of main (src/mini-pic.cpp:160-164): the last mesh file, particles.vtp and
+        the phase profile trace.csv."""
         if not self.output_dir:
             return
-        self.write_mesh(self.ts)
-        pos = self.move.particles()[0] if self.resident else self.parts
-        mp.write_particles_vtp(os.path.join(self.output_dir, "particles.vtp"), pos)
+        with self.trace.phase("OutputMesh"):
+            self.write_mesh(self.ts)
+        with self.trace.phase("OutputParticles"):
+            pos = self.move.particles()[0] if self.resident else self.parts
+            mp.write_particles_vtp(os.path.join(self.output_dir, "particles.vtp"), pos)
+        self.trace.write_profile(os.path.join(self.output_dir, "trace.csv"))

@@ -184,7 +184,13 @@ def test_vtk_output_layout(mp, orc, cp, tmp_path):
     for _ in range(10):
         run.step()
     run.finish()
-    assert sorted(os.listdir(out)) == ["mesh_0010.vtu", "mesh_0011.vtu", "particles.vtp"]
+    assert sorted(os.listdir(out)) == ["mesh_0010.vtu", "mesh_0011.vtu", "particles.vtp", "trace.csv"]
+    rows = [line.split(",") for line in (out / "trace.csv").read_text().splitlines()]
+    assert rows[0] == ["function_name", "total_time", "exclusive_time", "exclusive_proportion", "number_of_calls"]
+    calls = {r[0]: int(r[4]) for r in rows[1:]}
+    assert calls["InjectIons"] == calls["MoveParticles"] == calls["computePhi"] == calls["updateEf"] == 10
+    assert calls["OutputMesh"] == 2 and calls["OutputParticles"] == 1
+    assert [r[0] for r in rows[-3:]] == ["traceEnter", "traceExit", "traceTotalTime"]
     text = (out / "mesh_0011.vtu").read_text()
     root = ET.fromstring(text)
     piece = root.find("UnstructuredGrid/Piece")
