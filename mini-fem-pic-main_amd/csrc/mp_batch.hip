@@ -17,8 +17,10 @@
 
 namespace mpd {
 
+// The push of k_push_soa (same prologue: state, run and cell loaded together, no branch on
+// the cell before the start tet is evaluated) with the run's q/m and dt from the run table.
 template <int WALL, int DET>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
              const int64_t* __restrict__ n_dev, const double* __restrict__ par, int32_t nn,
              double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
@@ -30,26 +32,29 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     WalkStats st{0, 0};
     if (i < n) {
-        int32_t c = s.cell[i];
-        if (c >= 0) {
+        const int32_t c_in = s.cell[i];
+        const int32_t r = s.run[i];
+        double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
+        const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool live = c_in >= 0;
+        int32_t c = live ? c_in : 0;
+        const double qm = par[3 * r], dt = par[3 * r + 1];
+        const DevTet* t = tets + c;
+        StartTet g0;
+        load_start(t, g0);
+        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+        vx = vx + (qm * ex) * dt;
+        vy = vy + (qm * ey) * dt;
+        vz = vz + (qm * ez) * dt;
+        double x = x0 + vx * dt;
+        double y = y0 + vy * dt;
+        double z = z0 + vz * dt;
+        const int rr = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+        if (live) {
             act = true;
-            const int32_t r = s.run[i];
-            const double qm = par[3 * r], dt = par[3 * r + 1];
             base = r * nn;
-            const DevTet* t = tets + c;
-            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-            double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
-            const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
-            vx = vx + (qm * ex) * dt;
-            vy = vy + (qm * ey) * dt;
-            vz = vz + (qm * ez) * dt;
-            double x = x0 + vx * dt;
-            double y = y0 + vy * dt;
-            double z = z0 + vz * dt;
-            const int rr = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             crossed = (st.moves - st.refl) > 0;
-            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             if (rr == kInside) {
                 s.cell[i] = c;
                 ok = true;
@@ -60,6 +65,11 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
                 lost = true;
                 capped = (rr == kCapped);
             }
+            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+        } else {
+            st.moves = 0;
+            st.refl = 0;
         }
     }
     if (!DET) scatter_lds(tets, den, ok, fc, l0, l1, l2, l3, base);
