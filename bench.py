@@ -70,13 +70,14 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
 
 
 def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
-            tpp=1, tune=False, stage=0):
+            tpp=1, tune=False, stage=0, pid_order=0):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
     tuned = None
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block,
-                       particles_per_thread=ppt, threads_per_particle=tpp, stage_tets=stage)
+                       particles_per_thread=ppt, threads_per_particle=tpp, stage_tets=stage,
+                       keep_pid_order=pid_order)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         if tune:
@@ -382,6 +383,7 @@ def main():
     ap.add_argument("--tpp", type=int, default=1, help="threads per particle (1, 4)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use the options given)")
     ap.add_argument("--stage", type=int, default=0, help="stage start tets in LDS (1) or gather per lane (0)")
+    ap.add_argument("--det-pid-order", type=int, default=0, help="deterministic line: keep pid order (1) or cell sort (0)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
@@ -410,7 +412,7 @@ def main():
     extra = {}
     if args.det_steps > 0 and mode == mp.SCATTER_ATOMIC:
         d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval,
-                    shift=args.sort_shift, tune=not args.no_tune)
+                    shift=args.sort_shift, tune=not args.no_tune, pid_order=args.det_pid_order)
         extra["deterministic"] = {"value": d["rate"], "ms_per_step": 1e3 * d["wall_s"] / args.det_steps,
                                   "steps": args.det_steps, "options": d["options"]}
         log(f"gpu deterministic: {d['rate'] / 1e9:.3f} G particle-steps/s")

@@ -78,6 +78,10 @@ typedef struct mp_options {
                                       LDS once (TPP=1, PPT=1, aggregation 0 or 2, or the
                                       deterministic scatter); 0 (default): every lane
                                       gathers its own */
+    int32_t keep_pid_order;        /* 1: the periodic pass is a stable compaction that keeps the
+                                      particles in pid order instead of a cell sort; the
+                                      deterministic records are then written contiguously by
+                                      slot.  0 (default): cell sort */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
@@ -143,7 +147,8 @@ int  mp_get_particles(mp_ctx* ctx, double* pos, double* vel, int32_t* cell, int6
 int  mp_get_density(mp_ctx* ctx, double* den);
 int  mp_get_counters(mp_ctx* ctx, mp_counters* out);
 int  mp_reset_counters(mp_ctx* ctx);
-/* Sort the resident particles by cell and drop the lost ones now. */
+/* Sort the resident particles by cell (or, with keep_pid_order, compact them in pid order)
+ * and drop the lost ones now. */
 int  mp_sort_now(mp_ctx* ctx);
 /* The context's hipStream_t, for callers that capture or time its work. */
 void* mp_stream(mp_ctx* ctx);

@@ -230,10 +230,12 @@ extern "C" int mp_adjoint(mp_ctx* c, const double* pos0, const double* vel0, con
     if (rc) return rc;
     // one cell sort for locality (pids are irrelevant to the summed gradients)
     {
-        const int32_t keep = c->opt.sort_interval;
+        const int32_t keep = c->opt.sort_interval, keep_order = c->opt.keep_pid_order;
         c->opt.sort_interval = 1;
+        c->opt.keep_pid_order = 0;
         rc = mp_sort_now(c);
         c->opt.sort_interval = keep;
+        c->opt.keep_pid_order = keep_order;
         if (rc) return rc;
     }
     const int64_t m = c->n_slots;

@@ -106,6 +106,7 @@ mp_options default_options() {
     o.scatter_agg = 2;
     o.sort_shift = 0;
     o.stage_tets = 0;
+    o.keep_pid_order = 0;
     return o;
 }
 
@@ -199,8 +200,51 @@ int det_finish(mp_ctx* c, int64_t m, double spwt) {
 
 // Counting sort of the resident particles by bucket = cell >> sort_shift, dropping the lost
 // ones; order inside a bucket is whatever the atomics hand out (pid is carried along).
-int sort_now(mp_ctx* c) {
+// Stable compaction of the resident particles, dropping the lost ones and keeping pid order
+// (pid order is first restored by a scatter on pid ranks if a cell sort broke it).
+int compact_now(mp_ctx* c) {
     c->since_sort = 0;
+    if (c->n_slots == 0) {
+        c->pid_ordered = true;
+        return MP_OK;
+    }
+    const bool by_pid = !c->pid_ordered;
+    const int64_t m = by_pid ? c->next_pid : c->n_slots;  // flag domain: pids or slots
+    if (2 * (m + 1) > c->cmp_n) {
+        dfree(c->cmp_buf);
+        c->cmp_buf = nullptr;
+        c->cmp_n = 0;
+        if (dalloc(&c->cmp_buf, 2 * (m + 1) + (m + 1) / 2) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "compaction scratch allocation failed");
+        c->cmp_n = 2 * (m + 1) + (m + 1) / 2;
+    }
+    int rc = reserve_scan(c, m + 1);
+    if (rc) return rc;
+    int32_t* flag = c->cmp_buf;
+    int32_t* off = c->cmp_buf + (m + 1);
+    const Soa& src = c->soa[c->cur];
+    const Soa& dst = c->soa[1 - c->cur];
+    if (by_pid) {
+        HIP_TRY(c, mpd::launch_fill_i32(flag, m + 1, 0, c->stream));
+        HIP_TRY(c, mpd::launch_mark_pid(src.cell, src.pid, c->n_slots, flag, c->stream));
+    } else {
+        HIP_TRY(c, mpd::launch_flag_soa(src.cell, c->n_slots, flag, c->stream));
+        HIP_TRY(c, mpd::launch_fill_i32(flag + m, 1, 0, c->stream));
+    }
+    HIP_TRY(c, mpd::launch_excl_scan(flag, m + 1, off, c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, off + m, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, mpd::launch_move_soa(src, c->n_slots, off, by_pid, dst, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cur = 1 - c->cur;
+    c->n_slots = *c->pinned_i32;
+    c->pid_ordered = true;
+    return MP_OK;
+}
+
+int sort_now(mp_ctx* c) {
+    if (c->opt.keep_pid_order) return compact_now(c);
+    c->since_sort = 0;
+    c->pid_ordered = false;
     if (c->n_slots == 0) return MP_OK;
     const int shift = c->opt.sort_shift;
     const int64_t nb = (((int64_t)c->n_tets - 1) >> shift) + 1;
@@ -227,7 +271,8 @@ bool good_options(const mp_options& o) {
            (o.threads_per_particle == 1 || (o.threads_per_particle == 4 && o.particles_per_thread == 1)) &&
            (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4) &&
            o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
-           o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1);
+           o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
+           (o.keep_pid_order == 0 || o.keep_pid_order == 1);
 }
 
 }  // namespace
@@ -308,7 +353,7 @@ void mp_destroy(mp_ctx* c) {
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
-                    c->inj_tmp};
+                    c->inj_tmp, c->cmp_buf};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -416,6 +461,7 @@ int mp_set_particles(mp_ctx* c, const double* pos, const double* vel, const int3
     HIP_TRY(c, hipSetDevice(c->device));
     c->n_slots = 0;
     c->next_pid = 0;
+    c->pid_ordered = true;
     HIP_TRY(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters, c->stream));
     return mp_add_particles(c, pos, vel, cell, n);
 }
@@ -472,7 +518,8 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         if (rc) return rc;
     }
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    const int64_t n_rec = 4 * c->next_pid;  // record slots 4*pid+v for every pid ever issued
+    // record slots 4*slot+v while the slots are in pid order, else 4*pid+v for every pid issued
+    const int64_t n_rec = c->pid_ordered ? 4 * c->n_slots : 4 * c->next_pid;
     if (det) {
         int rc = reserve_records(c, n_rec, bit_width((uint64_t)c->n_nodes));
         if (rc) return rc;
@@ -501,6 +548,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.ppt = c->opt.particles_per_thread;
     a.tpp = c->opt.threads_per_particle;
     a.stage = c->opt.stage_tets != 0;
+    a.rec_by_slot = c->pid_ordered;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));

@@ -51,7 +51,7 @@ template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-           int32_t* __restrict__ part, int cap, int ppt_arg) {
+           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot) {
     const int ppt = MULTI ? ppt_arg : 1;
     const int64_t b0 = xcd_block() * ppt;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
@@ -94,7 +94,7 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                     s.cell[i] = c;
                     ok = true;
                     fc = c;
-                    if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3);
+                    if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tets + c, l0, l1, l2, l3);
                 } else {
                     s.cell[i] = -1;
                     n_lost++;
@@ -143,7 +143,7 @@ template <int WALL, int DET, int AGG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_staged(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
               double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-              int32_t* __restrict__ part, int cap) {
+              int32_t* __restrict__ part, int cap, bool rec_by_slot) {
     __shared__ DevTet stage[kStageSlots];
     __shared__ int32_t skey[kStageSlots];
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -213,8 +213,9 @@ k_push_staged(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, doub
                 const DevTet* t = tets + c;
                 n0 = t->con[0]; n1 = t->con[1]; n2 = t->con[2]; n3 = t->con[3];
             }
-            if (DET) put_records_nodes(keys, vals, s.pid[i], (uint32_t)n0, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3,
-                                       l0, l1, l2, l3);
+            if (DET)
+                put_records_nodes(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], (uint32_t)n0, (uint32_t)n1,
+                                  (uint32_t)n2, (uint32_t)n3, l0, l1, l2, l3);
         } else {
             s.cell[i] = -1;
             lost = true;
@@ -269,7 +270,7 @@ template <int WALL, int DET, int AGG>
 __global__ void __launch_bounds__(256)
 k_push_tpp4(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
             double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-            int32_t* __restrict__ part, int cap) {
+            int32_t* __restrict__ part, int cap, bool rec_by_slot) {
     const int sub = threadIdx.x & 3;
     const int64_t i = (xcd_block() * blockDim.x + threadIdx.x) >> 2;
     bool act = false, lost = false, crossed = false, capped = false, ok = false;
@@ -350,7 +351,7 @@ k_push_tpp4(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double
                 ok = true;
                 node = t->con[sub];
                 if (DET) {
-                    const int64_t r = 4 * (int64_t)s.pid[i] + sub;
+                    const int64_t r = 4 * (rec_by_slot ? i : (int64_t)s.pid[i]) + sub;
                     keys[r] = (uint32_t)node;
                     vals[r] = lw;
                 }
@@ -819,6 +820,35 @@ __global__ void k_compact_aos(const mp_particle* __restrict__ in, int64_t n,
     if (in[i].cell_index >= 0) out[off[i]] = in[i];
 }
 
+// ---- stable compaction and pid order of the resident particles ---------------------------
+// flag[i] = 1 for a particle still in the domain.
+__global__ void k_flag_soa(const int32_t* __restrict__ cell, int64_t n, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = cell[i] >= 0 ? 1 : 0;
+}
+
+// Copy particle i to slot dst_slot(i): off[i] (stable compaction) or off[pid] (pid order).
+template <bool BY_PID>
+__global__ void k_move_soa(Soa src, int64_t n, const int32_t* __restrict__ off, Soa dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = src.cell[i];
+    if (c < 0) return;
+    const int32_t pid = src.pid[i];
+    const int64_t d = off[BY_PID ? (int64_t)pid : i];
+    dst.x[d] = src.x[i]; dst.y[d] = src.y[i]; dst.z[d] = src.z[i];
+    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    dst.cell[d] = c;
+    dst.pid[d] = pid;
+}
+
+// present[pid] = 1 for every particle still in the domain (present pre-cleared).
+__global__ void k_mark_pid(const int32_t* __restrict__ cell, const int32_t* __restrict__ pid, int64_t n,
+                           int32_t* __restrict__ present) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cell[i] >= 0) present[pid[i]] = 1;
+}
+
 // ---- field / particle staging --------------------------------------------------------
 
 __global__ void k_set_field(DevTet* __restrict__ tets, const double* __restrict__ ef, int32_t nt) {
@@ -886,7 +916,7 @@ static hipError_t launch_push_staged(const PushArgs& a, hipStream_t st) {
     const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
 #define MP_STAGED(W, D, G)                                                                              \
     hipLaunchKernelGGL((k_push_staged<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
-                       a.den, a.keys, a.vals, a.part, a.cap)
+                       a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
     if (a.wall == 0) {
         if (a.det) MP_STAGED(0, 1, 0);
         else if (a.agg == 0) MP_STAGED(0, 0, 0);
@@ -907,7 +937,7 @@ static hipError_t launch_push_tpp4(const PushArgs& a, hipStream_t st) {
     const unsigned g = (grid_for(4 * a.n, bs) + kXcds - 1) / kXcds * kXcds;
 #define MP_TPP4(W, D, G)                                                                            \
     hipLaunchKernelGGL((k_push_tpp4<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
-                       a.den, a.keys, a.vals, a.part, a.cap)
+                       a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
     if (a.wall == 0) {
         if (a.det) MP_TPP4(0, 1, 0);
         else if (a.agg == 0) MP_TPP4(0, 0, 0);
@@ -935,10 +965,10 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     do {                                                                                                  \
         if (ppt == 1)                                                                                     \
             hipLaunchKernelGGL((k_push_soa<W, D, G, false>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, \
-                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1);                      \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1, a.rec_by_slot);       \
         else                                                                                              \
             hipLaunchKernelGGL((k_push_soa<W, D, G, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n,  \
-                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt);                    \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt, a.rec_by_slot);     \
     } while (0)
 #define MP_PUSH_W(W)                          \
     do {                                      \
@@ -1072,6 +1102,28 @@ hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, kSortThreads)), dim3(kSortThreads), 0, st, src, n, shift,
                        nb, cursor, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_soa(const int32_t* cell, int64_t n, int32_t* flag, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flag_soa, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, n, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool by_pid, const Soa& dst,
+                           hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (by_pid)
+        hipLaunchKernelGGL(k_move_soa<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, off, dst);
+    else
+        hipLaunchKernelGGL(k_move_soa<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, src, n, off, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_pid, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, pid, n, present);
     return hipGetLastError();
 }
 

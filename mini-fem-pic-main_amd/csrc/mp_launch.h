@@ -37,6 +37,7 @@ struct PushArgs {
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
     int tpp;  // threads per particle: 1, or 4 (one lane per tet vertex)
     bool stage;  // stage each workgroup's distinct start tets in LDS
+    bool rec_by_slot;  // deterministic records at 4*slot+v (slots in pid order), else 4*pid+v
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
@@ -59,6 +60,11 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 int64_t scan_scratch_ints(int64_t n);
 hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
+// stable compaction / pid reordering of the resident particles
+hipError_t launch_flag_soa(const int32_t* cell, int64_t n, int32_t* flag, hipStream_t st);
+hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool by_pid, const Soa& dst,
+                           hipStream_t st);
+hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st);
 // Counting sort by key = run * nb + (cell >> shift) (run term only when the Soa has runs).
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
                             int32_t* cnt, hipStream_t st);

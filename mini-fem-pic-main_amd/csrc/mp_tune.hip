@@ -29,6 +29,7 @@ struct Saved {
     void* mem = nullptr;
     Soa soa{};
     int64_t n_slots = 0, next_pid = 0, since_sort = 0;
+    bool pid_ordered = true;
     unsigned long long ctr[mpd::kNumCounters] = {};
     std::vector<double> den;
 };
@@ -50,6 +51,7 @@ int save(mp_ctx* c, Saved& s) {
     s.n_slots = c->n_slots;
     s.next_pid = c->next_pid;
     s.since_sort = c->since_sort;
+    s.pid_ordered = c->pid_ordered;
     if (alloc_soa(&s.mem, &s.soa, std::max<int64_t>(c->n_slots, 1)) != hipSuccess) {
         (void)hipGetLastError();
         return set_err(c, MP_ERR_ALLOC, "mp_autotune: no device memory for the saved particle state");
@@ -69,6 +71,7 @@ int restore(mp_ctx* c, const Saved& s) {
     c->n_slots = s.n_slots;
     c->next_pid = s.next_pid;
     c->since_sort = s.since_sort;
+    c->pid_ordered = s.pid_ordered;
     HIP_TRY(c, hipMemcpyAsync(c->ctr, s.ctr, sizeof(s.ctr), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->den_out, s.den.data(), sizeof(double) * s.den.size(), hipMemcpyHostToDevice,
                               c->stream));
@@ -163,6 +166,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
     sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
     sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
     sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
+    if (mode == MP_SCATTER_DETERMINISTIC) sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
     sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
 
     // leave the run as it was, with the chosen options

@@ -454,3 +454,36 @@ def test_autotune_leaves_state_and_keeps_parity(mp, orc):
             check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_keep_pid_order_matches_oracle(mp, orc, wall):
+    """keep_pid_order: the periodic pass compacts the lost particles in pid order instead of
+    sorting by cell, and deterministic records are written by slot.  Starts in cell-sorted
+    order (so the first compaction restores pid order by a scatter on pid ranks), injects
+    more particles midway, alternates the two density modes."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=31 + wall, n=60000)
+    dt = 6e-7
+    half = 40000
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos[:half], vel[:half], cell[:half], wall_mode=wall)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(wall_mode=wall, sort_interval=2)
+        mv.upload_field(ef)
+        mv.set_particles(pos[:half], vel[:half], cell[:half])
+        for k in range(7):
+            if k == 2:
+                mv.set_options(keep_pid_order=1)
+            if k == 4:
+                mv.add_particles(pos[half:], vel[half:], cell[half:])
+                run.pos = np.ascontiguousarray(np.concatenate([run.pos, pos[half:]]))
+                run.vel = np.ascontiguousarray(np.concatenate([run.vel, vel[half:]]))
+                run.cell = np.ascontiguousarray(np.concatenate([run.cell, cell[half:]]).astype(np.int32))
+                run.pid = np.concatenate([run.pid, np.arange(half, len(pos))])
+            mode = (k + 1) % 2
+            mv.step(QM, dt, SPWT, mode)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
+            gp, gv, gc, gid = mv.get_particles()
+            assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell), f"step {k}"
+            assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        if wall == 0:
+            assert run.stats["lost"] > 0
