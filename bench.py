@@ -161,7 +161,9 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
     [0.3, 0.7]^3) so a few thousand nodes take most of the scatter, Maxwellian velocities
     (sigma 3000 m/s, no drift), frozen sinusoidal field, reflecting walls, cell sort every 8
     steps.  The same initial state is run in deterministic mode (the configuration's mode) and
-    in atomic mode; the difference of the two step times is the cost of the sorted reduce."""
+    in atomic mode; the difference of the two step times is the cost of the sorted reduce.  A
+    third run keeps the particles in pid order (compaction instead of the cell sort), the
+    deterministic mode's faster layout."""
     t = time.perf_counter()
     mesh = mp.Mesh.box_mesh(100, 1.0)
     ef = mesh.field_sinusoid(AMP)
@@ -176,7 +178,9 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(sort_interval=8, wall_mode=mp.WALL_REFLECT, sort_shift=shift)
         mv.upload_field(ef)
-        for name, mode in (("deterministic", mp.SCATTER_DETERMINISTIC), ("atomic", mp.SCATTER_ATOMIC)):
+        for name, mode, pid_order in (("deterministic", mp.SCATTER_DETERMINISTIC, 0), ("atomic", mp.SCATTER_ATOMIC, 0),
+                                      ("deterministic_pid_order", mp.SCATTER_DETERMINISTIC, 1)):
+            mv.set_options(keep_pid_order=pid_order)
             mv.set_particles(pos, vel, cell)
             for _ in range(warmup):
                 mv.step(QM, dt, SPWT, mode)
@@ -207,6 +211,7 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):
     return {"value": d["value"], "unit": "particle-steps/s", "mode": "deterministic", "steps": steps,
             "ms_per_step": d["ms_per_step"], "push_gbs": d["push_gbs"], "atomic": a,
             "deterministic": d, "sorted_reduce_ms_per_step": d["ms_per_step"] - a["ms_per_step"],
+            "deterministic_pid_order": res["deterministic_pid_order"],
             "workload": f"100^3 six-tet box (6,000,000 tets, 1,030,301 nodes), {n} particles from a "
                         "3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2), Maxwellian sigma 3000 m/s, "
                         f"no drift, reflecting walls, sort every 8 steps (shift {shift}), dt {dt:g} s"}

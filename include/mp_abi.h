@@ -108,6 +108,8 @@ const char* mp_last_error(const mp_ctx* ctx);
 
 int  mp_get_options(const mp_ctx* ctx, mp_options* opt);
 int  mp_set_options(mp_ctx* ctx, const mp_options* opt);
+/* Shorthand for the sort interval alone: cell-sort (or compact) every k steps, 0 = never. */
+int  mp_set_sort_interval(mp_ctx* ctx, int32_t k);
 
 /* FESolver::ef (src/FESolver.h:49): per-element E, ef[n_tets][3], copied to the device.
  * MoveParticles copies it every step (src/mini-pic.cpp:276). */

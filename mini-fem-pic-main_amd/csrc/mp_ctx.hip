@@ -376,6 +376,13 @@ int mp_set_options(mp_ctx* c, const mp_options* o) {
     return MP_OK;
 }
 
+int mp_set_sort_interval(mp_ctx* c, int32_t k) {
+    if (!c) return MP_ERR_ARG;
+    if (k < 0) return set_err(c, MP_ERR_ARG, "mp_set_sort_interval: interval must be >= 0");
+    c->opt.sort_interval = k;
+    return MP_OK;
+}
+
 int mp_upload_field(mp_ctx* c, const double* ef, int32_t n_tets) {
     if (!c) return MP_ERR_ARG;
     if (!ef || n_tets != c->n_tets) return set_err(c, MP_ERR_ARG, "mp_upload_field: ef must be [n_tets][3]");

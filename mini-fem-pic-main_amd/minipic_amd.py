@@ -77,6 +77,7 @@ SIGNATURES = {
     "mp_last_error": (C.c_char_p, [_P]),
     "mp_get_options": (C.c_int, [_P, C.POINTER(MpOptions)]),
     "mp_set_options": (C.c_int, [_P, C.POINTER(MpOptions)]),
+    "mp_set_sort_interval": (C.c_int, [_P, _I32]),
     "mp_upload_field": (C.c_int, [_P, _P, _I32]),
     "mp_step_aos": (C.c_int, [_P, _P, C.POINTER(C.c_size_t), _D, _D, _D, _P, _I32]),
     "mp_set_particles": (C.c_int, [_P, _P, _P, _P, _I64]),

@@ -487,3 +487,35 @@ def test_keep_pid_order_matches_oracle(mp, orc, wall):
             assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
         if wall == 0:
             assert run.stats["lost"] > 0
+
+
+def test_error_paths(mp):
+    """Calls out of order or with bad arguments return their codes and a message, and leave
+    the context usable."""
+    mesh, ef, pos, vel, cell = box_case(mp, n=2000, seed=41)
+    L = mp.lib()
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_particles(pos, vel, cell)
+        with pytest.raises(mp.MpError) as e:  # no field yet
+            mv.step(QM, 1e-7, SPWT, 0)
+        assert e.value.code == 4
+        with pytest.raises(mp.MpError):
+            mv.set_options(block_size=100)
+        with pytest.raises(mp.MpError):
+            mv.set_options(threads_per_particle=4, particles_per_thread=2)
+        with pytest.raises(mp.MpError):
+            mv.step(QM, 1e-7, SPWT, 7)
+        assert L.mp_set_sort_interval(mv._h, -1) == 1 and b"interval" in L.mp_last_error(mv._h)
+        assert L.mp_set_sort_interval(mv._h, 3) == 0 and mv.options["sort_interval"] == 3
+        bad = cell.copy()
+        bad[5] = mesh.n_tets
+        with pytest.raises(mp.MpError):
+            mv.add_particles(pos, vel, bad)
+        mv.upload_field(ef)
+        mv.step(QM, 1e-7, SPWT, 0)
+        assert mv.num_active() > 0
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.upload_field(ef)
+        with pytest.raises(mp.MpError) as e:  # nothing resident to tune on
+            mv.autotune(QM, 1e-7, SPWT)
+        assert e.value.code == 4
