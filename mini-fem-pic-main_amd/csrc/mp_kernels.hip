@@ -421,26 +421,31 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
 }
 
 // Persistent, software-pipelined push (atomic scatter).  Each workgroup walks one contiguous
-// XCD-local range of the particle array 256 particles at a time; the cell and state of the
-// next chunk are loaded while the current chunk is pushed and walked, so the cell -> tet
-// gather chain of one chunk overlaps the streaming loads of the next.  The LDS hash keeps
-// accumulating across chunks (2048 slots) and is flushed to HBM with global atomics only
-// when more than 1024 slots are occupied, and once at the end: a chunk adds at most 1024
-// keys, so an insert always finds a slot.
+// XCD-local range of the particle array 256 particles at a time and loads the next chunk's
+// cell and state while the current chunk is pushed and walked.  Order matters on CDNA: the
+// vector-memory counter retires loads in issue order, so a wait for the current chunk's start
+// tet also waits for every load issued before it.  The start-tet loads are therefore issued
+// first and the next chunk's loads right after them (scheduling barriers keep that order;
+// the prefetch address is clamped in range rather than branched around, so no control flow
+// merges two different wait states).  The LDS hash (2048 slots) accumulates over
+// kPFlushEvery chunks between flushes, so most chunks pass no workgroup barrier; an insert
+// that finds no slot within kPProbe probes goes straight to a global atomic.
 constexpr int kPHash = 2048;
-constexpr int kPFlushAt = kPHash - 4 * 256;
+constexpr int kPFlushEvery = 4;
+constexpr int kPProbe = 32;
 
-__device__ __forceinline__ void lds_insert_occ(int32_t* hk, double* hv, int32_t* occ, int32_t node, double w) {
+__device__ __forceinline__ void lds_insert_or_global(int32_t* hk, double* hv, double* __restrict__ den,
+                                                     int32_t node, double w) {
     uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 11);
-    for (;;) {
+    for (int p = 0; p < kPProbe; p++) {
         const int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
-            if (prev == -1) atomicAdd(occ, 1);
             atomicAdd(&hv[h], w);
             return;
         }
         h = (h + 1) & (kPHash - 1);
     }
+    unsafeAtomicAdd(den + node, w);
 }
 
 template <int WALL>
@@ -449,76 +454,80 @@ k_push_persist(const DevTet* __restrict__ tets, Soa s, int64_t n, int64_t chunk,
                double* __restrict__ den, int32_t* __restrict__ part, int cap) {
     __shared__ int32_t hk[kPHash];
     __shared__ double hv[kPHash];
-    __shared__ int32_t occ;
     for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
         hk[q] = -1;
         hv[q] = 0.0;
     }
-    if (threadIdx.x == 0) occ = 0;
+    __syncthreads();
     const int64_t beg = xcd_block() * chunk;
     const int64_t end = beg + chunk < n ? beg + chunk : n;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
-    // prefetch of the first chunk
-    int32_t cn = -1;
-    double xn = 0, yn = 0, zn = 0, vxn = 0, vyn = 0, vzn = 0;
-    {
-        const int64_t j = beg + threadIdx.x;
-        if (j < end) {
-            cn = s.cell[j];
-            xn = ld(s.x + j); yn = ld(s.y + j); zn = ld(s.z + j);
-            vxn = ld(s.vx + j); vyn = ld(s.vy + j); vzn = ld(s.vz + j);
-        }
+    // the first chunk
+    int32_t c = -1;
+    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0;
+    if (beg < end) {
+        const int64_t i = beg + threadIdx.x;
+        const int64_t ii = i < end ? i : beg;
+        const int32_t ci = s.cell[ii];
+        x0 = ld(s.x + ii); y0 = ld(s.y + ii); z0 = ld(s.z + ii);
+        vx0 = ld(s.vx + ii); vy0 = ld(s.vy + ii); vz0 = ld(s.vz + ii);
+        c = i < end ? ci : -1;
     }
-    __syncthreads();
-    for (int64_t base = beg; base < end; base += blockDim.x) {
+    int k = 0;
+    for (int64_t base = beg; base < end; base += blockDim.x, k++) {
         const int64_t i = base + threadIdx.x;
-        int32_t c = cn;
-        const double x0 = xn, y0 = yn, z0 = zn;
-        double vx = vxn, vy = vyn, vz = vzn;
-        {
-            const int64_t j = i + blockDim.x;
-            cn = -1;
-            if (j < end) {
-                cn = s.cell[j];
-                xn = ld(s.x + j); yn = ld(s.y + j); zn = ld(s.z + j);
-                vxn = ld(s.vx + j); vyn = ld(s.vy + j); vzn = ld(s.vz + j);
-            }
-        }
-        if (i < end && c >= 0) {
+        const bool live = i < end && c >= 0;
+        const DevTet* t = tets + (live ? c : 0);
+        StartTet g0;
+        load_start(t, g0);
+        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+        // (compiler-only fences: no instruction, they keep the two load groups in this order)
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        // the next chunk, issued behind the start tet
+        const int64_t j = i + blockDim.x;
+        const int64_t jj = j < end ? j : beg;
+        const int32_t cj = s.cell[jj];
+        const double xn = ld(s.x + jj), yn = ld(s.y + jj), zn = ld(s.z + jj);
+        const double vxn = ld(s.vx + jj), vyn = ld(s.vy + jj), vzn = ld(s.vz + jj);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        double vx = vx0 + (qm * ex) * dt;
+        double vy = vy0 + (qm * ey) * dt;
+        double vz = vz0 + (qm * ez) * dt;
+        double x = x0 + vx * dt;
+        double y = y0 + vy * dt;
+        double z = z0 + vz * dt;
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        WalkStats st{0, 0};
+        int32_t cw = live ? c : 0;
+        const int r = walk_start<WALL>(tets, g0, cw, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+        if (live) {
             n_act++;
-            const DevTet* t = tets + c;
-            vx = vx + (qm * t->ef[0]) * dt;
-            vy = vy + (qm * t->ef[1]) * dt;
-            vz = vz + (qm * t->ef[2]) * dt;
-            double x = x0 + vx * dt;
-            double y = y0 + vy * dt;
-            double z = z0 + vz * dt;
-            double l0, l1, l2, l3;
-            WalkStats st{0, 0};
-            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             n_moves += st.moves;
             n_refl += st.refl;
             n_cross += (st.moves - st.refl) > 0;
-            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             if (r == kInside) {
-                s.cell[i] = c;
-                const DevTet* tf = tets + c;
+                s.cell[i] = cw;
+                const DevTet* tf = tets + cw;
                 const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
-                lds_insert_occ(hk, hv, &occ, n0, l0);
-                lds_insert_occ(hk, hv, &occ, n1, l1);
-                lds_insert_occ(hk, hv, &occ, n2, l2);
-                lds_insert_occ(hk, hv, &occ, n3, l3);
+                lds_insert_or_global(hk, hv, den, n0, l0);
+                lds_insert_or_global(hk, hv, den, n1, l1);
+                lds_insert_or_global(hk, hv, den, n2, l2);
+                lds_insert_or_global(hk, hv, den, n3, l3);
             } else {
                 s.cell[i] = -1;
                 n_lost++;
                 n_cap += r == kCapped;
             }
+            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
         }
-        __syncthreads();
-        const int32_t occupied = occ;  // every thread reads it before any next-chunk insert
-        __syncthreads();
-        if (occupied > kPFlushAt) {  // block-uniform
+        c = j < end ? cj : -1;
+        x0 = xn; y0 = yn; z0 = zn;
+        vx0 = vxn; vy0 = vyn; vz0 = vzn;
+        if ((k + 1) % kPFlushEvery == 0) {  // workgroup-uniform
+            __syncthreads();
             for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
                 const int32_t node = hk[q];
                 if (node >= 0) {
@@ -528,10 +537,9 @@ k_push_persist(const DevTet* __restrict__ tets, Soa s, int64_t n, int64_t chunk,
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) occ = 0;
-            __syncthreads();
         }
     }
+    __syncthreads();
     for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
         const int32_t node = hk[q];
         if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
