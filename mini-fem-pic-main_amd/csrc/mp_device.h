@@ -54,8 +54,8 @@ struct WalkStats {
 // XtoLtet (src/meshes.h:75-100) from cell c.  On return c is the last tet visited and
 // l0..l3 its barycentric weights.  WALL == 1 reflects at boundary faces instead of
 // reporting the particle lost; a reflection counts as one move toward cap.  The tet's
-// fields are read where they are used (not staged in registers): the kernel is latency
-// bound and keeping the VGPR count low (<= 64: 8 waves per SIMD) matters more.
+// fields are read where they are used rather than staged in registers, which keeps the
+// loop's live registers few.
 template <int WALL>
 __device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c, double& x,
                                     double& y, double& z, double& vx, double& vy, double& vz,
