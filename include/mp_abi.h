@@ -82,6 +82,10 @@ typedef struct mp_options {
                                       particles in pid order instead of a cell sort; the
                                       deterministic records are then written contiguously by
                                       slot.  0 (default): cell sort */
+    int32_t compact_crossers;      /* 1: each workgroup queues its crossing particles in LDS
+                                      after the first walk iteration and walks them densely
+                                      packed (TPP=1, PPT=1, aggregation 2 or the
+                                      deterministic scatter); 0 (default): every lane walks */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */

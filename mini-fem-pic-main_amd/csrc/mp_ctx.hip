@@ -107,6 +107,7 @@ mp_options default_options() {
     o.sort_shift = 0;
     o.stage_tets = 0;
     o.keep_pid_order = 0;
+    o.compact_crossers = 0;
     return o;
 }
 
@@ -272,7 +273,8 @@ bool good_options(const mp_options& o) {
            (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4) &&
            o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
            o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
-           (o.keep_pid_order == 0 || o.keep_pid_order == 1);
+           (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
+           (o.compact_crossers == 0 || o.compact_crossers == 1);
 }
 
 }  // namespace
@@ -556,6 +558,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.tpp = c->opt.threads_per_particle;
     a.stage = c->opt.stage_tets != 0;
     a.rec_by_slot = c->pid_ordered;
+    a.compact = c->opt.compact_crossers != 0;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));

@@ -117,14 +117,17 @@ __device__ __forceinline__ void load_start(const DevTet* __restrict__ t, StartTe
     s.n0 = t->nbr[0]; s.n1 = t->nbr[1]; s.n2 = t->nbr[2]; s.n3 = t->nbr[3];
 }
 
-// XtoLtet from cell c whose start-tet fields s are already in registers: the first
-// iteration runs on s; a particle that moves continues in walk() from the neighbour with
-// one move counted, exactly as walk() would have counted it.
+// The first iteration of XtoLtet on the start tet's fields s (already in registers).  Returns
+// kInside, kLost, kCapped (cap <= 0 and outside: a caller's way to evaluate the start tet
+// only), or kContinue: the walk goes on in walk() from c with st.moves counted so far (the
+// neighbour with one move, or the same tet for a reflecting boundary face, which walk()
+// re-evaluates and reflects).
+constexpr int kContinue = 3;
+
 template <int WALL>
-__device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const StartTet& s, int32_t& c,
-                                          double& x, double& y, double& z, double& vx, double& vy,
-                                          double& vz, double& l0, double& l1, double& l2, double& l3,
-                                          int cap, WalkStats& st) {
+__device__ __forceinline__ int first_step(const StartTet& s, int32_t& c, double x, double y, double z,
+                                          double& l0, double& l1, double& l2, double& l3, int cap,
+                                          WalkStats& st) {
     l0 = bary(s.a0, s.b0, s.g0, s.d0, x, y, z, s.vol);
     l1 = bary(s.a1, s.b1, s.g1, s.d1, x, y, z, s.vol);
     l2 = bary(s.a2, s.b2, s.g2, s.d2, x, y, z, s.vol);
@@ -132,7 +135,7 @@ __device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const
     const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
                      (l3 < 0 || l3 > 1.0);
     if (!out) return kInside;
-    if (cap <= 0) return kCapped;  // a caller's way to evaluate the start tet only
+    if (cap <= 0) return kCapped;
     int mi = 0;
     double ml = l0;
     if (l1 < ml) { ml = l1; mi = 1; }
@@ -142,13 +145,23 @@ __device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const
     if (nb >= 0) {  // cap >= 1 here, so the first move is never capped
         st.moves = 1;
         c = nb;
-        return walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+        return kContinue;
     }
     if (WALL == 0) {
         st.moves = 1;
         return kLost;
     }
-    // reflecting wall: let walk() redo this tet (it counts and performs the reflection)
+    return kContinue;
+}
+
+// XtoLtet from cell c whose start-tet fields s are already in registers.
+template <int WALL>
+__device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const StartTet& s, int32_t& c,
+                                          double& x, double& y, double& z, double& vx, double& vy,
+                                          double& vz, double& l0, double& l1, double& l2, double& l3,
+                                          int cap, WalkStats& st) {
+    const int r = first_step<WALL>(s, c, x, y, z, l0, l1, l2, l3, cap, st);
+    if (r != kContinue) return r;
     return walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
 }
 

@@ -235,6 +235,155 @@ k_push_staged(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, doub
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
+// ---- crossing particles compacted before the walk ---------------------------------------
+// After the first walk iteration most lanes are done (~70% of particles stay in their tet);
+// the rest would walk in every wave of the workgroup with most lanes idle.  Here the waves
+// append their crossing particles (state, next tet, moves so far) to an LDS queue with one
+// ballot and one LDS atomic per wave, and after a barrier the queue is walked by the first
+// lanes of the workgroup, densely packed: ~2 waves walk instead of 4.  A crossing particle
+// that finds the queue full walks in place.  Scatter and counters then cover both the
+// particles finished in place and the queued ones.  AGG 2 (LDS hash) or the deterministic
+// records.
+constexpr int kQueue = 128;
+
+template <int WALL, int DET>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
+k_push_compact(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
+               double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
+               int32_t* __restrict__ part, int cap, bool rec_by_slot) {
+    __shared__ double qx[kQueue], qy[kQueue], qz[kQueue], qvx[kQueue], qvy[kQueue], qvz[kQueue];
+    __shared__ int32_t qc[kQueue], qi[kQueue], qmv[kQueue];
+    __shared__ int32_t qn;
+    if (threadIdx.x == 0) qn = 0;
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    bool ok1 = false, ok2 = false;
+    int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    bool live = false, need = false;
+    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    int32_t c = 0;
+    WalkStats st{0, 0};
+    int r = kInside;
+    if (i < n) {
+        const int32_t c_in = s.cell[i];
+        vx = ld(s.vx + i); vy = ld(s.vy + i); vz = ld(s.vz + i);
+        const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+        __builtin_amdgcn_sched_barrier(0);
+        live = c_in >= 0;
+        c = live ? c_in : 0;
+        const DevTet* t = tets + c;
+        StartTet g0;
+        load_start(t, g0);
+        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+        vx = vx + (qm * ex) * dt;
+        vy = vy + (qm * ey) * dt;
+        vz = vz + (qm * ez) * dt;
+        x = x0 + vx * dt;
+        y = y0 + vy * dt;
+        z = z0 + vz * dt;
+        r = first_step<WALL>(g0, c, x, y, z, l0, l1, l2, l3, live ? cap : 0, st);
+        need = live && r == kContinue;
+    }
+    __syncthreads();  // qn is cleared
+    // append the crossing particles: one LDS atomic per wave
+    const unsigned long long mask = __ballot(need);
+    int slot = -1;
+    if (mask) {
+        const int lane = lane_id();
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&qn, __popcll(mask));
+        base = __shfl(base, leader, kWave);
+        if (need) slot = base + __popcll(mask & ((1ull << lane) - 1));
+    }
+    if (need && slot < kQueue) {
+        qx[slot] = x; qy[slot] = y; qz[slot] = z;
+        qvx[slot] = vx; qvy[slot] = vy; qvz[slot] = vz;
+        qc[slot] = c;
+        qi[slot] = (int32_t)threadIdx.x;
+        qmv[slot] = st.moves;
+    } else if (need) {
+        r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);  // queue full: walk here
+    }
+    const bool queued = need && slot < kQueue;
+    if (live && !queued) {
+        n_act++;
+        n_moves += st.moves;
+        n_refl += st.refl;
+        n_cross += (st.moves - st.refl) > 0;
+        if (r == kInside) {
+            s.cell[i] = c;
+            ok1 = true;
+            const DevTet* tf = tets + c;
+            a0 = tf->con[0]; a1 = tf->con[1]; a2 = tf->con[2]; a3 = tf->con[3];
+            if (DET) put_records_nodes(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], (uint32_t)a0, (uint32_t)a1,
+                                       (uint32_t)a2, (uint32_t)a3, l0, l1, l2, l3);
+        } else {
+            s.cell[i] = -1;
+            n_lost++;
+            n_cap += r == kCapped;
+        }
+        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+    }
+    __syncthreads();  // the queue is complete
+    const int nq = qn < kQueue ? qn : kQueue;
+    if ((int)threadIdx.x < nq) {
+        const int q = threadIdx.x;
+        const int64_t ip = xcd_block() * blockDim.x + qi[q];  // the queued particle's slot
+        double px = qx[q], py = qy[q], pz = qz[q], pvx = qvx[q], pvy = qvy[q], pvz = qvz[q];
+        int32_t pc = qc[q];
+        WalkStats pst{qmv[q], 0};
+        const int pr = walk<WALL>(tets, pc, px, py, pz, pvx, pvy, pvz, m0, m1, m2, m3, cap, pst);
+        n_act++;
+        n_moves += pst.moves;
+        n_refl += pst.refl;
+        n_cross += (pst.moves - pst.refl) > 0;
+        if (pr == kInside) {
+            s.cell[ip] = pc;
+            ok2 = true;
+            const DevTet* tf = tets + pc;
+            b0 = tf->con[0]; b1 = tf->con[1]; b2 = tf->con[2]; b3 = tf->con[3];
+            if (DET) put_records_nodes(keys, vals, rec_by_slot ? ip : (int64_t)s.pid[ip], (uint32_t)b0, (uint32_t)b1,
+                                       (uint32_t)b2, (uint32_t)b3, m0, m1, m2, m3);
+        } else {
+            s.cell[ip] = -1;
+            n_lost++;
+            n_cap += pr == kCapped;
+        }
+        st_(s.x + ip, px); st_(s.y + ip, py); st_(s.z + ip, pz);
+        st_(s.vx + ip, pvx); st_(s.vy + ip, pvy); st_(s.vz + ip, pvz);
+    }
+    if (!DET) {
+        __shared__ int32_t hk[kHashSlots];
+        __shared__ double hv[kHashSlots];
+        for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+            hk[q] = -1;
+            hv[q] = 0.0;
+        }
+        __syncthreads();
+        if (ok1) {
+            lds_insert(hk, hv, a0, l0);
+            lds_insert(hk, hv, a1, l1);
+            lds_insert(hk, hv, a2, l2);
+            lds_insert(hk, hv, a3, l3);
+        }
+        if (ok2) {
+            lds_insert(hk, hv, b0, m0);
+            lds_insert(hk, hv, b1, m1);
+            lds_insert(hk, hv, b2, m2);
+            lds_insert(hk, hv, b3, m3);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+            const int32_t node = hk[q];
+            if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
+        }
+    }
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
 // ---- four threads per particle ------------------------------------------------------------
 // Lane v (0..3) of a quad owns vertex v of the current tet: it gathers that vertex's 32 B row
 // (alpha, beta, gamma, delta) instead of all 128 B, computes lc_v, and the quad exchanges the
@@ -919,6 +1068,24 @@ static hipError_t launch_push_persist(const PushArgs& a, hipStream_t st) {
     return launch_tally_finish(a.part, g, a.ctr, st);
 }
 
+static hipError_t launch_push_compact(const PushArgs& a, hipStream_t st) {
+    const unsigned g = (grid_for(a.n, 256) + kXcds - 1) / kXcds * kXcds;
+#define MP_COMPACT(W, D)                                                                                   \
+    hipLaunchKernelGGL((k_push_compact<W, D>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, a.den, \
+                       a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
+    if (a.wall == 0) {
+        if (a.det) MP_COMPACT(0, 1);
+        else MP_COMPACT(0, 0);
+    } else {
+        if (a.det) MP_COMPACT(1, 1);
+        else MP_COMPACT(1, 0);
+    }
+#undef MP_COMPACT
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
 static hipError_t launch_push_staged(const PushArgs& a, hipStream_t st) {
     const int bs = a.block;
     const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
@@ -967,6 +1134,7 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
     const int ppt = a.ppt > 0 ? a.ppt : 1;
+    if (a.compact && ppt == 1 && (a.det || a.agg == 2)) return launch_push_compact(a, st);
     if (a.stage && ppt == 1 && (a.det || a.agg == 0 || a.agg == 2)) return launch_push_staged(a, st);
     const unsigned g = (grid_for(a.n, (int64_t)bs * ppt) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                                  \

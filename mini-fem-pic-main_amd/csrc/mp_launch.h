@@ -38,6 +38,7 @@ struct PushArgs {
     int tpp;  // threads per particle: 1, or 4 (one lane per tet vertex)
     bool stage;  // stage each workgroup's distinct start tets in LDS
     bool rec_by_slot;  // deterministic records at 4*slot+v (slots in pid order), else 4*pid+v
+    bool compact;      // compact the crossing particles of each workgroup before the walk
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);

@@ -1,6 +1,7 @@
 // Autotuner of the device-resident step (mp_autotune): picks threads per particle, particles
-// per thread, workgroup size, atomic-scatter aggregation, start-tet staging, cell-sort key
-// shift and sort interval for the particles currently resident, by timing short windows of mp_step from one
+// per thread, workgroup size, atomic-scatter aggregation, start-tet staging, crossing-particle
+// compaction, pid-order compaction, cell-sort key shift and sort interval for the particles
+// currently resident, by timing short windows of mp_step from one
 // saved state.
 //
 // Every candidate gives bit-identical cells, positions and velocities (the options change only
@@ -12,7 +13,7 @@
 //   steady state does.
 // Coordinate descent, one knob at a time in the order sort shift, sort interval, aggregation
 // (atomic mode), workgroup size, particles per thread, threads per particle, start-tet
-// staging, then the sort
+// staging, crossing-particle compaction, pid order (deterministic mode), then the sort
 // interval again under the chosen kernel.  The saved state, counters and density are restored
 // before returning, so tuning leaves the run where it was; only the options change.
 #include <algorithm>
@@ -166,6 +167,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
     sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
     sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
     sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
+    sweep([](mp_options& o, int v) { o.compact_crossers = v; }, {0, 1});
     if (mode == MP_SCATTER_DETERMINISTIC) sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
     sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
 

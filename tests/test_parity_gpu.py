@@ -519,3 +519,27 @@ def test_error_paths(mp):
         with pytest.raises(mp.MpError) as e:  # nothing resident to tune on
             mv.autotune(QM, 1e-7, SPWT)
         assert e.value.code == 4
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+@pytest.mark.parametrize("dt,sort_interval", [(4e-7, 3), (1.5e-6, 0)], ids=["sorted", "fast-unsorted"])
+def test_compact_crossers_matches_oracle(mp, orc, wall, dt, sort_interval):
+    """Crossing particles queued in LDS after the first walk iteration and walked densely
+    packed; the fast unsorted case overfills the queue (the overflow walks in place) and hits
+    the walk cap.  Same bits, same counters, both density modes."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=37 + wall, n=70001)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=6)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(compact_crossers=1, sort_interval=sort_interval, wall_mode=wall, walk_cap=6)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(4):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
