@@ -70,14 +70,14 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
 
 
 def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
-            tpp=1, tune=False, stage=0, pid_order=0, compact=0):
+            tpp=1, tune=False, stage=0, pid_order=0, compact=0, defer=0):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
     tuned = None
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block,
                        particles_per_thread=ppt, threads_per_particle=tpp, stage_tets=stage,
-                       keep_pid_order=pid_order, compact_crossers=compact)
+                       keep_pid_order=pid_order, compact_crossers=compact, defer_walk=defer)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         if tune:
@@ -390,6 +390,7 @@ def main():
     ap.add_argument("--stage", type=int, default=0, help="stage start tets in LDS (1) or gather per lane (0)")
     ap.add_argument("--det-pid-order", type=int, default=0, help="deterministic line: keep pid order (1) or cell sort (0)")
     ap.add_argument("--compact", type=int, default=0, help="compact crossing particles before the walk (1) or not (0)")
+    ap.add_argument("--defer", type=int, default=0, help="walk crossing particles in a second kernel (1) or not (0)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
@@ -411,7 +412,7 @@ def main():
     mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
     r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg,
                 args.sort_shift, args.block, args.ppt, args.tpp, tune=not args.no_tune, stage=args.stage,
-                compact=args.compact)
+                compact=args.compact, defer=args.defer)
     log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)

@@ -86,6 +86,12 @@ typedef struct mp_options {
                                       after the first walk iteration and walks them densely
                                       packed (TPP=1, PPT=1, aggregation 2 or the
                                       deterministic scatter); 0 (default): every lane walks */
+    int32_t defer_walk;            /* 1: the push kernel finishes only the particles its start
+                                      tet decides; those that cross into a neighbour are queued
+                                      per wave and walked, densely packed, by a second kernel
+                                      (TPP=1, PPT=1, no staging or compaction, aggregation
+                                      0, 1, 2, 4 or the deterministic scatter); 0 (default):
+                                      every lane walks in the push kernel */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */

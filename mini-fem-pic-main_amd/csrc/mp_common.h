@@ -279,4 +279,26 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
                       base + (uint32_t)tf->con[2], base + (uint32_t)tf->con[3], l0, l1, l2, l3);
 }
 
+// The atomic scatter a kernel's template arguments select (nothing when DET: the records are
+// written where each walk ends): AGG 0 one atomic per weight, 1 grouped per wave by tet,
+// 2 LDS hash per workgroup (every thread of the block calls it), 4 LDS hash per wave.
+template <int DET, int AGG>
+__device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, double* __restrict__ den, bool ok,
+                                            int32_t fc, double l0, double l1, double l2, double l3) {
+    if (DET) return;
+    if (AGG == 4) {
+        scatter_lds_wave(tets, den, ok, fc, l0, l1, l2, l3);
+    } else if (AGG == 2) {
+        scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
+    } else if (AGG == 1) {
+        scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
+    } else if (ok) {
+        const DevTet* t = tets + fc;
+        unsafeAtomicAdd(den + t->con[0], l0);
+        unsafeAtomicAdd(den + t->con[1], l1);
+        unsafeAtomicAdd(den + t->con[2], l2);
+        unsafeAtomicAdd(den + t->con[3], l3);
+    }
+}
+
 }  // namespace mpd

@@ -53,6 +53,10 @@ struct mp_ctx {
     // staging of injected particles (mp_add_particles)
     void* inj_tmp = nullptr;
     size_t inj_bytes = 0;
+    // deferred walk: per-wave queues of crossing particles and their counts
+    uint32_t* wq = nullptr;
+    int32_t* wq_cnt = nullptr;
+    int64_t wq_n = 0, wq_waves = 0;
     // per-block tallies of the push kernels
     int32_t* tally = nullptr;
     int64_t tally_n = 0;

@@ -47,18 +47,26 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
 #ifndef MP_PUSH_WAVES
 #define MP_PUSH_WAVES 5
 #endif
-template <int WALL, int DET, int AGG, bool MULTI>
+//
+// DEFER (ppt 1 only): a particle whose push takes it into a neighbour tet is not walked here.
+// It is stored pushed, with that neighbour as its cell, and listed for k_walk_deferred in its
+// wave's queue region: wq[wave slot * 64 + rank] = slot | (moves so far << 31), the count in
+// wq_cnt[wave slot] (wave slot = blockIdx.x * waves per block + wave; plain stores, no
+// atomics).  The waves here then stream without waiting on walk round trips.
+template <int WALL, int DET, int AGG, bool MULTI, bool DEFER = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot) {
+           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot,
+           uint32_t* __restrict__ wq = nullptr, int32_t* __restrict__ wq_cnt = nullptr) {
+    static_assert(!(DEFER && MULTI), "the deferred walk takes one particle per thread");
     const int ppt = MULTI ? ppt_arg : 1;
     const int64_t b0 = xcd_block() * ppt;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
     for (int k = 0; k < ppt; k++) {
         const int64_t i = (b0 + k) * blockDim.x + threadIdx.x;
-        bool ok = false;
-        int32_t fc = -1;
+        bool ok = false, defer = false;
+        int32_t fc = -1, dmoves = 0;
         double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
         if (i < n) {
             // The particle's state is loaded with its cell, not after it: only the start tet
@@ -84,8 +92,21 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             double y = y0 + vy * dt;
             double z = z0 + vz * dt;
             WalkStats st{0, 0};
-            const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
-            if (live) {
+            int r;
+            if (DEFER) {
+                r = first_step<WALL>(g0, c, x, y, z, l0, l1, l2, l3, live ? cap : 0, st);
+                defer = live && r == kContinue;
+                dmoves = st.moves;
+            } else {
+                r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+            }
+            if (DEFER && defer) {
+                // the walk's counts are taken where it finishes
+                n_act++;
+                s.cell[i] = c;
+                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+            } else if (live) {
                 n_act++;
                 n_cross += (st.moves - st.refl) > 0;
                 n_moves += st.moves;
@@ -104,26 +125,101 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
         }
-        if (!DET) {
-            if (AGG == 4) {
-                scatter_lds_wave(tets, den, ok, fc, l0, l1, l2, l3);
-            } else if (AGG == 2) {
-                scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
-            } else if (AGG == 1) {
-                scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
-            } else if (ok) {
-                const DevTet* t = tets + fc;
-                unsafeAtomicAdd(den + t->con[0], l0);
-                unsafeAtomicAdd(den + t->con[1], l1);
-                unsafeAtomicAdd(den + t->con[2], l2);
-                unsafeAtomicAdd(den + t->con[3], l3);
+        if (DEFER) {
+            const unsigned long long m = __ballot(defer);
+            const int64_t ws = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+            if (defer) {
+                const int rank = __popcll(m & ((1ull << lane_id()) - 1ull));
+                wq[ws * kWave + rank] = (uint32_t)i | ((uint32_t)dmoves << 31);
             }
+            if (lane_id() == 0) wq_cnt[ws] = __popcll(m);
         }
+        scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
     if (AGG == 4 && !DET)
         tally_wave(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
     else
         tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
+// ---- deferred walk ------------------------------------------------------------------------
+// Walks the particles k_push_soa<..., DEFER> listed, in full waves.  Workgroup b takes 64
+// wave slots of the push grid: those of push workgroups h = (k * G + t) * 8 + xcd, t < G,
+// G = 64 / waves per push workgroup, k = b / 8, xcd = b % 8.  Workgroups are dispatched to the
+// XCDs round-robin, so the walk runs on the XCD that pushed the particles and whose L2 holds
+// their start region's tets.  Lanes of the first wave prefix-sum the 64 counts into LDS; each
+// thread then finds its entry's slot by binary search over the sums.  Cells, positions and
+// velocities (reflecting walls) are written back, the scatter is the push kernel's.
+template <int WALL, int DET, int AGG>
+__global__ void __launch_bounds__(256)
+k_walk_deferred(const DevTet* __restrict__ tets, Soa s, const uint32_t* __restrict__ wq,
+                const int32_t* __restrict__ wq_cnt, int64_t n_waves, int wpb, double* __restrict__ den,
+                uint32_t* __restrict__ keys, double* __restrict__ vals, int32_t* __restrict__ part, int cap,
+                bool rec_by_slot) {
+    __shared__ int32_t pre[kWave + 1];
+    const int64_t xcd = blockIdx.x % kXcds, grp = blockIdx.x / kXcds;
+    const int G = kWave / wpb;
+    if (threadIdx.x < kWave) {
+        const int l = threadIdx.x;
+        const int64_t ws = ((grp * G + l / wpb) * kXcds + xcd) * wpb + l % wpb;
+        int v = ws < n_waves ? wq_cnt[ws] : 0;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(v, o, kWave);
+            if (l >= o) v += y;
+        }
+        pre[l + 1] = v;
+        if (l == 0) pre[0] = 0;
+    }
+    __syncthreads();
+    const int total = pre[kWave];
+    int n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    for (int b = 0; b < total; b += blockDim.x) {  // uniform over the workgroup
+        const int j = b + (int)threadIdx.x;
+        bool ok = false;
+        int32_t fc = -1;
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        if (j < total) {
+            int lo = 0, hi = kWave;  // pre[lo] <= j < pre[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const int64_t ws = ((grp * G + lo / wpb) * kXcds + xcd) * wpb + lo % wpb;
+            const uint32_t e = wq[ws * kWave + (j - pre[lo])];
+            const int64_t i = (int64_t)(e & 0x7FFFFFFFu);
+            WalkStats st{(int)(e >> 31), 0};
+            int32_t c = s.cell[i];
+            double x = ld(s.x + i), y = ld(s.y + i), z = ld(s.z + i);
+            double vx = 0.0, vy = 0.0, vz = 0.0;
+            if (WALL) {
+                vx = ld(s.vx + i);
+                vy = ld(s.vy + i);
+                vz = ld(s.vz + i);
+            }
+            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            n_cross += (st.moves - st.refl) > 0;
+            n_moves += st.moves;
+            n_refl += st.refl;
+            if (r == kInside) {
+                s.cell[i] = c;
+                ok = true;
+                fc = c;
+                if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tets + c, l0, l1, l2, l3);
+            } else {
+                s.cell[i] = -1;
+                n_lost++;
+                n_cap += (r == kCapped);
+            }
+            if (WALL) {
+                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+            }
+        }
+        scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
+    }
+    tally_counts(part, 0, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
 // ---- start tets staged in LDS -----------------------------------------------------------
@@ -1128,9 +1224,57 @@ static hipError_t launch_push_tpp4(const PushArgs& a, hipStream_t st) {
     return launch_tally_finish(a.part, g, a.ctr, st);
 }
 
+// Push without the walk beyond the start tet, then k_walk_deferred over the listed crossers.
+static hipError_t launch_push_defer(const PushArgs& a, hipStream_t st) {
+    const int bs = a.block;
+    const int wpb = bs / kWave;
+    const int64_t g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
+    const int64_t n_waves = g * wpb;
+    const int G = kWave / wpb;
+    const int64_t gw = kXcds * ((g / kXcds + G - 1) / G);
+#define MP_DEFER(W, D, A)                                                                                  \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_push_soa<W, D, A, false, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, \
+                           a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1, a.rec_by_slot, a.wq, a.wq_cnt); \
+        e = hipGetLastError();                                                                             \
+        if (e != hipSuccess) return e;                                                                     \
+        e = launch_tally_finish(a.part, (!D && A == 4) ? n_waves : g, a.ctr, st);                         \
+        if (e != hipSuccess) return e;                                                                     \
+        hipLaunchKernelGGL((k_walk_deferred<W, D, A>), dim3(gw), dim3(256), 0, st, a.tets, a.soa, a.wq,    \
+                           a.wq_cnt, n_waves, wpb, a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot);    \
+    } while (0)
+#define MP_DEFER_W(W)                           \
+    do {                                        \
+        if (a.det) MP_DEFER(W, 1, 0);           \
+        else if (a.agg == 4) MP_DEFER(W, 0, 4); \
+        else if (a.agg == 2) MP_DEFER(W, 0, 2); \
+        else if (a.agg == 1) MP_DEFER(W, 0, 1); \
+        else MP_DEFER(W, 0, 0);                 \
+    } while (0)
+    hipError_t e = hipSuccess;
+    if (a.wall == 0) MP_DEFER_W(0);
+    else MP_DEFER_W(1);
+#undef MP_DEFER_W
+#undef MP_DEFER
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, gw, a.ctr, st);
+}
+
+// Grid and queue sizes of the deferred walk for n particles in workgroups of `block`: queue
+// entries (64 per wave slot) and wave-slot counts.
+void defer_queue_size(int64_t n, int block, int64_t* entries, int64_t* waves) {
+    const int64_t g = (grid_for(n, block) + kXcds - 1) / kXcds * kXcds;
+    *waves = g * (block / kWave);
+    *entries = *waves * kWave;
+}
+
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
     if (a.tpp == 4) return launch_push_tpp4(a, st);
+    if (a.defer && a.wq && a.wq_cnt && a.ppt <= 1 && !a.stage && !a.compact && (a.det || a.agg != 3) &&
+        a.n < ((int64_t)1 << 31))
+        return launch_push_defer(a, st);
     if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
     const int ppt = a.ppt > 0 ? a.ppt : 1;

@@ -543,3 +543,32 @@ def test_compact_crossers_matches_oracle(mp, orc, wall, dt, sort_interval):
         c = mv.counters()
         for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
             assert c[key] == run.stats[key], key
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+@pytest.mark.parametrize("agg,block,pid_order", [(2, 256, 0), (0, 64, 1), (4, 128, 0), (1, 256, 1)],
+                         ids=["lds-256", "plain-64-pidorder", "wave-128", "grouped-256-pidorder"])
+@pytest.mark.parametrize("dt,sort_interval", [(4e-7, 3), (1.5e-6, 0)], ids=["sorted", "fast-unsorted"])
+def test_defer_walk_matches_oracle(mp, orc, wall, agg, block, pid_order, dt, sort_interval):
+    """The walk beyond the start tet deferred to a second kernel over per-wave queues, for
+    every aggregation and workgroup size (64, 128, 256 threads: 64, 32, 16 push workgroups per
+    walk workgroup) and both record layouts; the fast unsorted case hits the walk cap.  Same
+    bits, same counters, both density modes."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=41 + wall, n=70001)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=6)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(defer_walk=1, sort_interval=sort_interval, wall_mode=wall, walk_cap=6, scatter_agg=agg,
+                       block_size=block, keep_pid_order=pid_order)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for k in range(4):
+            mode = k % 2
+            mv.step(QM, dt, SPWT, mode)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+        assert c["crossed"] > 0 and (wall == 0 or c["reflections"] > 0)
