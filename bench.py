@@ -69,6 +69,31 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 16.0 * n_nodes)
 
 
+def run_gpu_many(mesh, ef, pos, vel, cell, steps, warmup, options):
+    """The same workload through mp_step_many (frozen field): each stretch of steps between two
+    periodic sorts is one launch with the particles held in registers.  Same options as the
+    per-step line (the multi-step kernel takes 256-thread workgroups, one particle per thread)."""
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(**options)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        mv.step_many(QM, DT, SPWT, mp.SCATTER_ATOMIC, warmup)
+        mv.sync()
+        mv.reset_counters()
+        mv.set_timing(True)
+        t0 = time.perf_counter()
+        mv.step_many(QM, DT, SPWT, mp.SCATTER_ATOMIC, steps)
+        mv.sync()
+        t1 = time.perf_counter()
+        push_ms, launches = mv.timing()
+        mv.set_timing(False)
+        c = mv.counters()
+    wall = t1 - t0
+    return {"value": c["particle_steps"] / wall, "ms_per_step": 1e3 * wall / steps, "steps": steps,
+            "push_ms_per_step": push_ms / steps, "launches": launches,
+            "steps_per_launch": steps / max(launches, 1), "particle_steps": c["particle_steps"]}
+
+
 def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
             tpp=1, tune=False, stage=0, pid_order=0, compact=0, defer=0):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
@@ -377,6 +402,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["atomic", "deterministic"], default="atomic")
     ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--many-steps", type=int, default=100,
+                    help="timed steps of the multi-step (mp_step_many) line (0: skip)")
     ap.add_argument("--det-steps", type=int, default=20, help="timed steps of the deterministic-mode line (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--particles", type=int, default=N_PART)
@@ -418,6 +445,12 @@ def main():
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
     achieved = bytes_per_launch / avg_push_s / 1e9
     extra = {}
+    if args.many_steps > 0 and mode == mp.SCATTER_ATOMIC:
+        m = run_gpu_many(mesh, ef, pos, vel, cell, args.many_steps, args.warmup, r["options"])
+        extra["multi_step"] = dict(m, note="mp_step_many: the steps between two periodic sorts in one launch, "
+                                           "particles kept in registers (frozen field); same options")
+        log(f"gpu atomic multi-step: {m['value'] / 1e9:.3f} G particle-steps/s, push {m['push_ms_per_step']:.3f} "
+            f"ms/step, {m['steps_per_launch']:.1f} steps per launch")
     if args.det_steps > 0 and mode == mp.SCATTER_ATOMIC:
         d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval,
                     shift=args.sort_shift, tune=not args.no_tune, pid_order=args.det_pid_order)

@@ -147,6 +147,17 @@ int  mp_add_particles(mp_ctx* ctx, const double* pos, const double* vel,
 /* One MoveParticles step on the resident set: push + walk + scatter, then the nodal
  * density normalised by spwt / node_volume.  Stream-ordered, does not block the host. */
 int  mp_step(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode);
+/* n_steps consecutive mp_step calls under the field last uploaded (no reference counterpart:
+ * for runs whose field stays frozen).  Atomic mode pushes every stretch of steps between two
+ * periodic sorts in one launch, the particles kept in registers from step to step, so the
+ * state crosses HBM once per launch; aggregation 0, 2 or 4 (1 and 3 run as 2), 256-thread
+ * workgroups, one particle per thread.  Deterministic mode runs the steps one by one.
+ * Particles, counters and sorts are those of n_steps mp_step calls; the normalised density
+ * of every step is kept for mp_get_density_step, and the last one for mp_get_density. */
+int  mp_step_many(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode, int32_t n_steps);
+/* Normalised nodal density of step k (0-based) of the last mp_step_many, den[n_nodes]
+ * (blocks). */
+int  mp_get_density_step(mp_ctx* ctx, int32_t k, double* den);
 /* Block until all queued work on the context's stream is done. */
 int  mp_sync(mp_ctx* ctx);
 /* Number of resident particles still in the domain (blocks). */

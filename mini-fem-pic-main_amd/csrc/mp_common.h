@@ -215,9 +215,9 @@ __device__ __forceinline__ void lds_insert_wave(int32_t* hk, double* hv, int32_t
     }
 }
 
-__device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                                 bool ok, int32_t fc, double l0, double l1, double l2,
-                                                 double l3) {
+__device__ __forceinline__ void scatter_lds_wave_nodes(double* __restrict__ den, bool ok, int32_t n0, int32_t n1,
+                                                       int32_t n2, int32_t n3, double l0, double l1, double l2,
+                                                       double l3) {
     __shared__ int32_t hk_all[256 / kWave][kWaveSlots];
     __shared__ double hv_all[256 / kWave][kWaveSlots];
     const int w = threadIdx.x / kWave;
@@ -233,8 +233,6 @@ __device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (ok) {
-        const DevTet* t = tets + fc;
-        const int32_t n0 = t->con[0], n1 = t->con[1], n2 = t->con[2], n3 = t->con[3];
         lds_insert_wave(hk, hv, n0, l0);
         lds_insert_wave(hk, hv, n1, l1);
         lds_insert_wave(hk, hv, n2, l2);
@@ -252,6 +250,20 @@ __device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets, double* __restrict__ den,
+                                                 bool ok, int32_t fc, double l0, double l1, double l2,
+                                                 double l3) {
+    int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+    if (ok) {
+        const DevTet* t = tets + fc;
+        n0 = t->con[0];
+        n1 = t->con[1];
+        n2 = t->con[2];
+        n3 = t->con[3];
+    }
+    scatter_lds_wave_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
 }
 
 // Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the

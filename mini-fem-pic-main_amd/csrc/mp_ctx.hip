@@ -374,7 +374,7 @@ void mp_destroy(mp_ctx* c) {
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
-                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt};
+                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -593,6 +593,92 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         HIP_TRY(c, mpd::launch_normalise(c->den_raw, c->den_out, c->node_vol, c->n_nodes, spwt, c->stream));
     }
     c->since_sort++;
+    return MP_OK;
+}
+
+int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode, int32_t n_steps) {
+    if (!c) return MP_ERR_ARG;
+    if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
+        return set_err(c, MP_ERR_ARG, "mp_step_many: unknown scatter mode");
+    if (n_steps < 1) return set_err(c, MP_ERR_ARG, "mp_step_many: n_steps must be >= 1");
+    if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_step_many: no field uploaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t nn = c->n_nodes;
+    const int64_t need = nn * (int64_t)n_steps;
+    if (need > c->den_steps_n) {
+        dfree(c->den_steps);
+        c->den_steps = nullptr;
+        c->den_steps_n = 0;
+        if (dalloc(&c->den_steps, need) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "mp_step_many: density buffer allocation failed");
+        c->den_steps_n = need;
+    }
+    c->den_steps_k = 0;
+    if (mode == MP_SCATTER_DETERMINISTIC) {
+        // the sorted-record reduction is per step; only the step loop is saved
+        for (int32_t k = 0; k < n_steps; k++) {
+            int rc = mp_step(c, q_over_m, dt, spwt, mode);
+            if (rc) return rc;
+            HIP_TRY(c, hipMemcpyAsync(c->den_steps + k * nn, c->den_out, sizeof(double) * (size_t)nn,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
+        c->den_steps_k = n_steps;
+        return MP_OK;
+    }
+    HIP_TRY(c, mpd::launch_fill_f64(c->den_steps, need, 0.0, c->stream));
+    int32_t done = 0;
+    while (done < n_steps) {
+        const int32_t si = c->opt.sort_interval;
+        if (si > 0 && c->since_sort >= si) {
+            int rc = sort_now(c);
+            if (rc) return rc;
+        }
+        int32_t chunk = n_steps - done;
+        if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - c->since_sort);
+        int rc = reserve_tally(c, c->n_slots, 256);
+        if (rc) return rc;
+        mpd::PushArgs a;
+        a.tets = c->tets;
+        a.soa = c->soa[c->cur];
+        a.n = c->n_slots;
+        a.qm = q_over_m;
+        a.dt = dt;
+        a.den = c->den_steps + done * nn;
+        a.keys = nullptr;
+        a.vals = nullptr;
+        a.part = c->tally;
+        a.ctr = c->ctr;
+        a.cap = c->opt.walk_cap;
+        a.wall = c->opt.wall_mode;
+        a.det = false;
+        a.agg = c->opt.scatter_agg;
+        a.block = 256;
+        a.ppt = 1;
+        a.tpp = 1;
+        a.stage = false;
+        a.rec_by_slot = false;
+        a.compact = false;
+        if (c->timing) HIP_TRY(c, record_event(c));
+        HIP_TRY(c, mpd::launch_push_steps(a, c->n_nodes, chunk, c->stream));
+        if (c->timing) HIP_TRY(c, record_event(c));
+        c->since_sort += chunk;
+        done += chunk;
+    }
+    HIP_TRY(c, mpd::launch_normalise_steps(c->den_steps, c->node_vol, c->n_nodes, n_steps, spwt, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->den_out, c->den_steps + (int64_t)(n_steps - 1) * nn, sizeof(double) * (size_t)nn,
+                              hipMemcpyDeviceToDevice, c->stream));
+    c->den_steps_k = n_steps;
+    return MP_OK;
+}
+
+int mp_get_density_step(mp_ctx* c, int32_t k, double* den) {
+    if (!c || !den) return MP_ERR_ARG;
+    if (k < 0 || k >= c->den_steps_k)
+        return set_err(c, MP_ERR_ARG, "mp_get_density_step: no such step in the last mp_step_many");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(den, c->den_steps + (int64_t)k * c->n_nodes, sizeof(double) * (size_t)c->n_nodes,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MP_OK;
 }
 

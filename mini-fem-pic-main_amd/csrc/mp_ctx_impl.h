@@ -53,6 +53,10 @@ struct mp_ctx {
     // staging of injected particles (mp_add_particles)
     void* inj_tmp = nullptr;
     size_t inj_bytes = 0;
+    // per-step densities of the last mp_step_many, [steps][n_nodes]
+    double* den_steps = nullptr;
+    int64_t den_steps_n = 0;
+    int32_t den_steps_k = 0;
     // deferred walk: per-wave queues of crossing particles and their counts
     uint32_t* wq = nullptr;
     int32_t* wq_cnt = nullptr;

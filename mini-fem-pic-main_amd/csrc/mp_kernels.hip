@@ -142,6 +142,144 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
         tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
+// ---- several steps per launch under a frozen field ------------------------------------------
+// K consecutive steps of k_push_soa in one launch.  Each thread keeps its particle's state and
+// its current tet's record (coefficients, neighbours, node ids, E) in registers from one step
+// to the next: the state crosses HBM once per launch instead of twice per step, and a step in
+// which the particle stays in its tet loads nothing.  Step k scatters into den + k * nn.  The
+// per-particle arithmetic is k_push_soa's, in the same order, so K launches of k_push_soa and
+// one of these leave the same bits (atomic density sums aside).  Atomic scatters only.
+#ifndef MP_STEPS_WAVES
+#define MP_STEPS_WAVES 4
+#endif
+// AGG 2 here: one LDS hash per workgroup keyed by (step, node) = the offset k * nn + node into
+// the launch's densities, so the steps need no workgroup barrier between them; the table is
+// flushed (one global atomic per occupied slot) every kStepFlush steps and at the end.  A
+// contribution that finds no slot within kStepProbe probes (more distinct keys than the table
+// holds, e.g. unsorted particles) is added to HBM directly.
+constexpr int kStepSlots = 2048;
+constexpr int kStepProbe = 32;
+constexpr int kStepFlush = 10;
+
+__device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __restrict__ den, uint32_t key,
+                                            double w) {
+    uint32_t h = (key * 0x9E3779B1u) >> (32 - 11);
+    for (int p = 0; p < kStepProbe; p++) {
+        const uint32_t prev = atomicCAS(&hk[h], 0xFFFFFFFFu, key);
+        if (prev == 0xFFFFFFFFu || prev == key) {
+            atomicAdd(&hv[h], w);
+            return;
+        }
+        h = (h + 1) & (kStepSlots - 1);
+    }
+    unsafeAtomicAdd(den + key, w);
+}
+
+// Every thread of the workgroup calls it: global atomics for the occupied slots, which are
+// then emptied; barriers on both sides.
+__device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __restrict__ den) {
+    __syncthreads();
+    for (int s = threadIdx.x; s < kStepSlots; s += blockDim.x) {
+        const uint32_t key = hk[s];
+        if (key != 0xFFFFFFFFu) {
+            unsafeAtomicAdd(den + key, hv[s]);
+            hk[s] = 0xFFFFFFFFu;
+            hv[s] = 0.0;
+        }
+    }
+    __syncthreads();
+}
+template <int WALL, int AGG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
+k_push_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, double* __restrict__ den,
+             int32_t nn, int K, int32_t* __restrict__ part, int cap) {
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    int32_t c_in = -1;
+    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    if (i < n) {
+        c_in = s.cell[i];
+        vx = ld(s.vx + i);
+        vy = ld(s.vy + i);
+        vz = ld(s.vz + i);
+        x = ld(s.x + i);
+        y = ld(s.y + i);
+        z = ld(s.z + i);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    bool live = c_in >= 0;
+    int32_t c = live ? c_in : 0;
+    TetRegs t;
+    load_tet_regs(tets + c, t);
+    __shared__ uint32_t hk[AGG == 2 ? kStepSlots : 1];
+    __shared__ double hv[AGG == 2 ? kStepSlots : 1];
+    if (AGG == 2) {
+        for (int q = threadIdx.x; q < kStepSlots; q += blockDim.x) {
+            hk[q] = 0xFFFFFFFFu;
+            hv[q] = 0.0;
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < K; k++) {  // uniform: every thread takes every step (workgroup scatters)
+        bool ok = false;
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        if (live) {
+            n_act++;
+            vx = vx + (qm * t.ex) * dt;
+            vy = vy + (qm * t.ey) * dt;
+            vz = vz + (qm * t.ez) * dt;
+            x = x + vx * dt;
+            y = y + vy * dt;
+            z = z + vz * dt;
+            WalkStats st{0, 0};
+            const int r = walk_regs<WALL>(tets, t, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            n_cross += (st.moves - st.refl) > 0;
+            n_moves += st.moves;
+            n_refl += st.refl;
+            if (r == kInside) {
+                ok = true;
+            } else {
+                live = false;
+                n_lost++;
+                n_cap += (r == kCapped);
+            }
+        }
+        double* dk = den + (int64_t)k * nn;
+        if (AGG == 4) {
+            scatter_lds_wave_nodes(dk, ok, t.m0, t.m1, t.m2, t.m3, l0, l1, l2, l3);
+        } else if (AGG == 2) {
+            if (ok) {
+                const uint32_t base = (uint32_t)k * (uint32_t)nn;
+                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
+                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
+                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
+                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
+            }
+            if ((k + 1) % kStepFlush == 0 || k + 1 == K) step_flush(hk, hv, den);
+        } else if (ok) {
+            unsafeAtomicAdd(dk + t.m0, l0);
+            unsafeAtomicAdd(dk + t.m1, l1);
+            unsafeAtomicAdd(dk + t.m2, l2);
+            unsafeAtomicAdd(dk + t.m3, l3);
+        }
+    }
+    if (c_in >= 0) {
+        s.cell[i] = live ? c : -1;
+        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+    }
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
+// den[k * nn + j] *= spwt / nvol[j] for the K steps of a k_push_steps launch (k_normalise's
+// expression; the raw sums are replaced in place).
+__global__ void k_normalise_steps(double* __restrict__ den, const double* __restrict__ nvol, int32_t nn,
+                                  int64_t total, double spwt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    den[i] = den[i] * (spwt / nvol[i % nn]);
+}
+
 // ---- deferred walk ------------------------------------------------------------------------
 // Walks the particles k_push_soa<..., DEFER> listed, in full waves.  Workgroup b takes 64
 // wave slots of the push grid: those of push workgroups h = (k * G + t) * 8 + xcd, t < G,
@@ -1259,6 +1397,39 @@ static hipError_t launch_push_defer(const PushArgs& a, hipStream_t st) {
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_tally_finish(a.part, gw, a.ctr, st);
+}
+
+hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStream_t st) {
+    if (a.n <= 0 || k_steps <= 0) return hipSuccess;
+    // (step, node) keys are 32-bit offsets below the empty-slot marker
+    if ((int64_t)nn * k_steps >= (int64_t)0xFFFFFFFFu) return hipErrorInvalidValue;
+    const unsigned g = (grid_for(a.n, 256) + kXcds - 1) / kXcds * kXcds;
+#define MP_STEPS(W, A)                                                                                    \
+    hipLaunchKernelGGL((k_push_steps<W, A>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, a.den, \
+                       nn, k_steps, a.part, a.cap)
+    const int agg = a.agg == 4 ? 4 : a.agg == 0 ? 0 : 2;
+    if (a.wall == 0) {
+        if (agg == 4) MP_STEPS(0, 4);
+        else if (agg == 0) MP_STEPS(0, 0);
+        else MP_STEPS(0, 2);
+    } else {
+        if (agg == 4) MP_STEPS(1, 4);
+        else if (agg == 0) MP_STEPS(1, 0);
+        else MP_STEPS(1, 2);
+    }
+#undef MP_STEPS
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_tally_finish(a.part, g, a.ctr, st);
+}
+
+hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, int k_steps, double spwt,
+                                  hipStream_t st) {
+    const int64_t total = (int64_t)nn * k_steps;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_normalise_steps, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, den, nvol, nn,
+                       total, spwt);
+    return hipGetLastError();
 }
 
 // Grid and queue sizes of the deferred walk for n particles in workgroups of `block`: queue

@@ -84,6 +84,8 @@ SIGNATURES = {
     "mp_set_particles": (C.c_int, [_P, _P, _P, _P, _I64]),
     "mp_add_particles": (C.c_int, [_P, _P, _P, _P, _I64]),
     "mp_step": (C.c_int, [_P, _D, _D, _D, _I32]),
+    "mp_step_many": (C.c_int, [_P, _D, _D, _D, _I32, _I32]),
+    "mp_get_density_step": (C.c_int, [_P, _I32, _P]),
     "mp_sync": (C.c_int, [_P]),
     "mp_num_active": (C.c_int, [_P, C.POINTER(_I64)]),
     "mp_get_particles": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(_I64)]),
@@ -513,6 +515,16 @@ class Mover:
 
     def step(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC):
         self._check(lib().mp_step(self._h, q_over_m, dt, spwt, mode))
+
+    def step_many(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC, n_steps=1):
+        """n_steps steps under the frozen field; atomic mode keeps the particles in registers
+        between the steps of a sort interval (one launch per interval)."""
+        self._check(lib().mp_step_many(self._h, q_over_m, dt, spwt, mode, n_steps))
+
+    def density_step(self, k):
+        den = np.empty(self.n_nodes, np.float64)
+        self._check(lib().mp_get_density_step(self._h, k, _ptr(den)))
+        return den
 
     def sync(self):
         self._check(lib().mp_sync(self._h))

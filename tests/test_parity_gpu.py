@@ -572,3 +572,60 @@ def test_defer_walk_matches_oracle(mp, orc, wall, agg, block, pid_order, dt, sor
         for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
             assert c[key] == run.stats[key], key
         assert c["crossed"] > 0 and (wall == 0 or c["reflections"] > 0)
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+@pytest.mark.parametrize("agg", [2, 0, 4, 1], ids=["lds", "plain", "wave", "grouped-as-lds"])
+@pytest.mark.parametrize("dt,sort_interval,n_steps", [(4e-7, 3, 7), (1.5e-6, 0, 5)],
+                         ids=["sorted-3", "fast-unsorted"])
+def test_step_many_matches_oracle(mp, orc, wall, agg, dt, sort_interval, n_steps):
+    """mp_step_many: the steps between two sorts in one launch, the particles kept in
+    registers (7 steps with a sort every 3 = launches of 2, 3 and 2 steps after the first
+    step's carried-over interval; the fast unsorted case runs 5 steps in one launch and hits
+    the walk cap).  Every step's density, the particles and the counters against the oracle
+    stepped one step at a time."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=53 + wall, n=70001)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=6)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=sort_interval, wall_mode=wall, walk_cap=6, scatter_agg=agg)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        mv.step(QM, dt, SPWT, 0)  # one ordinary step first: the multi-step call starts mid-interval
+        check_den(mv.density(), run.step(QM, dt, SPWT), 0, mp)
+        mv.step_many(QM, dt, SPWT, 0, n_steps)
+        for k in range(n_steps):
+            ref = run.step(QM, dt, SPWT)
+            check_den(mv.density_step(k), ref, 0, mp)
+        check_den(mv.density(), ref, 0, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+        assert c["crossed"] > 0 and (wall == 0 or c["reflections"] > 0)
+        with pytest.raises(mp.MpError):
+            mv.density_step(n_steps)
+
+
+def test_step_many_deterministic_and_timing(mp, orc):
+    """Deterministic mode through mp_step_many (one step at a time): every step's density
+    bit-exact; the push timing counts one launch per multi-step stretch in atomic mode."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=59, n=50001)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=4)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        mv.step_many(QM, 4e-7, SPWT, 1, 3)
+        for k in range(3):
+            check_den(mv.density_step(k), run.step(QM, 4e-7, SPWT), 1, mp)
+        mv.set_timing(True)
+        mv.step_many(QM, 4e-7, SPWT, 0, 6)  # 1 step to the sort, then 4, then 1
+        for k in range(6):
+            check_den(mv.density_step(k), run.step(QM, 4e-7, SPWT), 0, mp)
+        ms, launches = mv.timing()
+        mv.set_timing(False)
+        assert launches == 3 and ms > 0
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
