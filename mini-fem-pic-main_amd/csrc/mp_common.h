@@ -142,41 +142,52 @@ __device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets,
 // Atomic scatter aggregated per workgroup by node: the block's (node, weight) pairs are summed
 // in an LDS hash table (open addressing, linear probing, ds_cmpst + ds_add_f64), then each
 // occupied slot issues one global atomic.  A block of 256 cell-sorted particles touches a few
-// dozen nodes, so ~1024 contributions become ~50 global atomics, however the particles
-// reordered since the last sort.  Every thread of the block must call it.
-constexpr int kHashSlots = 1024;  // >= 4 * max block size: every insert finds a slot
+// dozen nodes, so ~1024 contributions become ~40 global atomics, however the particles
+// reordered since the last sort.  A contribution that finds no slot within kNodeProbe probes
+// (unsorted particles on a large mesh) is added to HBM directly.  Every thread of the block
+// must call it.
+constexpr int kHashSlots = 1024;  // tables keyed by tet (one insert per particle)
+// 256 slots: ~40 nodes per workgroup after a cell sort; a smaller table is cheaper to clear
+// and flush (measured 2% on the whole push against 1024 slots).
+#ifndef MP_NODE_HASH_BITS
+#define MP_NODE_HASH_BITS 8
+#endif
+constexpr int kNodeSlots = 1 << MP_NODE_HASH_BITS;
+constexpr int kNodeProbe = 32;
 
-__device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, int32_t node, double w) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 10);
-    for (;;) {
+__device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, double* __restrict__ den, int32_t node,
+                                           double w) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - MP_NODE_HASH_BITS);
+    for (int p = 0; p < kNodeProbe; p++) {
         const int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
             atomicAdd(&hv[h], w);
             return;
         }
-        h = (h + 1) & (kHashSlots - 1);
+        h = (h + 1) & (kNodeSlots - 1);
     }
+    unsafeAtomicAdd(den + node, w);
 }
 
 // The same over the final tet's node ids (already in registers).
 __device__ __forceinline__ void scatter_lds_nodes(double* __restrict__ den, bool ok, int32_t n0, int32_t n1,
                                                   int32_t n2, int32_t n3, double l0, double l1, double l2,
                                                   double l3) {
-    __shared__ int32_t hk[kHashSlots];
-    __shared__ double hv[kHashSlots];
-    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+    __shared__ int32_t hk[kNodeSlots];
+    __shared__ double hv[kNodeSlots];
+    for (int s = threadIdx.x; s < kNodeSlots; s += blockDim.x) {
         hk[s] = -1;
         hv[s] = 0.0;
     }
     __syncthreads();
     if (ok) {
-        lds_insert(hk, hv, n0, l0);
-        lds_insert(hk, hv, n1, l1);
-        lds_insert(hk, hv, n2, l2);
-        lds_insert(hk, hv, n3, l3);
+        lds_insert(hk, hv, den, n0, l0);
+        lds_insert(hk, hv, den, n1, l1);
+        lds_insert(hk, hv, den, n2, l2);
+        lds_insert(hk, hv, den, n3, l3);
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+    for (int s = threadIdx.x; s < kNodeSlots; s += blockDim.x) {
         const int32_t node = hk[s];
         if (node >= 0) unsafeAtomicAdd(den + node, hv[s]);
     }

@@ -590,27 +590,27 @@ k_push_compact(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, dou
         st_(s.vx + ip, pvx); st_(s.vy + ip, pvy); st_(s.vz + ip, pvz);
     }
     if (!DET) {
-        __shared__ int32_t hk[kHashSlots];
-        __shared__ double hv[kHashSlots];
-        for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+        __shared__ int32_t hk[kNodeSlots];
+        __shared__ double hv[kNodeSlots];
+        for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
             hk[q] = -1;
             hv[q] = 0.0;
         }
         __syncthreads();
         if (ok1) {
-            lds_insert(hk, hv, a0, l0);
-            lds_insert(hk, hv, a1, l1);
-            lds_insert(hk, hv, a2, l2);
-            lds_insert(hk, hv, a3, l3);
+            lds_insert(hk, hv, den, a0, l0);
+            lds_insert(hk, hv, den, a1, l1);
+            lds_insert(hk, hv, den, a2, l2);
+            lds_insert(hk, hv, den, a3, l3);
         }
         if (ok2) {
-            lds_insert(hk, hv, b0, m0);
-            lds_insert(hk, hv, b1, m1);
-            lds_insert(hk, hv, b2, m2);
-            lds_insert(hk, hv, b3, m3);
+            lds_insert(hk, hv, den, b0, m0);
+            lds_insert(hk, hv, den, b1, m1);
+            lds_insert(hk, hv, den, b2, m2);
+            lds_insert(hk, hv, den, b3, m3);
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+        for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
             const int32_t node = hk[q];
             if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
         }
@@ -634,16 +634,16 @@ __device__ __forceinline__ double quad_bcast(double v) {
 }
 
 __device__ __forceinline__ void lds_scatter_one(double* __restrict__ den, bool ok, int32_t node, double w) {
-    __shared__ int32_t hk[kHashSlots];
-    __shared__ double hv[kHashSlots];
-    for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+    __shared__ int32_t hk[kNodeSlots];
+    __shared__ double hv[kNodeSlots];
+    for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
         hk[q] = -1;
         hv[q] = 0.0;
     }
     __syncthreads();
-    if (ok) lds_insert(hk, hv, node, w);
+    if (ok) lds_insert(hk, hv, den, node, w);
     __syncthreads();
-    for (int q = threadIdx.x; q < kHashSlots; q += blockDim.x) {
+    for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
         const int32_t nd = hk[q];
         if (nd >= 0) unsafeAtomicAdd(den + nd, hv[q]);
     }

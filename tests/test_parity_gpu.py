@@ -629,3 +629,24 @@ def test_step_many_deterministic_and_timing(mp, orc):
         assert launches == 3 and ms > 0
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+
+
+@pytest.mark.parametrize("opts", [dict(scatter_agg=2), dict(compact_crossers=1), dict(stage_tets=1),
+                                  dict(threads_per_particle=4)],
+                         ids=["lds-hash", "compact", "staged", "tpp4"])
+def test_node_hash_overflow_unsorted_large_mesh(mp, orc, opts):
+    """Unsorted particles on a 20^3 box (9261 nodes): a workgroup's 256 particles bring up to
+    1024 distinct nodes (1536 with the compaction queue's second set), more than the node hash
+    holds within its probe limit, so contributions spill to direct HBM atomics.  Same bits and
+    densities as the oracle."""
+    mesh, ef, pos, vel, cell = box_case(mp, n_hex=20, side=0.1, n=40001, seed=61)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=0, **opts)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for _ in range(2):
+            mv.step(QM, 4e-7, SPWT, 0)
+            check_den(mv.density(), run.step(QM, 4e-7, SPWT), 0, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gc, run.cell) and gp.tobytes() == run.pos.tobytes()
