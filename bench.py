@@ -152,18 +152,24 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(sort_interval=10, sort_shift=3)
         mv.upload_field(ef)
-        for how in ("batched", "sequential"):
+        for how in ("batched", "sequential", "multi_step"):
             with mp.Batch(mv, n_runs) as b:
                 for r, (pos, vel, cell) in enumerate(runs):
                     b.set_run(r, pos, vel, cell, QM, dt, SPWT)
-                go = b.step if how == "batched" else b.step_sequential
-                for _ in range(warmup):
-                    go()
+                if how == "multi_step":
+                    b.step_many(warmup)
+                else:
+                    go = b.step if how == "batched" else b.step_sequential
+                    for _ in range(warmup):
+                        go()
                 b.sync()
                 b.reset_counters()
                 t0 = time.perf_counter()
-                for _ in range(steps):
-                    go()
+                if how == "multi_step":
+                    b.step_many(steps)
+                else:
+                    for _ in range(steps):
+                        go()
                 b.sync()
                 wall = time.perf_counter() - t0
                 c = b.counters()
@@ -174,10 +180,15 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
             "sequential_value": out["sequential"]["value"],
             "sequential_ms_per_step": out["sequential"]["ms_per_step"],
             "speedup_vs_sequential": out["batched"]["value"] / out["sequential"]["value"],
+            "multi_step_value": out["multi_step"]["value"],
+            "multi_step_ms_per_step": out["multi_step"]["ms_per_step"],
+            "multi_step_speedup_vs_sequential": out["multi_step"]["value"] / out["sequential"]["value"],
             "steps": steps, "runs": n_runs, "particles": int(counts.sum()),
             "workload": "64 runs on a 40^3 six-tet box (384,000 tets), counts log-uniform 1e5..4e6, "
                         "drift U[2,8] km/s, sigma U[500,3000] m/s, frozen sinusoidal field, dt 4e-7 s; "
-                        "one hipGraph launch per step vs 64 per-run launch sequences"}
+                        "one hipGraph launch per step vs 64 per-run launch sequences; multi_step: "
+                        "mp_batch_step_many, one launch per sort interval (10 steps) with the particles in "
+                        "registers"}
 
 
 def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3):

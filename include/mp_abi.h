@@ -219,6 +219,15 @@ int  mp_batch_set_run(mp_batch* b, int32_t run, const double* pos, const double*
 int  mp_batch_step(mp_batch* b, int32_t mode);
 /* The same step as n_runs separate per-run launches (the sequential baseline). */
 int  mp_batch_step_sequential(mp_batch* b, int32_t mode);
+/* n_steps mp_batch_step calls under the frozen field; atomic mode pushes every stretch of
+ * steps between two periodic sorts in one launch with the particles kept in registers (as
+ * mp_step_many does), deterministic mode steps one by one.  Every step's densities are kept
+ * for mp_batch_get_density_step, the last step's for mp_batch_get_density.
+ * n_runs * n_nodes must stay below 2^32. */
+int  mp_batch_step_many(mp_batch* b, int32_t mode, int32_t n_steps);
+/* Normalised density of run `run` at step k (0-based) of the last mp_batch_step_many,
+ * den[n_nodes] (blocks). */
+int  mp_batch_get_density_step(mp_batch* b, int32_t k, int32_t run, double* den);
 int  mp_batch_set_graph(mp_batch* b, int32_t enable);
 int  mp_batch_sync(mp_batch* b);
 /* Normalised density of run `run` after the last step, den[n_nodes] (blocks). */

@@ -76,6 +76,90 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
+// K steps of k_push_batch in one launch, as k_push_steps does for one run: the particle and
+// its tet's record stay in registers between the steps; step k scatters into den + k * R * nn
+// (run r's nodes at r * nn) through the (step, node)-keyed LDS hash.  Atomic scatter only.
+template <int WALL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ par, int32_t nn,
+                   int32_t R, double* __restrict__ den, int K, int32_t* __restrict__ part, int cap) {
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    int32_t c_in = -1, r = 0;
+    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    if (i < n) {
+        c_in = s.cell[i];
+        r = s.run[i];
+        vx = ld(s.vx + i);
+        vy = ld(s.vy + i);
+        vz = ld(s.vz + i);
+        x = ld(s.x + i);
+        y = ld(s.y + i);
+        z = ld(s.z + i);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    bool live = c_in >= 0;
+    int32_t c = live ? c_in : 0;
+    const double qm = par[3 * r], dt = par[3 * r + 1];
+    TetRegs t;
+    load_tet_regs(tets + c, t);
+    __shared__ uint32_t hk[kStepSlots];
+    __shared__ double hv[kStepSlots];
+    for (int q = threadIdx.x; q < kStepSlots; q += blockDim.x) {
+        hk[q] = 0xFFFFFFFFu;
+        hv[q] = 0.0;
+    }
+    __syncthreads();
+    const uint32_t per_step = (uint32_t)R * (uint32_t)nn;
+    for (int k = 0; k < K; k++) {  // uniform over the workgroup
+        if (live) {
+            n_act++;
+            vx = vx + (qm * t.ex) * dt;
+            vy = vy + (qm * t.ey) * dt;
+            vz = vz + (qm * t.ez) * dt;
+            x = x + vx * dt;
+            y = y + vy * dt;
+            z = z + vz * dt;
+            WalkStats st{0, 0};
+            double l0, l1, l2, l3;
+            const int rr = walk_regs<WALL>(tets, t, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            n_cross += (st.moves - st.refl) > 0;
+            n_moves += st.moves;
+            n_refl += st.refl;
+            if (rr == kInside) {
+                const uint32_t base = (uint32_t)k * per_step + (uint32_t)r * (uint32_t)nn;
+                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
+                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
+                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
+                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
+            } else {
+                live = false;
+                n_lost++;
+                n_cap += (rr == kCapped);
+            }
+        }
+        if ((k + 1) % kStepFlush == 0 || k + 1 == K) step_flush(hk, hv, den);
+    }
+    if (c_in >= 0) {
+        s.cell[i] = live ? c : -1;
+        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
+    }
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+}
+
+// den[q] *= spwt of the run / node volume over the K steps of a k_push_batch_steps launch,
+// q = k * R * nn + run * nn + node (k_normalise_batch's expression, in place).
+__global__ void k_normalise_batch_steps(double* __restrict__ den, const double* __restrict__ nvol,
+                                        const double* __restrict__ par, int32_t nn, int64_t per_step,
+                                        int64_t total) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int64_t rk = q % per_step;
+    const int64_t r = rk / nn;
+    den[q] = den[q] * (par[3 * r + 2] / nvol[rk - r * nn]);
+}
+
 // den[k] = raw[k] * (spwt of run k / nn) / node volume, raw cleared; k in [k0, k0 + count).
 __global__ void k_normalise_batch(double* __restrict__ raw, double* __restrict__ out,
                                   const double* __restrict__ nvol, const double* __restrict__ par,
@@ -155,6 +239,10 @@ struct mp_batch {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int64_t graph_slots[2] = {-1, -1};
+    // per-step densities of the last mp_batch_step_many, [steps][R * nn]
+    double* den_steps = nullptr;
+    int64_t den_steps_n = 0;
+    int32_t den_steps_k = 0;
 };
 
 namespace {
@@ -384,7 +472,7 @@ void mp_batch_destroy(mp_batch* b) {
     drop_graphs(b);
     void* bufs[] = {b->soa_mem[0], b->soa_mem[1], b->run_mem[0], b->run_mem[1], b->n_dev, b->par_d, b->den_raw,
                     b->den_out, b->ctr, b->tally, b->cnt, b->cursor, b->runs_d, b->keys[0], b->keys[1], b->vals[0],
-                    b->vals[1], b->sort_tmp, b->seg_bounds};
+                    b->vals[1], b->sort_tmp, b->seg_bounds, b->den_steps};
     for (void* p : bufs) dfree(p);
     if (b->pinned) (void)hipHostFree(b->pinned);
     delete b;
@@ -475,6 +563,86 @@ int mp_batch_step(mp_batch* b, int32_t mode) {
         b->graph_slots[k] = b->n_slots;
     }
     HIP_TRY(c, hipGraphLaunch(b->exec[k], c->stream));
+    return MP_OK;
+}
+
+int mp_batch_step_many(mp_batch* b, int32_t mode, int32_t n_steps) {
+    if (!b) return MP_ERR_ARG;
+    mp_ctx* c = b->ctx;
+    if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
+        return set_err(c, MP_ERR_ARG, "mp_batch_step_many: unknown scatter mode");
+    if (n_steps < 1) return set_err(c, MP_ERR_ARG, "mp_batch_step_many: n_steps must be >= 1");
+    int rc = check_ready(b);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t per_step = (int64_t)b->R * b->nn;
+    const int64_t need = per_step * n_steps;
+    if (need > b->den_steps_n) {
+        dfree(b->den_steps);
+        b->den_steps = nullptr;
+        b->den_steps_n = 0;
+        if (dalloc(&b->den_steps, need) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "mp_batch_step_many: density buffer allocation failed");
+        b->den_steps_n = need;
+    }
+    b->den_steps_k = 0;
+    if (mode == MP_SCATTER_DETERMINISTIC) {
+        for (int32_t k = 0; k < n_steps; k++) {
+            rc = mp_batch_step(b, mode);
+            if (rc) return rc;
+            HIP_TRY(c, hipMemcpyAsync(b->den_steps + k * per_step, b->den_out, sizeof(double) * (size_t)per_step,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
+        b->den_steps_k = n_steps;
+        return MP_OK;
+    }
+    HIP_TRY(c, mpd::launch_fill_f64(b->den_steps, need, 0.0, c->stream));
+    int32_t done = 0;
+    while (done < n_steps) {
+        rc = maybe_sort(b);
+        if (rc) return rc;
+        const int32_t si = c->opt.sort_interval;
+        int32_t chunk = n_steps - done;
+        if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - b->since_sort);
+        // (step, node) keys of one launch are 32-bit offsets below the empty-slot marker
+        chunk = (int32_t)std::min<int64_t>(chunk, std::max<int64_t>(1, (int64_t)0xFFFFFFFEu / per_step));
+        if (per_step * chunk >= (int64_t)0xFFFFFFFFu)
+            return set_err(c, MP_ERR_ARG, "mp_batch_step_many: n_runs * n_nodes too large for the step keys");
+        rc = reserve_tally_b(b, b->n_slots);
+        if (rc) return rc;
+        if (b->n_slots > 0) {
+            const unsigned g = (unsigned)(((b->n_slots + 255) / 256 + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds);
+#define MP_BS(W) hipLaunchKernelGGL((mpd::k_push_batch_steps<W>), dim3(g), dim3(256), 0, c->stream, c->tets, \
+                                    b->soa[b->cur], b->n_slots, b->par_d, b->nn, b->R, \
+                                    b->den_steps + (int64_t)done * per_step, chunk, b->tally, c->opt.walk_cap)
+            if (c->opt.wall_mode == MP_WALL_ABSORB) MP_BS(0);
+            else MP_BS(1);
+#undef MP_BS
+            HIP_TRY(c, hipGetLastError());
+            HIP_TRY(c, mpd::launch_tally_finish(b->tally, g, b->ctr, c->stream));
+        }
+        b->since_sort += chunk;
+        done += chunk;
+    }
+    hipLaunchKernelGGL(mpd::k_normalise_batch_steps, dim3((unsigned)((need + 255) / 256)), dim3(256), 0, c->stream,
+                       b->den_steps, c->node_vol, b->par_d, b->nn, per_step, need);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(b->den_out, b->den_steps + (int64_t)(n_steps - 1) * per_step,
+                              sizeof(double) * (size_t)per_step, hipMemcpyDeviceToDevice, c->stream));
+    b->den_steps_k = n_steps;
+    return MP_OK;
+}
+
+int mp_batch_get_density_step(mp_batch* b, int32_t k, int32_t run, double* den) {
+    if (!b || !den) return MP_ERR_ARG;
+    mp_ctx* c = b->ctx;
+    if (k < 0 || k >= b->den_steps_k)
+        return set_err(c, MP_ERR_ARG, "mp_batch_get_density_step: no such step in the last mp_batch_step_many");
+    if (run < 0 || run >= b->R) return set_err(c, MP_ERR_ARG, "mp_batch_get_density_step: bad run");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(den, b->den_steps + ((int64_t)k * b->R + run) * b->nn, sizeof(double) * (size_t)b->nn,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MP_OK;
 }
 

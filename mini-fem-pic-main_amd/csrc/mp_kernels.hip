@@ -152,43 +152,7 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 #ifndef MP_STEPS_WAVES
 #define MP_STEPS_WAVES 4
 #endif
-// AGG 2 here: one LDS hash per workgroup keyed by (step, node) = the offset k * nn + node into
-// the launch's densities, so the steps need no workgroup barrier between them; the table is
-// flushed (one global atomic per occupied slot) every kStepFlush steps and at the end.  A
-// contribution that finds no slot within kStepProbe probes (more distinct keys than the table
-// holds, e.g. unsorted particles) is added to HBM directly.
-constexpr int kStepSlots = 2048;
-constexpr int kStepProbe = 32;
-constexpr int kStepFlush = 10;
-
-__device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __restrict__ den, uint32_t key,
-                                            double w) {
-    uint32_t h = (key * 0x9E3779B1u) >> (32 - 11);
-    for (int p = 0; p < kStepProbe; p++) {
-        const uint32_t prev = atomicCAS(&hk[h], 0xFFFFFFFFu, key);
-        if (prev == 0xFFFFFFFFu || prev == key) {
-            atomicAdd(&hv[h], w);
-            return;
-        }
-        h = (h + 1) & (kStepSlots - 1);
-    }
-    unsafeAtomicAdd(den + key, w);
-}
-
-// Every thread of the workgroup calls it: global atomics for the occupied slots, which are
-// then emptied; barriers on both sides.
-__device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __restrict__ den) {
-    __syncthreads();
-    for (int s = threadIdx.x; s < kStepSlots; s += blockDim.x) {
-        const uint32_t key = hk[s];
-        if (key != 0xFFFFFFFFu) {
-            unsafeAtomicAdd(den + key, hv[s]);
-            hk[s] = 0xFFFFFFFFu;
-            hv[s] = 0.0;
-        }
-    }
-    __syncthreads();
-}
+// AGG 2 here: the (step, node)-keyed LDS hash of step_insert / step_flush.
 template <int WALL, int AGG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
 k_push_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, double* __restrict__ den,

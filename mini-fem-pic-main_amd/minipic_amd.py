@@ -105,6 +105,8 @@ SIGNATURES = {
     "mp_batch_set_run": (C.c_int, [_P, _I32, _P, _P, _P, _I64, _D, _D, _D]),
     "mp_batch_step": (C.c_int, [_P, _I32]),
     "mp_batch_step_sequential": (C.c_int, [_P, _I32]),
+    "mp_batch_step_many": (C.c_int, [_P, _I32, _I32]),
+    "mp_batch_get_density_step": (C.c_int, [_P, _I32, _I32, _P]),
     "mp_batch_set_graph": (C.c_int, [_P, _I32]),
     "mp_batch_sync": (C.c_int, [_P]),
     "mp_batch_get_density": (C.c_int, [_P, _I32, _P]),
@@ -642,6 +644,16 @@ class Batch:
 
     def step_sequential(self, mode=SCATTER_ATOMIC):
         self.mover._check(lib().mp_batch_step_sequential(self._h, mode))
+
+    def step_many(self, n_steps, mode=SCATTER_ATOMIC):
+        """n_steps steps of every run; atomic mode keeps the particles in registers between the
+        steps of a sort interval (one launch per interval)."""
+        self.mover._check(lib().mp_batch_step_many(self._h, mode, n_steps))
+
+    def density_step(self, k, run):
+        den = np.empty(self.n_nodes, np.float64)
+        self.mover._check(lib().mp_batch_get_density_step(self._h, k, run, _ptr(den)))
+        return den
 
     def set_graph(self, enable=True):
         self.mover._check(lib().mp_batch_set_graph(self._h, int(bool(enable))))

@@ -650,3 +650,34 @@ def test_node_hash_overflow_unsorted_large_mesh(mp, orc, opts):
             check_den(mv.density(), run.step(QM, 4e-7, SPWT), 0, mp)
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gc, run.cell) and gp.tobytes() == run.pos.tobytes()
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "deterministic"])
+def test_batch_step_many_matches_per_run_oracle(mp, orc, mode):
+    """mp_batch_step_many: every run's density at every step, its particles and the counters
+    against its own serial MoveParticles; sort every 3 steps, 7 steps (launches of 3, 3, 1)."""
+    mesh, ef, runs = _ensemble(mp)
+    oracles = [orc.Run(mesh.tets, ef, mesh.node_volume, r["pos"], r["vel"], r["cell"]) for r in runs]
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=3, sort_shift=2)
+        mv.upload_field(ef)
+        with mp.Batch(mv, len(runs)) as b:
+            for k, r in enumerate(runs):
+                b.set_run(k, r["pos"], r["vel"], r["cell"], r["qm"], r["dt"], r["spwt"])
+            b.step_many(7, mode)
+            for step in range(7):
+                for k, r in enumerate(runs):
+                    ref = oracles[k].step(r["qm"], r["dt"], r["spwt"])
+                    check_den(b.density_step(step, k), ref, mode, mp)
+                    if step == 6:
+                        check_den(b.density(k), ref, mode, mp)
+            for k in range(len(runs)):
+                gp, gv, gc, gid = b.get_particles(k)
+                o = oracles[k]
+                assert np.array_equal(gid, o.pid) and np.array_equal(gc, o.cell)
+                assert gp.tobytes() == o.pos.tobytes() and gv.tobytes() == o.vel.tobytes()
+            c = b.counters()
+            assert c["particle_steps"] == sum(o.stats["particle_steps"] for o in oracles)
+            assert c["lost"] == sum(o.stats["lost"] for o in oracles) > 0
+            with pytest.raises(mp.MpError):
+                b.density_step(7, 0)
