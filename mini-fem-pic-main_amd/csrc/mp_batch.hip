@@ -76,9 +76,10 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
 }
 
-// K steps of k_push_batch in one launch, as k_push_steps does for one run: the particle and
-// its tet's record stay in registers between the steps; step k scatters into den + k * R * nn
-// (run r's nodes at r * nn) through the (step, node)-keyed LDS hash.  Atomic scatter only.
+// K <= kStepMaxLaunch steps of k_push_batch in one launch, as k_push_steps does for one run:
+// the particle and its tet's record stay in registers, each lane runs its steps on its own
+// (lane_steps); step k scatters into den + k * R * nn (run r's nodes at r * nn) through the
+// (step, node)-keyed LDS hash.  Atomic scatter only.
 template <int WALL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ par, int32_t nn,
@@ -110,36 +111,9 @@ k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const doub
         hv[q] = 0.0;
     }
     __syncthreads();
-    const uint32_t per_step = (uint32_t)R * (uint32_t)nn;
-    for (int k = 0; k < K; k++) {  // uniform over the workgroup
-        if (live) {
-            n_act++;
-            vx = vx + (qm * t.ex) * dt;
-            vy = vy + (qm * t.ey) * dt;
-            vz = vz + (qm * t.ez) * dt;
-            x = x + vx * dt;
-            y = y + vy * dt;
-            z = z + vz * dt;
-            WalkStats st{0, 0};
-            double l0, l1, l2, l3;
-            const int rr = walk_regs<WALL>(tets, t, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
-            n_cross += (st.moves - st.refl) > 0;
-            n_moves += st.moves;
-            n_refl += st.refl;
-            if (rr == kInside) {
-                const uint32_t base = (uint32_t)k * per_step + (uint32_t)r * (uint32_t)nn;
-                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
-                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
-                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
-                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
-            } else {
-                live = false;
-                n_lost++;
-                n_cap += (rr == kCapped);
-            }
-        }
-        if ((k + 1) % kStepFlush == 0 || k + 1 == K) step_flush(hk, hv, den);
-    }
+    lane_steps<WALL, true>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den, (uint32_t)r * (uint32_t)nn,
+                           (uint32_t)R * (uint32_t)nn, hk, hv, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+    step_flush(hk, hv, den);
     if (c_in >= 0) {
         s.cell[i] = live ? c : -1;
         st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
@@ -605,6 +579,7 @@ int mp_batch_step_many(mp_batch* b, int32_t mode, int32_t n_steps) {
         int32_t chunk = n_steps - done;
         if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - b->since_sort);
         // (step, node) keys of one launch are 32-bit offsets below the empty-slot marker
+        chunk = (int32_t)std::min<int64_t>(chunk, mpd::kStepMaxLaunch);
         chunk = (int32_t)std::min<int64_t>(chunk, std::max<int64_t>(1, (int64_t)0xFFFFFFFEu / per_step));
         if (per_step * chunk >= (int64_t)0xFFFFFFFFu)
             return set_err(c, MP_ERR_ARG, "mp_batch_step_many: n_runs * n_nodes too large for the step keys");

@@ -326,13 +326,12 @@ __device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, dou
 
 // Multi-step kernels: one LDS hash per workgroup keyed by (step, node) = the offset of the
 // contribution in the launch's densities (step k of a launch at k * nodes per step), so the
-// steps need no workgroup barrier between them; the table is flushed (one global atomic per
-// occupied slot) every kStepFlush steps and at the end.  A contribution that finds no slot
-// within kStepProbe probes (more distinct keys than the table holds, e.g. unsorted particles)
-// is added to HBM directly.
+// steps need no workgroup barrier between them and lanes may be at different steps; the table
+// is flushed (one global atomic per occupied slot) once, at the end of the launch, which holds
+// at most kStepMaxLaunch steps.  A contribution that finds no slot within kStepProbe probes
+// (more distinct keys than the table holds, e.g. unsorted particles) is added to HBM directly.
 constexpr int kStepSlots = 2048;
 constexpr int kStepProbe = 32;
-constexpr int kStepFlush = 10;
 
 __device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __restrict__ den, uint32_t key,
                                             double w) {
@@ -361,6 +360,106 @@ __device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __r
         }
     }
     __syncthreads();
+}
+
+// K steps of one particle whose tet record t is in registers, each lane on its own: one
+// iteration of the loop is one barycentric evaluation (preceded by the push when the lane
+// starts a step); a lane whose particle moves loads the next tet's record for its next
+// iteration, and a lane that finishes a step goes on with the next one while other lanes still
+// walk.  A wave thus takes about max over its lanes of (K + moves) iterations instead of the
+// sum over the steps of its slowest walk.  The arithmetic of every step is walk_regs' after the
+// push, in the same order.  Step k's weights go to key0 + k * per_step + node: the LDS hash
+// (HASH) or direct atomics on den.  Counters as in k_push_soa.
+template <int WALL, bool HASH>
+__device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetRegs& t, int32_t& c, bool& live,
+                                           double& x, double& y, double& z, double& vx, double& vy, double& vz,
+                                           double qm, double dt, int K, int cap, double* __restrict__ den,
+                                           uint32_t key0, uint32_t per_step, uint32_t* hk, double* hv, int& n_act,
+                                           int& n_lost, int& n_cross, int& n_moves, int& n_cap, int& n_refl) {
+    int k = 0;
+    bool start = true;
+    WalkStats st{0, 0};
+    bool active = live && K > 0;
+    while (active) {
+        if (start) {
+            n_act++;
+            vx = vx + (qm * t.ex) * dt;
+            vy = vy + (qm * t.ey) * dt;
+            vz = vz + (qm * t.ez) * dt;
+            x = x + vx * dt;
+            y = y + vy * dt;
+            z = z + vz * dt;
+            st.moves = 0;
+            st.refl = 0;
+            start = false;
+        }
+        const StartTet& g = t.g;
+        const double l0 = bary(g.a0, g.b0, g.g0, g.d0, x, y, z, g.vol);
+        const double l1 = bary(g.a1, g.b1, g.g1, g.d1, x, y, z, g.vol);
+        const double l2 = bary(g.a2, g.b2, g.g2, g.d2, x, y, z, g.vol);
+        const double l3 = bary(g.a3, g.b3, g.g3, g.d3, x, y, z, g.vol);
+        const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
+                         (l3 < 0 || l3 > 1.0);
+        if (!out) {
+            const uint32_t base = key0 + (uint32_t)k * per_step;
+            if (HASH) {
+                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
+                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
+                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
+                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
+            } else {
+                unsafeAtomicAdd(den + base + t.m0, l0);
+                unsafeAtomicAdd(den + base + t.m1, l1);
+                unsafeAtomicAdd(den + base + t.m2, l2);
+                unsafeAtomicAdd(den + base + t.m3, l3);
+            }
+            n_cross += (st.moves - st.refl) > 0;
+            n_moves += st.moves;
+            n_refl += st.refl;
+            k++;
+            start = true;
+            active = k < K;
+            continue;
+        }
+        int mi = 0;
+        double ml = l0;
+        if (l1 < ml) { ml = l1; mi = 1; }
+        if (l2 < ml) { ml = l2; mi = 2; }
+        if (l3 < ml) { ml = l3; mi = 3; }
+        const int32_t nb = mi == 0 ? g.n0 : mi == 1 ? g.n1 : mi == 2 ? g.n2 : g.n3;
+        const bool capped = st.moves >= cap;
+        if (!capped) st.moves++;
+        if (!capped && nb >= 0) {
+            c = nb;
+            load_tet_regs(tets + nb, t);
+            continue;
+        }
+        if (capped || WALL == 0) {
+            n_cross += (st.moves - st.refl) > 0;
+            n_moves += st.moves;
+            n_refl += st.refl;
+            n_lost++;
+            n_cap += capped;
+            live = false;
+            active = false;
+            continue;
+        }
+        // reflection about the face plane, as in walk()
+        const double b = mi == 0 ? g.b0 : mi == 1 ? g.b1 : mi == 2 ? g.b2 : g.b3;
+        const double gm = mi == 0 ? g.g0 : mi == 1 ? g.g1 : mi == 2 ? g.g2 : g.g3;
+        const double d = mi == 0 ? g.d0 : mi == 1 ? g.d1 : mi == 2 ? g.d2 : g.d3;
+        const double gx = -b, gy = gm, gz = -d;
+        const double gg = (gx * gx + gy * gy) + gz * gz;
+        const double sc = ((2.0 * ml) * (6.0 * g.vol)) / gg;
+        x = x - sc * gx;
+        y = y - sc * gy;
+        z = z - sc * gz;
+        const double w = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
+        vx = vx - w * gx;
+        vy = vy - w * gy;
+        vz = vz - w * gz;
+        st.refl++;
+    }
 }
 
 }  // namespace mpd

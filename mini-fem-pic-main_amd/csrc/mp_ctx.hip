@@ -635,6 +635,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
         }
         int32_t chunk = n_steps - done;
         if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - c->since_sort);
+        chunk = std::min<int32_t>(chunk, mpd::kStepMaxLaunch);
         int rc = reserve_tally(c, c->n_slots, 256);
         if (rc) return rc;
         mpd::PushArgs a;

@@ -41,6 +41,9 @@ struct Soa {
 
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };
 
+// Most steps one multi-step launch takes (its LDS hash holds ~40 keys per step and workgroup).
+constexpr int kStepMaxLaunch = 16;
+
 __device__ __forceinline__ double bary(double a, double b, double g, double d, double x, double y,
                                        double z, double vol) {
     return ((1.0 / 6.0) * (((a - x * b) + y * g) - z * d)) / vol;

@@ -152,7 +152,8 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 #ifndef MP_STEPS_WAVES
 #define MP_STEPS_WAVES 4
 #endif
-// AGG 2 here: the (step, node)-keyed LDS hash of step_insert / step_flush.
+// AGG != 0: the (step, node)-keyed LDS hash of step_insert / step_flush; AGG 0: direct
+// atomics.  The lanes run their steps independently (lane_steps); K <= kStepMaxLaunch.
 template <int WALL, int AGG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
 k_push_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, double* __restrict__ den,
@@ -175,58 +176,19 @@ k_push_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, doubl
     int32_t c = live ? c_in : 0;
     TetRegs t;
     load_tet_regs(tets + c, t);
-    __shared__ uint32_t hk[AGG == 2 ? kStepSlots : 1];
-    __shared__ double hv[AGG == 2 ? kStepSlots : 1];
-    if (AGG == 2) {
+    constexpr bool kHash = AGG != 0;
+    __shared__ uint32_t hk[kHash ? kStepSlots : 1];
+    __shared__ double hv[kHash ? kStepSlots : 1];
+    if (kHash) {
         for (int q = threadIdx.x; q < kStepSlots; q += blockDim.x) {
             hk[q] = 0xFFFFFFFFu;
             hv[q] = 0.0;
         }
         __syncthreads();
     }
-    for (int k = 0; k < K; k++) {  // uniform: every thread takes every step (workgroup scatters)
-        bool ok = false;
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-        if (live) {
-            n_act++;
-            vx = vx + (qm * t.ex) * dt;
-            vy = vy + (qm * t.ey) * dt;
-            vz = vz + (qm * t.ez) * dt;
-            x = x + vx * dt;
-            y = y + vy * dt;
-            z = z + vz * dt;
-            WalkStats st{0, 0};
-            const int r = walk_regs<WALL>(tets, t, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
-            n_cross += (st.moves - st.refl) > 0;
-            n_moves += st.moves;
-            n_refl += st.refl;
-            if (r == kInside) {
-                ok = true;
-            } else {
-                live = false;
-                n_lost++;
-                n_cap += (r == kCapped);
-            }
-        }
-        double* dk = den + (int64_t)k * nn;
-        if (AGG == 4) {
-            scatter_lds_wave_nodes(dk, ok, t.m0, t.m1, t.m2, t.m3, l0, l1, l2, l3);
-        } else if (AGG == 2) {
-            if (ok) {
-                const uint32_t base = (uint32_t)k * (uint32_t)nn;
-                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
-                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
-                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
-                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
-            }
-            if ((k + 1) % kStepFlush == 0 || k + 1 == K) step_flush(hk, hv, den);
-        } else if (ok) {
-            unsafeAtomicAdd(dk + t.m0, l0);
-            unsafeAtomicAdd(dk + t.m1, l1);
-            unsafeAtomicAdd(dk + t.m2, l2);
-            unsafeAtomicAdd(dk + t.m3, l3);
-        }
-    }
+    lane_steps<WALL, kHash>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den, 0u, (uint32_t)nn, hk, hv,
+                            n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+    if (kHash) step_flush(hk, hv, den);
     if (c_in >= 0) {
         s.cell[i] = live ? c : -1;
         st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
@@ -1371,14 +1333,12 @@ hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStre
 #define MP_STEPS(W, A)                                                                                    \
     hipLaunchKernelGGL((k_push_steps<W, A>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, a.den, \
                        nn, k_steps, a.part, a.cap)
-    const int agg = a.agg == 4 ? 4 : a.agg == 0 ? 0 : 2;
+    if (k_steps > kStepMaxLaunch) return hipErrorInvalidValue;
     if (a.wall == 0) {
-        if (agg == 4) MP_STEPS(0, 4);
-        else if (agg == 0) MP_STEPS(0, 0);
+        if (a.agg == 0) MP_STEPS(0, 0);
         else MP_STEPS(0, 2);
     } else {
-        if (agg == 4) MP_STEPS(1, 4);
-        else if (agg == 0) MP_STEPS(1, 0);
+        if (a.agg == 0) MP_STEPS(1, 0);
         else MP_STEPS(1, 2);
     }
 #undef MP_STEPS

@@ -45,8 +45,8 @@ struct PushArgs {
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
-// k_steps steps in one launch (atomic scatter; aggregation 0, 2 or 4, others run as 2); step k
-// scatters into a.den + k * nn.
+// k_steps <= kStepMaxLaunch steps in one launch (atomic scatter; aggregation 0 = direct
+// atomics, any other = the (step, node) LDS hash); step k scatters into a.den + k * nn.
 hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStream_t st);
 hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, int k_steps, double spwt,
                                   hipStream_t st);
