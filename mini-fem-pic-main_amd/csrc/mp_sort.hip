@@ -16,11 +16,29 @@ namespace mpd {
 
 namespace {
 
+// Tiles of the histogram and sorting kernels (threads, items per thread).  Measured on 80M
+// records with 18-bit keys (deterministic steps of 20M particles): 1024 x 32 histogram and
+// 512 x 20 sort tiles (up to 96 KB of LDS per workgroup, within gfx950's 160 KB) make the
+// deterministic step 15% faster than 512 x 8 for both.
+#ifndef MP_SORT_HIST_THREADS
+#define MP_SORT_HIST_THREADS 1024
+#endif
+#ifndef MP_SORT_HIST_ITEMS
+#define MP_SORT_HIST_ITEMS 32
+#endif
+#ifndef MP_SORT_THREADS
+#define MP_SORT_THREADS 512
+#endif
+#ifndef MP_SORT_ITEMS
+#define MP_SORT_ITEMS 20
+#endif
+
 template <unsigned Bits>
 using SortCfg = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<512, 8>, rocprim::kernel_config<512, 8>,
-                                        Bits, rocprim::block_radix_rank_algorithm::match>>;
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<MP_SORT_HIST_THREADS, MP_SORT_HIST_ITEMS>,
+                                        rocprim::kernel_config<MP_SORT_THREADS, MP_SORT_ITEMS>, Bits,
+                                        rocprim::block_radix_rank_algorithm::match>>;
 
 // Digit width for an end_bit-wide key: the fewest passes of at most 11 bits, then the
 // narrowest width (>= 8) that keeps that pass count.
