@@ -192,7 +192,9 @@ typedef struct mp_tune_trial {
  * sort_shift and sort_interval for the resident particles and the given step parameters:
  * coordinate descent over the knobs, each candidate timed (HIP events) over a window of
  * max(interval, min_steps) steps that starts from the same saved state and holds one cell
- * sort per sort interval, as a long run does.  Results do not depend on the options, so
+ * sort per sort interval, as a long run does; a candidate replaces the incumbent only when
+ * more than 1% faster, and the winner is timed twice more against the starting options,
+ * keeping the faster.  Results do not depend on the options, so
  * only time is compared.  The particle state, counters and density are restored before
  * returning; the chosen options are set on ctx and written to *best (may be NULL) with their
  * time per step.  trials[0..max_trials) receives the timed candidates in order (may be NULL

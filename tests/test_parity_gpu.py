@@ -447,7 +447,10 @@ def test_autotune_leaves_state_and_keeps_parity(mp, orc):
         assert mv.options == best
         assert best["threads_per_particle"] in (1, 4) and best["particles_per_thread"] in (1, 2, 4)
         assert best["block_size"] in (64, 128, 256) and 0 <= best["scatter_agg"] <= 4
-        assert len(trials) > 20 and ms == min(t for _, t in trials) and ms > 0
+        # the reported time is one of the chosen options' timings, within 1% of the fastest trial
+        # or from the final re-check against the starting options
+        assert len(trials) > 20 and ms > 0
+        assert any(o == best and t == ms for o, t in trials)
         for k in range(3):
             mode = k % 2
             mv.step(QM, dt, SPWT, mode)
