@@ -7,10 +7,13 @@ Workload (one GPU): a 60x60x60 six-tet box of side 1 m (1,296,000 tets, 226,981 
 walls, cell sort every 10 steps.  Particles stay resident in HBM between steps.
 
 Prints ONE JSON line.  value = particle-steps/s over the timed steps (whole step: push,
-walk, scatter, normalisation, periodic sort).  roofline = the push kernel's algorithmic
-HBM bytes per launch (104 B per active particle + 192 B per tet + 16 B per node) over its
-average launch time from HIP events on the context stream.  cpu_baseline = the oracle's
-OpenMP port of the reference kernel body on the host cores, on a bounded sample.
+walk, scatter, normalisation, periodic sort) of the frozen-field run through mp_step_many,
+which pushes the steps between two periodic sorts in one launch with the particles held in
+registers; per_step = the same workload one mp_step (one MoveParticles) per step.  roofline
+= the push kernel's algorithmic HBM bytes (104 B per active particle-step + 192 B per tet
+and 16 B per node per step) per launch over its average launch time from HIP events on the
+context stream; traffic = the HBM bytes rocprofv3 measured per launch.  cpu_baseline = the
+oracle's OpenMP port of the reference kernel body on the host cores, on a bounded sample.
 """
 import argparse
 import json
@@ -52,13 +55,14 @@ def cpu_model():
     return platform.processor()
 
 
-def measured_traffic():
-    """HBM bytes per push launch from the committed rocprofv3 --pmc summary (FETCH_SIZE and
-    WRITE_SIZE runs of this workload, calibrated on a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "k_push_soa_box60_20M_traffic.txt")
+def measured_traffic(name="k_push_soa_box60_20M_traffic.txt", key="traffic_bytes"):
+    """HBM bytes from the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of
+    this workload, calibrated on a copy kernel of known size): per push launch for the per-step
+    kernel, per step for the multi-step kernel; None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes"):
+            if line.split() and line.split()[0] == key:
                 return float(line.split()[1])
     except OSError:
         pass
@@ -69,14 +73,22 @@ def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 16.0 * n_nodes)
 
 
-def run_gpu_many(mesh, ef, pos, vel, cell, steps, warmup, options):
-    """The same workload through mp_step_many (frozen field): each stretch of steps between two
-    periodic sorts is one launch with the particles held in registers.  Same options as the
-    per-step line (the multi-step kernel takes 256-thread workgroups, one particle per thread)."""
+def run_gpu_many(mesh, ef, pos, vel, cell, steps, warmup, options, tune=True):
+    """The workload through mp_step_many (frozen field): each stretch of steps between two
+    periodic sorts is one launch with the particles held in registers (256-thread workgroups,
+    one particle per thread).  With tune, mp_autotune times its candidates through
+    mp_step_many first (untimed; the particle state is restored)."""
+    tuned = None
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(**options)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
+        if tune:
+            t = time.perf_counter()
+            best, best_ms, trials = mv.autotune(QM, DT, SPWT, mp.SCATTER_ATOMIC, min_steps=10, multi_step=True)
+            tuned = {"options": best, "ms_per_step": best_ms, "trials": len(trials),
+                     "tuner_s": time.perf_counter() - t}
+            log(f"autotune multi-step ({len(trials)} trials, {tuned['tuner_s']:.1f} s): {best} at {best_ms:.3f} ms/step")
         mv.step_many(QM, DT, SPWT, mp.SCATTER_ATOMIC, warmup)
         mv.sync()
         mv.reset_counters()
@@ -88,10 +100,12 @@ def run_gpu_many(mesh, ef, pos, vel, cell, steps, warmup, options):
         push_ms, launches = mv.timing()
         mv.set_timing(False)
         c = mv.counters()
+        opts = mv.options
     wall = t1 - t0
     return {"value": c["particle_steps"] / wall, "ms_per_step": 1e3 * wall / steps, "steps": steps,
-            "push_ms_per_step": push_ms / steps, "launches": launches,
-            "steps_per_launch": steps / max(launches, 1), "particle_steps": c["particle_steps"]}
+            "push_ms": push_ms, "push_ms_per_step": push_ms / steps, "launches": launches,
+            "steps_per_launch": steps / max(launches, 1), "particle_steps": c["particle_steps"],
+            "counters": c, "options": opts, "tuned": tuned}
 
 
 def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
@@ -415,6 +429,8 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=10)
     ap.add_argument("--many-steps", type=int, default=100,
                     help="timed steps of the multi-step (mp_step_many) line (0: skip)")
+    ap.add_argument("--headline", choices=["multi", "per-step"], default="multi",
+                    help="which atomic-mode line is value/roofline (the other is reported beside it)")
     ap.add_argument("--det-steps", type=int, default=20, help="timed steps of the deterministic-mode line (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--particles", type=int, default=N_PART)
@@ -455,13 +471,45 @@ def main():
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
     achieved = bytes_per_launch / avg_push_s / 1e9
+    per_step = {
+        "value": r["rate"], "ms_per_step": 1e3 * r["wall_s"] / args.steps, "steps": args.steps,
+        "path": "mp_step: one MoveParticles step per call (k_push_soa)", "options": r["options"],
+        "autotune": r["tuned"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_push_soa", "avg_launch_ms": avg_push_s * 1e3,
+                     "bytes_per_launch": bytes_per_launch}}
+    head, roof, path, head_steps = r, per_step["roofline"], per_step["path"], args.steps
     extra = {}
     if args.many_steps > 0 and mode == mp.SCATTER_ATOMIC:
-        m = run_gpu_many(mesh, ef, pos, vel, cell, args.many_steps, args.warmup, r["options"])
-        extra["multi_step"] = dict(m, note="mp_step_many: the steps between two periodic sorts in one launch, "
-                                           "particles kept in registers (frozen field); same options")
+        m = run_gpu_many(mesh, ef, pos, vel, cell, args.many_steps, args.warmup,
+                         dict(sort_interval=args.sort_interval, sort_shift=args.sort_shift, scatter_agg=args.agg),
+                         tune=not args.no_tune)
         log(f"gpu atomic multi-step: {m['value'] / 1e9:.3f} G particle-steps/s, push {m['push_ms_per_step']:.3f} "
             f"ms/step, {m['steps_per_launch']:.1f} steps per launch")
+        m_launch_s = m["push_ms"] / 1e3 / max(m["launches"], 1)
+        m_bytes = push_bytes(m["particle_steps"], m["steps"], mesh.n_tets, mesh.n_nodes) / max(m["launches"], 1)
+        m_ach = m_bytes / m_launch_s / 1e9
+        tps = measured_traffic("k_push_steps_box60_20M_traffic.txt", "traffic_bytes_per_step")
+        multi = {
+            "value": m["value"], "ms_per_step": m["ms_per_step"], "steps": m["steps"],
+            "path": "mp_step_many: the steps between two periodic sorts in one launch (k_push_steps), particles and "
+                    "their tet records held in registers, lanes stepping independently (frozen field)",
+            "options": m["options"], "autotune": m["tuned"], "steps_per_launch": m["steps_per_launch"],
+            "roofline": {"bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": m_ach / HBM_PEAK_GBS,
+                         "traffic": None if tps is None else tps * m["steps_per_launch"],
+                         "kernel": "k_push_steps", "avg_launch_ms": m_launch_s * 1e3,
+                         "bytes_per_launch": m_bytes,
+                         "note": "achieved counts the algorithmic bytes of every step the launch takes (the "
+                                 "state is read and written once per launch, so traffic is well below them)"}}
+        if args.headline == "multi":
+            head = {"rate": m["value"], "wall_s": m["ms_per_step"] * m["steps"] / 1e3, "counters": m["counters"],
+                    "options": m["options"], "tuned": m["tuned"]}
+            roof, path, head_steps = multi["roofline"], multi["path"], m["steps"]
+            extra["per_step"] = per_step
+        else:
+            extra["multi_step"] = multi
     if args.det_steps > 0 and mode == mp.SCATTER_ATOMIC:
         d = run_gpu(mesh, ef, pos, vel, cell, args.det_steps, 2, mp.SCATTER_DETERMINISTIC, args.sort_interval,
                     shift=args.sort_shift, tune=not args.no_tune, pid_order=args.det_pid_order)
@@ -488,15 +536,15 @@ def main():
                "single_thread_sample": f"first {cb['sample']} particles, 2 steps; the full MoveParticles line adds "
                                        "the 2 x 88 B record copies, the stable erase and the normalisation"}
         log(f"cpu baseline: {cb['value'] / 1e6:.2f} M particle-steps/s on {cb['threads']} threads")
-    c = r["counters"]
+    c = head["counters"]
     out = {
         "metric": "FP64 particle-steps/s (push+tet-walk+scatter)",
-        "value": r["rate"],
+        "value": head["rate"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head_steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * r["wall_s"] / args.steps,
+        "ms_per_step": 1e3 * head["wall_s"] / head_steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -505,13 +553,10 @@ def main():
         "config": {"workload": "push-only scale test: 60^3 six-tet box, side 1 m (1,296,000 tets, 226,981 nodes), "
                                "20M FP64 particles, frozen sinusoidal field, absorbing walls",
                    "particles": n, "tets": mesh.n_tets, "nodes": mesh.n_nodes, "dt": DT,
-                   "scatter": args.mode, "options": r["options"], "autotune": r["tuned"],
+                   "scatter": args.mode, "path": path, "options": head["options"], "autotune": head["tuned"],
                    "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
                    "lost": c["lost"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_push_soa", "avg_launch_ms": avg_push_s * 1e3,
-                     "bytes_per_launch": bytes_per_launch},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     out.update(extra)

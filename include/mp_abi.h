@@ -203,6 +203,9 @@ typedef struct mp_tune_trial {
 int  mp_autotune(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode, int32_t min_steps,
                  mp_options* best, double* best_ms_per_step, mp_tune_trial* trials, int32_t max_trials,
                  int32_t* n_trials);
+/* OR into mp_autotune's mode: time the candidates through mp_step_many (frozen-field runs)
+ * and sweep only the knobs its kernel takes (sort shift, sort interval, aggregation 0 or 2). */
+#define MP_TUNE_MULTI_STEP 0x100
 
 /* ---- batched ensemble: many independent runs on one mesh (no reference counterpart) -- */
 /* An ensemble of n_runs independent particle sets pushed through the context's mesh and

@@ -41,8 +41,9 @@ struct Soa {
 
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };
 
-// Most steps one multi-step launch takes (its LDS hash holds ~40 keys per step and workgroup).
-constexpr int kStepMaxLaunch = 16;
+// Most steps one multi-step launch takes (its 2048-slot LDS hash holds ~40 keys per step and
+// workgroup after a cell sort; keys beyond its capacity go to HBM directly).
+constexpr int kStepMaxLaunch = 32;
 
 __device__ __forceinline__ double bary(double a, double b, double g, double d, double x, double y,
                                        double z, double vol) {

@@ -1033,17 +1033,28 @@ k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, 
 }
 
 // Moves every particle still in the domain into its bucket's range (cursor = bucket offsets,
-// advanced atomically, one returning atomic per bucket per workgroup).
+// advanced atomically, one returning atomic per bucket per workgroup).  The particle's whole
+// record is loaded with its cell, before the workgroup's hash and atomics, so the HBM reads
+// overlap the atomics' round trip instead of following it.
 __global__ void __launch_bounds__(kSortThreads)
 k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ cursor, Soa dst) {
     __shared__ SortSlots t;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t c = -1, pid = 0;
+    uint16_t run = 0;
+    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    if (i < n) {
+        c = src.cell[i];
+        x = ld(src.x + i); y = ld(src.y + i); z = ld(src.z + i);
+        vx = ld(src.vx + i); vy = ld(src.vy + i); vz = ld(src.vz + i);
+        pid = src.pid[i];
+        if (src.run) run = src.run[i];
+    }
     for (int q = threadIdx.x; q < kSortHash; q += blockDim.x) {
         t.key[q] = -1;
         t.cnt[q] = 0;
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t c = (i < n) ? src.cell[i] : -1;
     int slot = -1, rank = 0;
     if (c >= 0) {
         slot = sort_slot(t, bucket_key(c, src.run, i, shift, nb));
@@ -1055,11 +1066,13 @@ k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ 
     __syncthreads();
     if (c < 0) return;
     const int64_t d = (int64_t)t.base[slot] + rank;
-    dst.x[d] = src.x[i]; dst.y[d] = src.y[i]; dst.z[d] = src.z[i];
-    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    // plain stores: a bucket's particles come from several workgroups, and L2 merges their
+    // partial lines before they reach HBM
+    dst.x[d] = x; dst.y[d] = y; dst.z[d] = z;
+    dst.vx[d] = vx; dst.vy[d] = vy; dst.vz[d] = vz;
     dst.cell[d] = c;
-    dst.pid[d] = src.pid[i];
-    if (src.run) dst.run[d] = src.run[i];
+    dst.pid[d] = pid;
+    if (src.run) dst.run[d] = run;
 }
 
 // ---- exclusive scan (int32), 3 phases --------------------------------------------------

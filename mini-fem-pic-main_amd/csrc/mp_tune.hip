@@ -85,6 +85,7 @@ struct Trial {
     const Saved* saved;
     double qm, dt, spwt;
     int32_t mode;
+    bool multi;
     int32_t min_steps;
     hipEvent_t ev[2];
     std::vector<mp_tune_trial> log;
@@ -98,11 +99,19 @@ struct Trial {
         if (rc) return rc;
         const int interval = o.sort_interval > 0 ? o.sort_interval : min_steps;
         const int timed = std::max(interval, min_steps);
-        for (int k = 0; k < interval; k++)
-            if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        if (multi) {
+            if ((rc = mp_step_many(c, qm, dt, spwt, mode, interval))) return rc;
+        } else {
+            for (int k = 0; k < interval; k++)
+                if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        }
         HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-        for (int k = 0; k < timed; k++)
-            if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        if (multi) {
+            if ((rc = mp_step_many(c, qm, dt, spwt, mode, timed))) return rc;
+        } else {
+            for (int k = 0; k < timed; k++)
+                if ((rc = mp_step(c, qm, dt, spwt, mode))) return rc;
+        }
         HIP_TRY(c, hipEventRecord(ev[1], c->stream));
         HIP_TRY(c, hipEventSynchronize(ev[1]));
         float ms = 0.f;
@@ -124,6 +133,8 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
                            mp_options* best, double* best_ms_per_step, mp_tune_trial* trials,
                            int32_t max_trials, int32_t* n_trials) {
     if (!c) return MP_ERR_ARG;
+    const bool multi = (mode & MP_TUNE_MULTI_STEP) != 0;
+    mode &= ~MP_TUNE_MULTI_STEP;
     if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
         return set_err(c, MP_ERR_ARG, "mp_autotune: unknown scatter mode");
     if (min_steps < 1 || min_steps > 1000) return set_err(c, MP_ERR_ARG, "mp_autotune: min_steps must be 1..1000");
@@ -138,7 +149,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
     c->timing = false;
     Saved saved;
     int rc = save(c, saved);
-    Trial tr{c, &saved, q_over_m, dt, spwt, mode, min_steps, {nullptr, nullptr}, {}};
+    Trial tr{c, &saved, q_over_m, dt, spwt, mode, multi, min_steps, {nullptr, nullptr}, {}};
     if (!rc && (hipEventCreate(&tr.ev[0]) != hipSuccess || hipEventCreate(&tr.ev[1]) != hipSuccess))
         rc = set_err(c, MP_ERR_HIP, "mp_autotune: hipEventCreate failed");
 
@@ -162,17 +173,26 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
             }
         }
     };
-    sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
-    sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
-    if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
-    sweep([](mp_options& o, int v) { o.defer_walk = v; }, {0, 1});
-    sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
-    sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
-    sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
-    sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
-    sweep([](mp_options& o, int v) { o.compact_crossers = v; }, {0, 1});
-    if (mode == MP_SCATTER_DETERMINISTIC) sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
-    sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
+    if (multi && mode == MP_SCATTER_ATOMIC) {
+        // the multi-step kernel: aggregation 0 or the (step, node) hash, one particle per thread
+        sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
+        sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
+        sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 2});
+        sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
+        sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
+    } else {
+        sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
+        sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
+        if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
+        sweep([](mp_options& o, int v) { o.defer_walk = v; }, {0, 1});
+        sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
+        sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
+        sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
+        sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
+        sweep([](mp_options& o, int v) { o.compact_crossers = v; }, {0, 1});
+        if (mode == MP_SCATTER_DETERMINISTIC) sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
+        sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
+    }
     // the winner against the starting options once more, both timed again (best of two)
     if (!rc && memcmp(&cur, &start, sizeof(cur)) != 0) {
         double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;

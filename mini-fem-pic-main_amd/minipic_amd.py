@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libminipic_amd.so")
 MP_OK = 0
 SCATTER_ATOMIC = 0
 SCATTER_DETERMINISTIC = 1
+TUNE_MULTI_STEP = 0x100  # OR into autotune's mode: time candidates through mp_step_many
 WALL_ABSORB = 0
 WALL_REFLECT = 1
 NODE_NORMAL, NODE_OPEN, NODE_INLET, NODE_FIXED = 0, 1, 2, 3
@@ -583,13 +584,15 @@ class Mover:
                                      C.byref(dd_out)))
         return float(loss.value), g, float(dd_out.value)
 
-    def autotune(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC, min_steps=10, max_trials=64):
+    def autotune(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC, min_steps=10, max_trials=64, multi_step=False):
         """mp_autotune: (chosen options, their ms per step, [(options, ms per step)] of every
-        timed candidate).  The chosen options are set; the particle state is left unchanged."""
+        timed candidate).  The chosen options are set; the particle state is left unchanged.
+        multi_step: time the candidates through mp_step_many."""
         best, ms = MpOptions(), C.c_double()
         trials = (MpTuneTrial * max_trials)()
         nt = C.c_int32()
-        self._check(lib().mp_autotune(self._h, q_over_m, dt, spwt, mode, min_steps, C.byref(best), C.byref(ms),
+        m = mode | (TUNE_MULTI_STEP if multi_step else 0)
+        self._check(lib().mp_autotune(self._h, q_over_m, dt, spwt, m, min_steps, C.byref(best), C.byref(ms),
                                       trials, max_trials, C.byref(nt)))
         as_dict = lambda o: {k: int(getattr(o, k)) for k, _ in MpOptions._fields_}  # noqa: E731
         log = [(as_dict(trials[k].opt), float(trials[k].ms_per_step)) for k in range(min(nt.value, max_trials))]

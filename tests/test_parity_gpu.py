@@ -684,3 +684,30 @@ def test_batch_step_many_matches_per_run_oracle(mp, orc, mode):
             assert c["lost"] == sum(o.stats["lost"] for o in oracles) > 0
             with pytest.raises(mp.MpError):
                 b.density_step(7, 0)
+
+
+def test_autotune_multi_step_keeps_state_and_parity(mp, orc):
+    """mp_autotune timing candidates through mp_step_many: the particle state is restored, the
+    chosen options are among those the multi-step kernel takes, and mp_step_many afterwards
+    still matches the oracle bit for bit (particles) and within 1e-12 (densities)."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=67, n=120000)
+    dt = 4e-7
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        mv.step_many(QM, dt, SPWT, 0, 2)
+        for _ in range(2):
+            run.step(QM, dt, SPWT)
+        before = mv.get_particles()
+        best, ms, trials = mv.autotune(QM, dt, SPWT, mp.SCATTER_ATOMIC, min_steps=4, multi_step=True)
+        after = mv.get_particles()
+        for a, b in zip(before, after):
+            assert a.tobytes() == b.tobytes()
+        assert mv.options == best and best["scatter_agg"] in (0, 2) and ms > 0
+        assert len({t["sort_interval"] for t, _ in trials}) > 3
+        mv.step_many(QM, dt, SPWT, 0, 5)
+        for k in range(5):
+            check_den(mv.density_step(k), run.step(QM, dt, SPWT), 0, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
