@@ -376,13 +376,14 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
                                            double qm, double dt, int K, int cap, double* __restrict__ den,
                                            uint32_t key0, uint32_t per_step, uint32_t* hk, double* hv, int& n_act,
                                            int& n_lost, int& n_cross, int& n_moves, int& n_cap, int& n_refl) {
-    int k = 0;
-    bool start = true;
+    // few live counters: steps started = k (+1 for a step that loses the particle), the moves
+    // summed, crossings and reflections packed (crossings <= K < 256 in the low byte)
+    int k = 0, moves = 0, cross_refl = 0;
+    bool start = true, lost = false, capped = false;
     WalkStats st{0, 0};
     bool active = live && K > 0;
     while (active) {
         if (start) {
-            n_act++;
             vx = vx + (qm * t.ex) * dt;
             vy = vy + (qm * t.ey) * dt;
             vz = vz + (qm * t.ez) * dt;
@@ -413,9 +414,8 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
                 unsafeAtomicAdd(den + base + t.m2, l2);
                 unsafeAtomicAdd(den + base + t.m3, l3);
             }
-            n_cross += (st.moves - st.refl) > 0;
-            n_moves += st.moves;
-            n_refl += st.refl;
+            moves += st.moves;
+            cross_refl += ((st.moves - st.refl) > 0) + (st.refl << 8);
             k++;
             start = true;
             active = k < K;
@@ -427,7 +427,7 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
         if (l2 < ml) { ml = l2; mi = 2; }
         if (l3 < ml) { ml = l3; mi = 3; }
         const int32_t nb = mi == 0 ? g.n0 : mi == 1 ? g.n1 : mi == 2 ? g.n2 : g.n3;
-        const bool capped = st.moves >= cap;
+        capped = st.moves >= cap;
         if (!capped) st.moves++;
         if (!capped && nb >= 0) {
             c = nb;
@@ -435,11 +435,9 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
             continue;
         }
         if (capped || WALL == 0) {
-            n_cross += (st.moves - st.refl) > 0;
-            n_moves += st.moves;
-            n_refl += st.refl;
-            n_lost++;
-            n_cap += capped;
+            moves += st.moves;
+            cross_refl += ((st.moves - st.refl) > 0) + (st.refl << 8);
+            lost = true;
             live = false;
             active = false;
             continue;
@@ -460,6 +458,12 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
         vz = vz - w * gz;
         st.refl++;
     }
+    n_act += k + lost;
+    n_lost += lost;
+    n_cap += lost && capped;
+    n_moves += moves;
+    n_cross += cross_refl & 0xFF;
+    n_refl += cross_refl >> 8;
 }
 
 }  // namespace mpd
