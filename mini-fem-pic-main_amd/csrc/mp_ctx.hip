@@ -224,6 +224,7 @@ int det_finish(mp_ctx* c, int64_t m, double spwt) {
 // (pid order is first restored by a scatter on pid ranks if a cell sort broke it).
 int compact_now(mp_ctx* c) {
     c->since_sort = 0;
+    c->need_sort = false;
     if (c->n_slots == 0) {
         c->pid_ordered = true;
         return MP_OK;
@@ -261,9 +262,17 @@ int compact_now(mp_ctx* c) {
     return MP_OK;
 }
 
+// The periodic sort is due (every sort_interval steps, or before the step after a large
+// injection; never with sort_interval 0).
+bool sort_due(const mp_ctx* c) {
+    const int si = c->opt.sort_interval;
+    return si > 0 && (c->need_sort || c->since_sort >= si);
+}
+
 int sort_now(mp_ctx* c) {
     if (c->opt.keep_pid_order) return compact_now(c);
     c->since_sort = 0;
+    c->need_sort = false;
     c->pid_ordered = false;
     if (c->n_slots == 0) return MP_OK;
     const int shift = c->opt.sort_shift;
@@ -531,7 +540,7 @@ int mp_add_particles(mp_ctx* c, const double* pos, const double* vel, const int3
     c->next_pid += n;
     // a large batch (over an eighth of the set) is sorted in before the next step; small
     // injections join the next periodic sort
-    if (8 * n > c->n_slots) c->since_sort = c->opt.sort_interval;
+    if (8 * n > c->n_slots) c->need_sort = true;
     return MP_OK;
 }
 
@@ -541,7 +550,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         return set_err(c, MP_ERR_ARG, "mp_step: unknown scatter mode");
     if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_step: no field uploaded");
     HIP_TRY(c, hipSetDevice(c->device));
-    if (c->opt.sort_interval > 0 && c->since_sort >= c->opt.sort_interval) {
+    if (sort_due(c)) {
         int rc = sort_now(c);
         if (rc) return rc;
     }
@@ -629,7 +638,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
     int32_t done = 0;
     while (done < n_steps) {
         const int32_t si = c->opt.sort_interval;
-        if (si > 0 && c->since_sort >= si) {
+        if (sort_due(c)) {
             int rc = sort_now(c);
             if (rc) return rc;
         }

@@ -28,6 +28,7 @@ struct mp_ctx {
     int64_t n_slots = 0;
     int64_t next_pid = 0;
     int64_t since_sort = 0;
+    bool need_sort = false;  // a large injection is sorted in before the next step
     bool pid_ordered = true;    // slots hold the particles in ascending pid (no cell sort since the set)
     int32_t* cmp_buf = nullptr;  // flags and offsets of the stable compaction / pid reordering
     int64_t cmp_n = 0;

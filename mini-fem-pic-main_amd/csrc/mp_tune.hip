@@ -30,6 +30,7 @@ struct Saved {
     void* mem = nullptr;
     Soa soa{};
     int64_t n_slots = 0, next_pid = 0, since_sort = 0;
+    bool need_sort = false;
     bool pid_ordered = true;
     unsigned long long ctr[mpd::kNumCounters] = {};
     std::vector<double> den;
@@ -52,6 +53,7 @@ int save(mp_ctx* c, Saved& s) {
     s.n_slots = c->n_slots;
     s.next_pid = c->next_pid;
     s.since_sort = c->since_sort;
+    s.need_sort = c->need_sort;
     s.pid_ordered = c->pid_ordered;
     if (alloc_soa(&s.mem, &s.soa, std::max<int64_t>(c->n_slots, 1)) != hipSuccess) {
         (void)hipGetLastError();
@@ -72,6 +74,7 @@ int restore(mp_ctx* c, const Saved& s) {
     c->n_slots = s.n_slots;
     c->next_pid = s.next_pid;
     c->since_sort = s.since_sort;
+    c->need_sort = s.need_sort;
     c->pid_ordered = s.pid_ordered;
     HIP_TRY(c, hipMemcpyAsync(c->ctr, s.ctr, sizeof(s.ctr), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->den_out, s.den.data(), sizeof(double) * s.den.size(), hipMemcpyHostToDevice,
