@@ -149,8 +149,10 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 // which the particle stays in its tet loads nothing.  Step k scatters into den + k * nn.  The
 // per-particle arithmetic is k_push_soa's, in the same order, so K launches of k_push_soa and
 // one of these leave the same bits (atomic density sums aside).  Atomic scatters only.
+// 5 waves per SIMD: the absorbing-wall kernels fit 96 VGPRs, the reflecting ones spill a few
+// bytes per lane
 #ifndef MP_STEPS_WAVES
-#define MP_STEPS_WAVES 4
+#define MP_STEPS_WAVES 5
 #endif
 // AGG != 0: the (step, node)-keyed LDS hash of step_insert / step_flush; AGG 0: direct
 // atomics.  The lanes run their steps independently (lane_steps); K <= kStepMaxLaunch.
