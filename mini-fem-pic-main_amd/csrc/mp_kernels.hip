@@ -1486,9 +1486,11 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
         hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start, end);
     hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, vals, start, end, nvol, nn, spwt,
                        run_spwt, nn_mesh, out, long_list, long_count);
-    // long segments exist only where a few hundred nodes hold most records: 4 waves per CU
-    // of 256 CUs is enough to fold them side by side
-    hipLaunchKernelGGL(k_fold_long, dim3(256), dim3(256), 0, st, vals, start, end, nvol, spwt, run_spwt,
+    // long segments exist only where a few thousand nodes hold most records: 4096 waves (16
+    // per CU) give each of them a wave of its own, so hot nodes do not queue behind each other
+    // (with 1024 waves a wave took three hot nodes in turn: 14 ms per step at 200M clustered
+    // particles)
+    hipLaunchKernelGGL(k_fold_long, dim3(1024), dim3(256), 0, st, vals, start, end, nvol, spwt, run_spwt,
                        nn_mesh, out, long_list, long_count);
     return hipGetLastError();
 }
