@@ -948,40 +948,61 @@ k_fold(const double* __restrict__ vals, const int32_t* __restrict__ start, const
     fold_out(s, n, spwt, run_spwt, nn_mesh, nvol, out);
 }
 
-// One wave per listed node; every wave strides over the list and leaves when it is done.
+// One wave per listed node; every wave strides over the list and leaves when it is done.  Per
+// 256 records: coalesced loads (the next chunk's issued before this one is folded), the wave
+// writes them to its LDS slice, and its first lane adds them in order from LDS, the reads
+// running ahead of the dependent adds (one v_add_f64 per record on the chain).
+constexpr int kLongChunk = 4 * kWave;
+
 __global__ void __launch_bounds__(256)
 k_fold_long(const double* __restrict__ vals, const int32_t* __restrict__ start,
             const int32_t* __restrict__ end, const double* __restrict__ nvol, double spwt,
             const double* __restrict__ run_spwt, int32_t nn_mesh, double* __restrict__ out,
             const int32_t* __restrict__ long_list, const int32_t* __restrict__ long_count) {
+    __shared__ double buf[256 / kWave][kLongChunk];
     const int lane = lane_id();
+    double* lb = buf[threadIdx.x / kWave];
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
     const int32_t count = *long_count;
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < count; i += waves) {
+    // the wave's index as a scalar: every loop below has a wave-uniform bound, so the compiler
+    // keeps the lanes together around the lane-0 fold
+    const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / kWave) +
+                       __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    for (int64_t i = w0; i < count; i += waves) {
         const int64_t n = long_list[i];
         const int64_t b0 = start[n], b1 = end[n];
         double acc = 0.0;
-        for (int64_t base = b0; base < b1; base += 4 * kWave) {
-            double v[4];
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int64_t j = b0 + q * kWave + lane;
+            v[q] = j < b1 ? vals[j] : 0.0;
+        }
+        for (int64_t base = b0; base < b1; base += kLongChunk) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) lb[q * kWave + lane] = v[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // the next chunk's loads fly while lane 0 folds this one
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int64_t j = base + q * kWave + lane;
+                const int64_t j = base + kLongChunk + q * kWave + lane;
                 v[q] = j < b1 ? vals[j] : 0.0;
             }
-            const int64_t left = b1 - base;
-            if (left >= 4 * kWave) {
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-#pragma unroll
-                    for (int r = 0; r < kWave; r++) acc = acc + readlane_f64(v[q], r);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int64_t rem = left - q * kWave;
-                    const int cnt = rem >= kWave ? kWave : (rem > 0 ? (int)rem : 0);
-                    for (int r = 0; r < cnt; r++) acc = acc + readlane_f64(v[q], r);
+            if (lane == 0) {
+                const int64_t left = b1 - base;
+                if (left >= kLongChunk) {
+#pragma unroll 32
+                    for (int r = 0; r < kLongChunk; r++) acc = acc + lb[r];
+                } else {
+                    for (int r = 0; r < (int)left; r++) acc = acc + lb[r];
                 }
             }
+            // lane 0's reads complete before the next chunk overwrites the slice
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         if (lane == 0) fold_out(acc, n, spwt, run_spwt, nn_mesh, nvol, out);
     }
@@ -1486,11 +1507,13 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
         hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start, end);
     hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, vals, start, end, nvol, nn, spwt,
                        run_spwt, nn_mesh, out, long_list, long_count);
-    // long segments exist only where a few thousand nodes hold most records: 4096 waves (16
-    // per CU) give each of them a wave of its own, so hot nodes do not queue behind each other
-    // (with 1024 waves a wave took three hot nodes in turn: 14 ms per step at 200M clustered
-    // particles)
-    hipLaunchKernelGGL(k_fold_long, dim3(1024), dim3(256), 0, st, vals, start, end, nvol, spwt, run_spwt,
+    // long segments exist only where a few thousand nodes hold most records: one wave each, so
+    // hot nodes do not queue behind each other (with 1024 waves a wave took three hot nodes in
+    // turn: 14 ms per step at 200M clustered particles)
+    // (up to 4096 workgroups = 16384 waves, one per listed node while the list is that long; the
+    // dispatcher hands the queued waves out as others finish)
+    const unsigned gl = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1024, ((int64_t)nn + 3) / 4));
+    hipLaunchKernelGGL(k_fold_long, dim3(gl), dim3(256), 0, st, vals, start, end, nvol, spwt, run_spwt,
                        nn_mesh, out, long_list, long_count);
     return hipGetLastError();
 }
