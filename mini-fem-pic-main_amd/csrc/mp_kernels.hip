@@ -878,14 +878,32 @@ __global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
 
 // Record range [start[node], end[node]) of each node in the sorted records (keys are node
 // ids; sentinels sort last).  start is pre-filled with -1: a node with no record keeps it.
+// Segment bounds of the sorted keys: start[node] = first record, end[node] = one past the
+// last.  Each thread takes 4 consecutive keys (one 16-byte load) and the two around them.
 __global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int32_t nn,
                              int32_t* __restrict__ start, int32_t* __restrict__ end) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m) return;
-    const uint32_t node = keys[j];
-    if (node >= (uint32_t)nn) return;
-    if (j == 0 || keys[j - 1] != node) start[node] = (int32_t)j;
-    if (j == m - 1 || keys[j + 1] != node) end[node] = (int32_t)(j + 1);
+    const int64_t j0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (j0 >= m) return;
+    uint32_t k[4];
+    if (j0 + 4 <= m) {
+        const uint4 q = *reinterpret_cast<const uint4*>(keys + j0);  // keys is 16-byte aligned
+        k[0] = q.x; k[1] = q.y; k[2] = q.z; k[3] = q.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) k[i] = j0 + i < m ? keys[j0 + i] : 0xFFFFFFFFu;
+    }
+    const uint32_t prev = j0 > 0 ? keys[j0 - 1] : 0u;
+    const uint32_t next = j0 + 4 < m ? keys[j0 + 4] : 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int64_t j = j0 + i;
+        const uint32_t node = k[i];
+        if (j >= m || node >= (uint32_t)nn) continue;
+        const uint32_t left = i == 0 ? prev : k[i - 1];
+        const uint32_t right = i == 3 ? next : k[i + 1];
+        if (j == 0 || left != node) start[node] = (int32_t)j;
+        if (j == m - 1 || right != node) end[node] = (int32_t)(j + 1);
+    }
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int q) {
@@ -1504,7 +1522,8 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
     if (e == hipSuccess) e = launch_fill_i32(long_count, 1, 0, st);
     if (e != hipSuccess) return e;
     if (m > 0)
-        hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, st, keys, m, nn, start, end);
+        hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for((m + 3) / 4, 256)), dim3(256), 0, st, keys, m, nn, start,
+                           end);
     hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, vals, start, end, nvol, nn, spwt,
                        run_spwt, nn_mesh, out, long_list, long_count);
     // long segments exist only where a few thousand nodes hold most records: one wave each, so
