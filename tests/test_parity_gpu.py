@@ -502,6 +502,11 @@ def test_error_paths(mp):
         with pytest.raises(mp.MpError) as e:  # no field yet
             mv.step(QM, 1e-7, SPWT, 0)
         assert e.value.code == 4
+        with pytest.raises(mp.MpError) as e:
+            mv.step_many(QM, 1e-7, SPWT, 0, 3)
+        assert e.value.code == 4
+        with pytest.raises(mp.MpError):  # no multi-step call yet
+            mv.density_step(0)
         with pytest.raises(mp.MpError):
             mv.set_options(block_size=100)
         with pytest.raises(mp.MpError):
@@ -517,6 +522,14 @@ def test_error_paths(mp):
         mv.upload_field(ef)
         mv.step(QM, 1e-7, SPWT, 0)
         assert mv.num_active() > 0
+        with pytest.raises(mp.MpError):
+            mv.step_many(QM, 1e-7, SPWT, 0, 0)
+        with pytest.raises(mp.MpError):
+            mv.step_many(QM, 1e-7, SPWT, 5, 2)
+        with pytest.raises(mp.MpError):  # the autotuner rejects an unknown mode bit
+            mv.autotune(QM, 1e-7, SPWT, 0x200)
+        mv.step_many(QM, 1e-7, SPWT, 0, 2)
+        assert mv.density_step(1).shape == (mesh.n_nodes,)
     with mp.Mover.from_mesh(mesh) as mv:
         mv.upload_field(ef)
         with pytest.raises(mp.MpError) as e:  # nothing resident to tune on
@@ -711,3 +724,29 @@ def test_autotune_multi_step_keeps_state_and_parity(mp, orc):
             check_den(mv.density_step(k), run.step(QM, dt, SPWT), 0, mp)
         gp, gv, gc, gid = mv.get_particles()
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+
+
+def test_step_many_long_and_empty(mp, orc):
+    """mp_step_many over more steps than one launch takes (no periodic sort: 40 steps = launches
+    of 32 and 8), and on an empty particle set (densities of zeros)."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=71, n=20000)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=0)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        mv.set_timing(True)
+        mv.step_many(QM, 2e-7, SPWT, 0, 40)
+        _, launches = mv.timing()
+        mv.set_timing(False)
+        assert launches == 2
+        for k in range(40):
+            check_den(mv.density_step(k), run.step(QM, 2e-7, SPWT), 0, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
+        assert mv.counters()["particle_steps"] == run.stats["particle_steps"]
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.upload_field(ef)
+        mv.set_particles(pos[:0], vel[:0], cell[:0])
+        mv.step_many(QM, 2e-7, SPWT, 0, 3)
+        assert not mv.density_step(2).any() and mv.counters()["particle_steps"] == 0
