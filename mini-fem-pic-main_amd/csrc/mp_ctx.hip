@@ -386,6 +386,7 @@ void mp_destroy(mp_ctx* c) {
                     c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
+    mpstage::stager_free(c->stage);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -437,9 +438,6 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
     if (!c->have_field) return set_err(c, MP_ERR_STATE, "mp_step_aos: no field uploaded");
     const int64_t n = (int64_t)*n_inout;
     if (n > INT32_MAX / 4) return set_err(c, MP_ERR_ARG, "mp_step_aos: too many particles");
-    for (int64_t i = 0; i < n; i++)
-        if (parts[i].cell_index < 0 || parts[i].cell_index >= c->n_tets)
-            return set_err(c, MP_ERR_ARG, "mp_step_aos: particle " + std::to_string(i) + " has cell_index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
     if (n == 0) {  // nothing to move: the density is all zero
         for (int32_t i = 0; i < c->n_nodes; i++) den_out[i] = 0.0;
@@ -464,7 +462,22 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
     }
     int rc = reserve_scan(c, n + 1);
     if (rc) return rc;
-    HIP_TRY(c, hipMemcpyAsync(c->aos, parts, sizeof(mp_particle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = sizeof(mp_particle) * (size_t)n;
+    if (bytes >= mpstage::kStageMin) {
+        HIP_TRY(c, mpstage::stager_init(c->stage));
+        HIP_TRY(c, mpstage::upload(c->stage, c->aos, parts, bytes, c->stream));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(c->aos, parts, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    // every cell_index must name a tet before the walk indexes the table with it (checked on
+    // the device, nothing written yet on failure)
+    HIP_TRY(c, mpd::launch_fill_i32(c->flags, 1, INT32_MAX, c->stream));
+    HIP_TRY(c, mpd::launch_check_cells_aos(c->aos, n, c->n_tets, c->flags, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->flags, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (*c->pinned_i32 != INT32_MAX)
+        return set_err(c, MP_ERR_ARG, "mp_step_aos: particle " + std::to_string(*c->pinned_i32) +
+                                          " has cell_index out of range");
     rc = reserve_tally(c, n, 256);
     if (rc) return rc;
     HIP_TRY(c, mpd::launch_push_aos(c->tets, c->aos, n, q_over_m, dt, c->den_raw, det ? c->keys[0] : nullptr,
@@ -485,8 +498,13 @@ int mp_step_aos(mp_ctx* c, mp_particle* parts, size_t* n_inout, double q_over_m,
     HIP_TRY(c, hipMemcpyAsync(den_out, c->den_out, sizeof(double) * (size_t)c->n_nodes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const int64_t kept = *c->pinned_i32;
-    if (kept > 0)
-        HIP_TRY(c, hipMemcpy(parts, c->aos_out, sizeof(mp_particle) * (size_t)kept, hipMemcpyDeviceToHost));
+    const size_t kept_bytes = sizeof(mp_particle) * (size_t)kept;
+    if (kept_bytes >= mpstage::kStageMin) {
+        HIP_TRY(c, mpstage::stager_init(c->stage));
+        HIP_TRY(c, mpstage::download(c->stage, parts, c->aos_out, kept_bytes, c->stream));
+    } else if (kept > 0) {
+        HIP_TRY(c, hipMemcpy(parts, c->aos_out, kept_bytes, hipMemcpyDeviceToHost));
+    }
     *n_inout = (size_t)kept;
     return MP_OK;
 }

@@ -6,6 +6,7 @@
 
 #include "mp_abi.h"
 #include "mp_launch.h"
+#include "mp_staging.h"
 
 using mpd::DevTet;
 using mpd::Soa;
@@ -51,6 +52,7 @@ struct mp_ctx {
     int32_t* offs = nullptr;
     int64_t aos_cap = 0;
     int32_t* pinned_i32 = nullptr;
+    mpstage::Stager stage;  // pinned chunks + copy threads for large host arrays
     // staging of injected particles (mp_add_particles)
     void* inj_tmp = nullptr;
     size_t inj_bytes = 0;

@@ -1179,6 +1179,16 @@ k_scan_down(const int32_t* __restrict__ in, int64_t n, const int32_t* __restrict
 
 // ---- stable compaction of the AoS drop-in array ---------------------------------------
 
+// first_bad = the lowest index whose cell_index is outside [0, n_tets) (left at its initial
+// value when every particle is in range)
+__global__ void k_check_cells_aos(const mp_particle* __restrict__ parts, int64_t n, int32_t n_tets,
+                                  int32_t* __restrict__ first_bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = parts[i].cell_index;
+    if (c < 0 || c >= n_tets) atomicMin(first_bad, (int32_t)i);
+}
+
 __global__ void k_flag_aos(const mp_particle* __restrict__ parts, int64_t n, int32_t* __restrict__ flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flag[i] = parts[i].cell_index >= 0 ? 1 : 0;
@@ -1598,6 +1608,13 @@ hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool b
 hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mark_pid, dim3(grid_for(n, 256)), dim3(256), 0, st, cell, pid, n, present);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_cells_aos(const mp_particle* p, int64_t n, int32_t n_tets, int32_t* first_bad,
+                                  hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_check_cells_aos, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, n_tets, first_bad);
     return hipGetLastError();
 }
 

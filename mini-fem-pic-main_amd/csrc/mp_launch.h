@@ -81,6 +81,8 @@ hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n,
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
                                const Soa& dst, hipStream_t st);
 hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st);
+hipError_t launch_check_cells_aos(const mp_particle* p, int64_t n, int32_t n_tets, int32_t* first_bad,
+                                  hipStream_t st);
 hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* off, mp_particle* out,
                               hipStream_t st);
 hipError_t launch_set_field(DevTet* tets, const double* ef, int32_t nt, hipStream_t st);
