@@ -376,11 +376,12 @@ def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
                         "timings include the particle upload (best of 3)"}
 
 
-def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30, sample=2_000_000):
+def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30, sample=2_000_000, coupled_steps=0):
     """The oracle's restatement of the reference kernel body on the host: OpenMP over all
     threads (atomic density adds) on the full workload until the budget is spent; one thread
-    on a sample; and one thread running the whole MoveParticles over 88 B Particle records
-    (copy in, kernel body, stable erase of the lost, normalisation, copy out) on the sample."""
+    on a sample; one thread running the whole MoveParticles over 88 B Particle records (copy
+    in, kernel body, stable erase of the lost, normalisation, copy out) on the sample; and the
+    12^3 coupled loop with that serial MoveParticles and the same host FE solver."""
     import oracle
 
     tets = mesh.tets
@@ -417,6 +418,27 @@ def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30, sample=2_
         parts = dev.copy()                     # the device -> host copy
     out["full_moveparticles_value"] = n2 / (time.perf_counter() - t0)
     out["sample"] = sample
+    if coupled_steps > 0:
+        # the 12^3 coupled loop with the serial MoveParticles restatement on the host
+        import tempfile
+
+        import coupled as cp
+
+        with tempfile.TemporaryDirectory() as d:
+            cmesh = cp.box_case(12, 0.1, d)
+        prm = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036,
+                            wall_potential=0.0, dt=5e-7, max_iter=coupled_steps)
+        run = cp.CoupledRun(cmesh, prm, move=oracle.AosMover(cmesh.tets, cmesh.node_volume, QM, prm.dt, 2e2),
+                            fe_flags=mp.FE_RESET_F1)
+        t_move = 0.0
+        t0 = time.perf_counter()
+        for _ in range(coupled_steps):
+            run.inject()
+            t_move += run.move_step()[1]
+            run.solve()
+        out["coupled_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / coupled_steps
+        out["coupled_move_ms_per_step"] = 1e3 * t_move / coupled_steps
+        out["coupled_steps"] = coupled_steps
     return out
 
 
@@ -526,7 +548,7 @@ def main():
         extra["coupled"] = run_coupled(args.coupled_steps)
     cpu = None
     if args.cpu_seconds > 0:
-        cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds)
+        cb = run_cpu_baseline(mesh, ef, pos, vel, cell, args.cpu_seconds, coupled_steps=args.coupled_steps)
         cpu = {"value": cb["value"], "unit": "particle-steps/s", "cores": cb["threads"], "kind": "port",
                "sample": f"the same 20M particles and mesh, {cb['steps']} steps from the same initial state "
                          f"({cb['wall_s']:.1f} s), OpenMP port of the reference kernel body with atomic "
@@ -535,6 +557,12 @@ def main():
                "full_moveparticles_1thread_value": cb["full_moveparticles_value"],
                "single_thread_sample": f"first {cb['sample']} particles, 2 steps; the full MoveParticles line adds "
                                        "the 2 x 88 B record copies, the stable erase and the normalisation"}
+        if "coupled_ms_per_step" in cb:
+            cpu["coupled_ms_per_step"] = cb["coupled_ms_per_step"]
+            cpu["coupled_move_ms_per_step"] = cb["coupled_move_ms_per_step"]
+            cpu["coupled_sample"] = (f"the 12^3 coupled loop, {cb['coupled_steps']} steps, with the oracle's serial "
+                                     "MoveParticles (one thread) and the same host injection and FE solver")
+            log(f"cpu coupled loop: {cb['coupled_ms_per_step']:.2f} ms/step (move {cb['coupled_move_ms_per_step']:.2f})")
         log(f"cpu baseline: {cb['value'] / 1e6:.2f} M particle-steps/s on {cb['threads']} threads")
     c = head["counters"]
     out = {
