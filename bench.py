@@ -467,6 +467,9 @@ def main():
     ap.add_argument("--det-pid-order", type=int, default=0, help="deterministic line: keep pid order (1) or cell sort (0)")
     ap.add_argument("--compact", type=int, default=0, help="compact crossing particles before the walk (1) or not (0)")
     ap.add_argument("--defer", type=int, default=0, help="walk crossing particles in a second kernel (1) or not (0)")
+    ap.add_argument("--gather-sort", type=int, default=0,
+                    help="multi-step line: the sort writes a permutation that the next launch gathers through (1), "
+                         "or copies the records (0)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
@@ -505,7 +508,8 @@ def main():
     extra = {}
     if args.many_steps > 0 and mode == mp.SCATTER_ATOMIC:
         m = run_gpu_many(mesh, ef, pos, vel, cell, args.many_steps, args.warmup,
-                         dict(sort_interval=args.sort_interval, sort_shift=args.sort_shift, scatter_agg=args.agg),
+                         dict(sort_interval=args.sort_interval, sort_shift=args.sort_shift, scatter_agg=args.agg,
+                              gather_sort=args.gather_sort),
                          tune=not args.no_tune)
         log(f"gpu atomic multi-step: {m['value'] / 1e9:.3f} G particle-steps/s, push {m['push_ms_per_step']:.3f} "
             f"ms/step, {m['steps_per_launch']:.1f} steps per launch")

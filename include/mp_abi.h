@@ -92,6 +92,10 @@ typedef struct mp_options {
                                       (TPP=1, PPT=1, no staging or compaction, aggregation
                                       0, 1, 2, 4 or the deterministic scatter); 0 (default):
                                       every lane walks in the push kernel */
+    int32_t gather_sort;           /* mp_step_many with the cell sort: 1 the sort's rank pass
+                                      writes only a permutation and the next multi-step
+                                      launch reads each particle through it into the other
+                                      buffer; 0 (default): the sort copies the records */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
@@ -204,7 +208,8 @@ int  mp_autotune(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t m
                  mp_options* best, double* best_ms_per_step, mp_tune_trial* trials, int32_t max_trials,
                  int32_t* n_trials);
 /* OR into mp_autotune's mode: time the candidates through mp_step_many (frozen-field runs)
- * and sweep only the knobs its kernel takes (sort shift, sort interval, aggregation 0 or 2). */
+ * and sweep only the knobs its kernel takes (sort shift, sort interval, aggregation 0 or 2,
+ * gather_sort). */
 #define MP_TUNE_MULTI_STEP 0x100
 
 /* ---- batched ensemble: many independent runs on one mesh (no reference counterpart) -- */

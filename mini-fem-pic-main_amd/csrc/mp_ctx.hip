@@ -109,6 +109,7 @@ mp_options default_options() {
     o.keep_pid_order = 0;
     o.compact_crossers = 0;
     o.defer_walk = 0;
+    o.gather_sort = 0;
     return o;
 }
 
@@ -294,6 +295,45 @@ int sort_now(mp_ctx* c) {
     return MP_OK;
 }
 
+// The rank half of a cell sort (the records stay where they are): perm[d] = the current slot
+// of the particle that belongs at slot d, lost particles dropped; *n_live = their count.  The
+// caller's next launch moves the records (k_push_steps with perm) and then takes the sort's
+// bookkeeping (gather_done), so a failed launch leaves the particles as they were.
+int rank_now(mp_ctx* c, int64_t* n_live) {
+    const int shift = c->opt.sort_shift;
+    const int64_t nb = (((int64_t)c->n_tets - 1) >> shift) + 1;
+    const int64_t nb1 = nb + 1;
+    int rc = reserve_scan(c, nb1);
+    if (rc) return rc;
+    if (c->n_slots > c->perm_n) {
+        dfree(c->perm);
+        c->perm = nullptr;
+        c->perm_n = 0;
+        if (dalloc(&c->perm, c->cap) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "sort permutation allocation failed");
+        c->perm_n = c->cap;
+    }
+    const Soa& src = c->soa[c->cur];
+    HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, mpd::launch_cell_rank(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cur, c->perm,
+                                     c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *n_live = *c->pinned_i32;
+    return MP_OK;
+}
+
+void gather_done(mp_ctx* c, int64_t n_live) {
+    c->cur = 1 - c->cur;
+    c->n_slots = n_live;
+    c->since_sort = 0;
+    c->need_sort = false;
+    c->pid_ordered = false;
+}
+
 bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
@@ -302,7 +342,8 @@ bool good_options(const mp_options& o) {
            o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
            o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
-           (o.compact_crossers == 0 || o.compact_crossers == 1) && (o.defer_walk == 0 || o.defer_walk == 1);
+           (o.compact_crossers == 0 || o.compact_crossers == 1) && (o.defer_walk == 0 || o.defer_walk == 1) &&
+           (o.gather_sort == 0 || o.gather_sort == 1);
 }
 
 }  // namespace
@@ -383,7 +424,7 @@ void mp_destroy(mp_ctx* c) {
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
-                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps};
+                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps, c->perm};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     mpstage::stager_free(c->stage);
@@ -656,19 +697,28 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
     int32_t done = 0;
     while (done < n_steps) {
         const int32_t si = c->opt.sort_interval;
+        // gather sort: the rank pass now, the record moves inside this launch
+        bool gather = false;
+        int64_t n_live = 0;
         if (sort_due(c)) {
-            int rc = sort_now(c);
+            gather = c->opt.gather_sort && !c->opt.keep_pid_order && c->n_slots > 0 &&
+                     c->n_slots <= (int64_t)INT32_MAX;
+            int rc = gather ? rank_now(c, &n_live) : sort_now(c);
             if (rc) return rc;
         }
         int32_t chunk = n_steps - done;
-        if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - c->since_sort);
+        if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - (gather ? 0 : c->since_sort));
         chunk = std::min<int32_t>(chunk, mpd::kStepMaxLaunch);
         int rc = reserve_tally(c, c->n_slots, 256);
         if (rc) return rc;
         mpd::PushArgs a;
         a.tets = c->tets;
-        a.soa = c->soa[c->cur];
-        a.n = c->n_slots;
+        a.soa = c->soa[gather ? 1 - c->cur : c->cur];
+        a.n = gather ? n_live : c->n_slots;
+        if (gather) {
+            a.src = c->soa[c->cur];
+            a.perm = c->perm;
+        }
         a.qm = q_over_m;
         a.dt = dt;
         a.den = c->den_steps + done * nn;
@@ -689,6 +739,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
         if (c->timing) HIP_TRY(c, record_event(c));
         HIP_TRY(c, mpd::launch_push_steps(a, c->n_nodes, chunk, c->stream));
         if (c->timing) HIP_TRY(c, record_event(c));
+        if (gather) gather_done(c, n_live);
         c->since_sort += chunk;
         done += chunk;
     }

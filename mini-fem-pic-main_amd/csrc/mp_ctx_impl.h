@@ -33,6 +33,8 @@ struct mp_ctx {
     bool pid_ordered = true;    // slots hold the particles in ascending pid (no cell sort since the set)
     int32_t* cmp_buf = nullptr;  // flags and offsets of the stable compaction / pid reordering
     int64_t cmp_n = 0;
+    int32_t* perm = nullptr;  // gather sort: source slot of each sorted slot
+    int64_t perm_n = 0;
     // counting-sort and scan scratch
     int32_t* cell_cnt = nullptr;
     int32_t* cell_cur = nullptr;

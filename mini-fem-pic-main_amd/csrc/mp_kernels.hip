@@ -156,28 +156,36 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 #endif
 // AGG != 0: the (step, node)-keyed LDS hash of step_insert / step_flush; AGG 0: direct
 // atomics.  The lanes run their steps independently (lane_steps); K <= kStepMaxLaunch.
-template <int WALL, int AGG>
+// GATHER: the launch completes a cell sort whose rank pass (k_cell_rank) wrote only the
+// permutation: particle i is read from slot perm[i] of src (pid included) and written to slot
+// i of s, so the sorted order costs a 4 B index instead of a 56 B record copy.
+template <int WALL, int AGG, bool GATHER>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
-k_push_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, double* __restrict__ den,
-             int32_t nn, int K, int32_t* __restrict__ part, int cap) {
+k_push_steps(const DevTet* __restrict__ tets, Soa s, Soa src, const int32_t* __restrict__ perm, int64_t n,
+             double qm, double dt, double* __restrict__ den, int32_t nn, int K, int32_t* __restrict__ part,
+             int cap) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
-    int32_t c_in = -1;
+    int32_t c_in = -1, pid = 0;
     double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
     if (i < n) {
-        c_in = s.cell[i];
-        vx = ld(s.vx + i);
-        vy = ld(s.vy + i);
-        vz = ld(s.vz + i);
-        x = ld(s.x + i);
-        y = ld(s.y + i);
-        z = ld(s.z + i);
+        const int64_t j = GATHER ? (int64_t)perm[i] : i;
+        const Soa& r = GATHER ? src : s;
+        c_in = r.cell[j];
+        vx = ld(r.vx + j);
+        vy = ld(r.vy + j);
+        vz = ld(r.vz + j);
+        x = ld(r.x + j);
+        y = ld(r.y + j);
+        z = ld(r.z + j);
+        if (GATHER) pid = r.pid[j];
     }
     __builtin_amdgcn_sched_barrier(0);
     bool live = c_in >= 0;
     int32_t c = live ? c_in : 0;
     TetRegs t;
     load_tet_regs(tets + c, t);
+    if (GATHER && i < n) s.pid[i] = pid;
     constexpr bool kHash = AGG != 0;
     __shared__ uint32_t hk[kHash ? kStepSlots : 1];
     __shared__ double hv[kHash ? kStepSlots : 1];
@@ -1116,6 +1124,32 @@ k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ 
     if (src.run) dst.run[d] = run;
 }
 
+// The rank pass of a gather sort: k_cell_scatter's bucket ranges, but only the particle's
+// source slot is written (perm[d] = i).  The launch that consumes perm moves the records.
+__global__ void __launch_bounds__(kSortThreads)
+k_cell_rank(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift, int32_t nb,
+            int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+    __shared__ SortSlots t;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = (i < n) ? cell[i] : -1;
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x) {
+        t.key[q] = -1;
+        t.cnt[q] = 0;
+    }
+    __syncthreads();
+    int slot = -1, rank = 0;
+    if (c >= 0) {
+        slot = sort_slot(t, bucket_key(c, run, i, shift, nb));
+        rank = atomicAdd(&t.cnt[slot], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kSortHash; q += blockDim.x)
+        if (t.key[q] >= 0) t.base[q] = atomicAdd(cursor + t.key[q], t.cnt[q]);
+    __syncthreads();
+    if (c < 0) return;
+    perm[(int64_t)t.base[slot] + rank] = (int32_t)i;
+}
+
 // ---- exclusive scan (int32), 3 phases --------------------------------------------------
 
 constexpr int kScanThreads = 256;
@@ -1394,17 +1428,23 @@ hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStre
     // (step, node) keys are 32-bit offsets below the empty-slot marker
     if ((int64_t)nn * k_steps >= (int64_t)0xFFFFFFFFu) return hipErrorInvalidValue;
     const unsigned g = (grid_for(a.n, 256) + kXcds - 1) / kXcds * kXcds;
-#define MP_STEPS(W, A)                                                                                    \
-    hipLaunchKernelGGL((k_push_steps<W, A>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, a.den, \
-                       nn, k_steps, a.part, a.cap)
+#define MP_STEPS(W, A, G)                                                                                 \
+    hipLaunchKernelGGL((k_push_steps<W, A, G>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.src, a.perm, a.n,   \
+                       a.qm, a.dt, a.den, nn, k_steps, a.part, a.cap)
+#define MP_STEPS_W(W, A)                  \
+    do {                                  \
+        if (a.perm) MP_STEPS(W, A, true); \
+        else MP_STEPS(W, A, false);       \
+    } while (0)
     if (k_steps > kStepMaxLaunch) return hipErrorInvalidValue;
     if (a.wall == 0) {
-        if (a.agg == 0) MP_STEPS(0, 0);
-        else MP_STEPS(0, 2);
+        if (a.agg == 0) MP_STEPS_W(0, 0);
+        else MP_STEPS_W(0, 2);
     } else {
-        if (a.agg == 0) MP_STEPS(1, 0);
-        else MP_STEPS(1, 2);
+        if (a.agg == 0) MP_STEPS_W(1, 0);
+        else MP_STEPS_W(1, 2);
     }
+#undef MP_STEPS_W
 #undef MP_STEPS
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1586,6 +1626,15 @@ hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, kSortThreads)), dim3(kSortThreads), 0, st, src, n, shift,
                        nb, cursor, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
+                            int32_t* cursor, int32_t* perm, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (n > (int64_t)INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cell_rank, dim3(grid_for(n, kSortThreads)), dim3(kSortThreads), 0, st, cell, run, n, shift,
+                       nb, cursor, perm);
     return hipGetLastError();
 }
 

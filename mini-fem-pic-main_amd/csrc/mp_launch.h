@@ -42,6 +42,10 @@ struct PushArgs {
     bool defer = false;  // walk the crossers in a second kernel (queue wq / wq_cnt, defer_queue_size)
     uint32_t* wq = nullptr;
     int32_t* wq_cnt = nullptr;
+    // launch_push_steps only: with perm set, particle i is read from slot perm[i] of src and
+    // written to slot i of soa (the gather half of a cell sort)
+    Soa src{};
+    const int32_t* perm = nullptr;
 };
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
@@ -80,6 +84,10 @@ hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n,
                             int32_t* cnt, hipStream_t st);
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
                                const Soa& dst, hipStream_t st);
+// The rank pass alone: perm[d] = source slot of the particle that belongs at slot d (cursor =
+// bucket offsets, advanced); lost particles get no slot.  n < 2^31.
+hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
+                            int32_t* cursor, int32_t* perm, hipStream_t st);
 hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st);
 hipError_t launch_check_cells_aos(const mp_particle* p, int64_t n, int32_t n_tets, int32_t* first_bad,
                                   hipStream_t st);

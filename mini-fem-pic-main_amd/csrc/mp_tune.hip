@@ -181,6 +181,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
         sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 2});
+        sweep([](mp_options& o, int v) { o.gather_sort = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
     } else {

@@ -726,6 +726,63 @@ def test_autotune_multi_step_keeps_state_and_parity(mp, orc):
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
 
 
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_step_many_gather_sort(mp, orc, wall):
+    """The gather sort (k_cell_rank writes only the permutation, the next k_push_steps launch
+    moves the records) against the copy sort and the oracle: a sort every 3 steps, particles
+    injected between two calls (sorted in before the next launch), every step's density, the
+    particles and the counters; then steps that lose every particle, so that a later sort
+    finds none in the domain."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=83 + wall, n=60001)
+    dt = 4e-7
+    half = len(pos) // 2
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos[:half], vel[:half], cell[:half], wall_mode=wall,
+                  walk_cap=6)
+    movers = []
+    try:
+        for gather in (1, 0):
+            mv = mp.Mover.from_mesh(mesh)
+            movers.append(mv)
+            mv.set_options(sort_interval=3, wall_mode=wall, walk_cap=6, gather_sort=gather)
+            assert mv.options["gather_sort"] == gather
+            mv.upload_field(ef)
+            mv.set_particles(pos[:half], vel[:half], cell[:half])
+        for mv in movers:
+            mv.step_many(QM, dt, SPWT, 0, 7)
+        for k in range(7):
+            ref = run.step(QM, dt, SPWT)
+            for mv in movers:
+                check_den(mv.density_step(k), ref, 0, mp)
+        for mv in movers:
+            mv.add_particles(pos[half:], vel[half:], cell[half:])
+        run.pos = np.ascontiguousarray(np.concatenate([run.pos, pos[half:]]))
+        run.vel = np.ascontiguousarray(np.concatenate([run.vel, vel[half:]]))
+        run.cell = np.ascontiguousarray(np.concatenate([run.cell, cell[half:]]).astype(np.int32))
+        run.pid = np.concatenate([run.pid, np.arange(half, len(pos))])
+        for mv in movers:
+            mv.step_many(QM, dt, SPWT, 0, 8)
+        for k in range(8):
+            ref = run.step(QM, dt, SPWT)
+            for mv in movers:
+                check_den(mv.density_step(k), ref, 0, mp)
+        for mv in movers:
+            gp, gv, gc, gid = mv.get_particles()
+            assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+            assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+            c = mv.counters()
+            for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+                assert c[key] == run.stats[key], key
+        for mv in movers:
+            mv.step_many(QM, 1e-3, SPWT, 0, 4)  # every particle is capped or leaves
+            assert mv.num_active() == 0
+            mv.step_many(QM, 1e-3, SPWT, 0, 4)  # sorts a set with no particle in the domain
+            assert mv.num_active() == 0 and not mv.density_step(3).any()
+            assert mv.get_particles()[3].size == 0
+    finally:
+        for mv in movers:
+            mv.close()
+
+
 def test_step_many_long_and_empty(mp, orc):
     """mp_step_many over more steps than one launch takes (no periodic sort: 40 steps = launches
     of 32 and 8), and on an empty particle set (densities of zeros)."""
