@@ -166,6 +166,11 @@ int  mp_get_density_step(mp_ctx* ctx, int32_t k, double* den);
 int  mp_sync(mp_ctx* ctx);
 /* Number of resident particles still in the domain (blocks). */
 int  mp_num_active(mp_ctx* ctx, int64_t* n_active);
+/* Diagnostics (no reference counterpart): the resident slots in their current order, which
+ * is bucket order after a cell sort, lost slots included until the next sort or compaction:
+ * cell[i] (-1 = lost) and pid[i] of slot i.  Either pointer may be NULL.  *n_inout: capacity
+ * in, number of slots out; MP_ERR_ARG (with the count) when the capacity is short (blocks). */
+int  mp_get_slots(mp_ctx* ctx, int32_t* cell, int32_t* pid, int64_t* n_inout);
 /* Download the particles still in the domain in ascending pid order (blocks).
  * Any output pointer may be NULL.  *n_inout: capacity in, count out. */
 int  mp_get_particles(mp_ctx* ctx, double* pos, double* vel, int32_t* cell, int64_t* pid,

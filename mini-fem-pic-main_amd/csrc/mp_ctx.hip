@@ -787,6 +787,20 @@ int mp_num_active(mp_ctx* c, int64_t* n_active) {
     return MP_OK;
 }
 
+int mp_get_slots(mp_ctx* c, int32_t* cell, int32_t* pid, int64_t* n_inout) {
+    if (!c || !n_inout) return MP_ERR_ARG;
+    const int64_t cap = *n_inout;
+    *n_inout = c->n_slots;
+    if ((cell || pid) && cap < c->n_slots) return set_err(c, MP_ERR_ARG, "mp_get_slots: capacity too small");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Soa& s = c->soa[c->cur];
+    const size_t bytes = sizeof(int32_t) * (size_t)c->n_slots;
+    if (cell && bytes) HIP_TRY(c, hipMemcpyAsync(cell, s.cell, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (pid && bytes) HIP_TRY(c, hipMemcpyAsync(pid, s.pid, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
 int mp_get_particles(mp_ctx* c, double* pos, double* vel, int32_t* cell, int64_t* pid, int64_t* n_inout) {
     if (!c || !n_inout) return MP_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));

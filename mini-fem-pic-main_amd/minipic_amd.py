@@ -89,6 +89,7 @@ SIGNATURES = {
     "mp_get_density_step": (C.c_int, [_P, _I32, _P]),
     "mp_sync": (C.c_int, [_P]),
     "mp_num_active": (C.c_int, [_P, C.POINTER(_I64)]),
+    "mp_get_slots": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.POINTER(_I64)]),
     "mp_get_particles": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(_I64)]),
     "mp_get_density": (C.c_int, [_P, _P]),
     "mp_get_counters": (C.c_int, [_P, C.POINTER(MpCounters)]),
@@ -539,6 +540,15 @@ class Mover:
         n = C.c_int64()
         self._check(lib().mp_num_active(self._h, C.byref(n)))
         return int(n.value)
+
+    def slots(self):
+        """The resident slots in their current order (bucket order after a cell sort), lost
+        ones included until the next sort: (cell, pid), int32 each."""
+        n = C.c_int64(0)
+        self._check(lib().mp_get_slots(self._h, None, None, C.byref(n)))
+        cell, pid = np.empty(n.value, np.int32), np.empty(n.value, np.int32)
+        self._check(lib().mp_get_slots(self._h, _ptr(cell), _ptr(pid), C.byref(n)))
+        return cell, pid
 
     def get_particles(self):
         """Particles still in the domain, ascending pid: (pos, vel, cell, pid)."""

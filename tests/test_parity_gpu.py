@@ -783,6 +783,40 @@ def test_step_many_gather_sort(mp, orc, wall):
             mv.close()
 
 
+def check_bucket_order(mv, shift, live_pids):
+    cell, pid = mv.slots()
+    assert (cell >= 0).all(), "a cell sort drops the lost particles"
+    assert (np.diff(cell >> shift) >= 0).all(), "slots not in bucket order"
+    assert np.array_equal(np.sort(pid), np.sort(live_pids))
+
+
+@pytest.mark.parametrize("shift", [0, 3])
+@pytest.mark.parametrize("gather", [0, 1], ids=["copy", "gather"])
+def test_cell_sort_orders_buckets(mp, shift, gather):
+    """The cell sort's result itself (parity holds for any permutation): from the host's random
+    order, every live particle once, in non-decreasing cell >> shift; then a few steps later,
+    from near-sorted order, with the lost particles dropped.  With the gather sort the sort
+    is completed by the next mp_step_many launch, run here with dt = 0 so that no particle
+    changes cell after it."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=91, n=300001)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=4, sort_shift=shift, gather_sort=gather)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        c0, p0 = mv.slots()
+        assert np.array_equal(p0, np.arange(len(pos))) and np.array_equal(c0, cell)
+        mv.sort_now()
+        check_bucket_order(mv, shift, np.arange(len(pos)))
+        mv.step_many(QM, 4e-7, SPWT, 0, 3)  # no sort: 3 of the 4 steps of the interval
+        live = mv.get_particles()[3]
+        assert 0 < len(live) < len(pos)
+        if gather:
+            mv.step_many(QM, 0.0, SPWT, 0, 2)  # the 4th step, then the sort and one more step
+        else:
+            mv.sort_now()
+        check_bucket_order(mv, shift, live)
+
+
 def test_step_many_long_and_empty(mp, orc):
     """mp_step_many over more steps than one launch takes (no periodic sort: 40 steps = launches
     of 32 and 8), and on an empty particle set (densities of zeros)."""
