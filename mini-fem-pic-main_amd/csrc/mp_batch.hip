@@ -80,9 +80,15 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
 // the particle and its tet's record stay in registers, each lane runs its steps on its own
 // (lane_steps); step k scatters into den + k * R * nn (run r's nodes at r * nn) through the
 // (step, node)-keyed LDS hash.  Atomic scatter only.
-// 5 waves per SIMD (96 VGPRs, a few bytes of spill): 5% faster than 4 waves at 102 VGPRs
+// log2 of the (step, node) LDS hash slots of a workgroup: 4096 slots (48 KB, 3 workgroups
+// per CU).  The runs of an ensemble are often sparse (a workgroup's particles spread over
+// hundreds of tets), so a 2048-slot table overflows into memory-side atomics, which bound the
+// launch; 4096 slots are 4% faster on 64 runs of 1e5..4e6 particles despite the lower occupancy.
+#ifndef MP_BATCH_STEP_BITS
+#define MP_BATCH_STEP_BITS 12
+#endif
 #ifndef MP_BATCH_STEPS_WAVES
-#define MP_BATCH_STEPS_WAVES 5
+#define MP_BATCH_STEPS_WAVES 3
 #endif
 template <int WALL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_BATCH_STEPS_WAVES, 8)))
@@ -108,16 +114,17 @@ k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const doub
     const double qm = par[3 * r], dt = par[3 * r + 1];
     TetRegs t;
     load_tet_regs(tets + c, t);
-    __shared__ uint32_t hk[kStepSlots];
-    __shared__ double hv[kStepSlots];
-    for (int q = threadIdx.x; q < kStepSlots; q += blockDim.x) {
+    __shared__ uint32_t hk[1 << MP_BATCH_STEP_BITS];
+    __shared__ double hv[1 << MP_BATCH_STEP_BITS];
+    for (int q = threadIdx.x; q < (1 << MP_BATCH_STEP_BITS); q += blockDim.x) {
         hk[q] = 0xFFFFFFFFu;
         hv[q] = 0.0;
     }
     __syncthreads();
-    lane_steps<WALL, true>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den, (uint32_t)r * (uint32_t)nn,
-                           (uint32_t)R * (uint32_t)nn, hk, hv, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
-    step_flush(hk, hv, den);
+    lane_steps<WALL, true, MP_BATCH_STEP_BITS>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den,
+                                               (uint32_t)r * (uint32_t)nn, (uint32_t)R * (uint32_t)nn, hk, hv, n_act,
+                                               n_lost, n_cross, n_moves, n_cap, n_refl);
+    step_flush<MP_BATCH_STEP_BITS>(hk, hv, den);
     if (c_in >= 0) {
         s.cell[i] = live ? c : -1;
         st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);

@@ -330,28 +330,32 @@ __device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, dou
 // is flushed (one global atomic per occupied slot) once, at the end of the launch, which holds
 // at most kStepMaxLaunch steps.  A contribution that finds no slot within kStepProbe probes
 // (more distinct keys than the table holds, e.g. unsorted particles) is added to HBM directly.
-constexpr int kStepSlots = 2048;
+// The table has 1 << BITS slots (12 B each).
+constexpr int kStepBits = 11;
+constexpr int kStepSlots = 1 << kStepBits;
 constexpr int kStepProbe = 32;
 
+template <int BITS = kStepBits>
 __device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __restrict__ den, uint32_t key,
                                             double w) {
-    uint32_t h = (key * 0x9E3779B1u) >> (32 - 11);
+    uint32_t h = (key * 0x9E3779B1u) >> (32 - BITS);
     for (int p = 0; p < kStepProbe; p++) {
         const uint32_t prev = atomicCAS(&hk[h], 0xFFFFFFFFu, key);
         if (prev == 0xFFFFFFFFu || prev == key) {
             atomicAdd(&hv[h], w);
             return;
         }
-        h = (h + 1) & (kStepSlots - 1);
+        h = (h + 1) & ((1u << BITS) - 1);
     }
     unsafeAtomicAdd(den + key, w);
 }
 
 // Every thread of the workgroup calls it: global atomics for the occupied slots, which are
 // then emptied; barriers on both sides.
+template <int BITS = kStepBits>
 __device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __restrict__ den) {
     __syncthreads();
-    for (int s = threadIdx.x; s < kStepSlots; s += blockDim.x) {
+    for (int s = threadIdx.x; s < (1 << BITS); s += blockDim.x) {
         const uint32_t key = hk[s];
         if (key != 0xFFFFFFFFu) {
             unsafeAtomicAdd(den + key, hv[s]);
@@ -370,7 +374,7 @@ __device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __r
 // sum over the steps of its slowest walk.  The arithmetic of every step is walk_regs' after the
 // push, in the same order.  Step k's weights go to key0 + k * per_step + node: the LDS hash
 // (HASH) or direct atomics on den.  Counters as in k_push_soa.
-template <int WALL, bool HASH>
+template <int WALL, bool HASH, int BITS = kStepBits>
 __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetRegs& t, int32_t& c, bool& live,
                                            double& x, double& y, double& z, double& vx, double& vy, double& vz,
                                            double qm, double dt, int K, int cap, double* __restrict__ den,
@@ -404,10 +408,10 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
         if (!out) {
             const uint32_t base = key0 + (uint32_t)k * per_step;
             if (HASH) {
-                step_insert(hk, hv, den, base + (uint32_t)t.m0, l0);
-                step_insert(hk, hv, den, base + (uint32_t)t.m1, l1);
-                step_insert(hk, hv, den, base + (uint32_t)t.m2, l2);
-                step_insert(hk, hv, den, base + (uint32_t)t.m3, l3);
+                step_insert<BITS>(hk, hv, den, base + (uint32_t)t.m0, l0);
+                step_insert<BITS>(hk, hv, den, base + (uint32_t)t.m1, l1);
+                step_insert<BITS>(hk, hv, den, base + (uint32_t)t.m2, l2);
+                step_insert<BITS>(hk, hv, den, base + (uint32_t)t.m3, l3);
             } else {
                 unsafeAtomicAdd(den + base + t.m0, l0);
                 unsafeAtomicAdd(den + base + t.m1, l1);
