@@ -154,13 +154,20 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 #ifndef MP_STEPS_WAVES
 #define MP_STEPS_WAVES 5
 #endif
+// threads per workgroup of k_push_steps, and log2 of its (step, node) LDS hash slots
+#ifndef MP_STEPS_THREADS
+#define MP_STEPS_THREADS 256
+#endif
+#ifndef MP_STEPS_BITS
+#define MP_STEPS_BITS 11
+#endif
 // AGG != 0: the (step, node)-keyed LDS hash of step_insert / step_flush; AGG 0: direct
 // atomics.  The lanes run their steps independently (lane_steps); K <= kStepMaxLaunch.
 // GATHER: the launch completes a cell sort whose rank pass (k_cell_rank) wrote only the
 // permutation: particle i is read from slot perm[i] of src (pid included) and written to slot
 // i of s, so the sorted order costs a 4 B index instead of a 56 B record copy.
 template <int WALL, int AGG, bool GATHER>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
+__global__ void __launch_bounds__(MP_STEPS_THREADS) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
 k_push_steps(const DevTet* __restrict__ tets, Soa s, Soa src, const int32_t* __restrict__ perm, int64_t n,
              double qm, double dt, double* __restrict__ den, int32_t nn, int K, int32_t* __restrict__ part,
              int cap) {
@@ -187,18 +194,18 @@ k_push_steps(const DevTet* __restrict__ tets, Soa s, Soa src, const int32_t* __r
     load_tet_regs(tets + c, t);
     if (GATHER && i < n) s.pid[i] = pid;
     constexpr bool kHash = AGG != 0;
-    __shared__ uint32_t hk[kHash ? kStepSlots : 1];
-    __shared__ double hv[kHash ? kStepSlots : 1];
+    __shared__ uint32_t hk[kHash ? (1 << MP_STEPS_BITS) : 1];
+    __shared__ double hv[kHash ? (1 << MP_STEPS_BITS) : 1];
     if (kHash) {
-        for (int q = threadIdx.x; q < kStepSlots; q += blockDim.x) {
+        for (int q = threadIdx.x; q < (1 << MP_STEPS_BITS); q += blockDim.x) {
             hk[q] = 0xFFFFFFFFu;
             hv[q] = 0.0;
         }
         __syncthreads();
     }
-    lane_steps<WALL, kHash>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den, 0u, (uint32_t)nn, hk, hv,
-                            n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
-    if (kHash) step_flush(hk, hv, den);
+    lane_steps<WALL, kHash, MP_STEPS_BITS>(tets, t, c, live, x, y, z, vx, vy, vz, qm, dt, K, cap, den, 0u,
+                                           (uint32_t)nn, hk, hv, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+    if (kHash) step_flush<MP_STEPS_BITS>(hk, hv, den);
     if (c_in >= 0) {
         s.cell[i] = live ? c : -1;
         st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
@@ -1560,10 +1567,10 @@ hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStre
     if (a.n <= 0 || k_steps <= 0) return hipSuccess;
     // (step, node) keys are 32-bit offsets below the empty-slot marker
     if ((int64_t)nn * k_steps >= (int64_t)0xFFFFFFFFu) return hipErrorInvalidValue;
-    const unsigned g = (grid_for(a.n, 256) + kXcds - 1) / kXcds * kXcds;
+    const unsigned g = (grid_for(a.n, MP_STEPS_THREADS) + kXcds - 1) / kXcds * kXcds;
 #define MP_STEPS(W, A, G)                                                                                 \
-    hipLaunchKernelGGL((k_push_steps<W, A, G>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.src, a.perm, a.n,   \
-                       a.qm, a.dt, a.den, nn, k_steps, a.part, a.cap)
+    hipLaunchKernelGGL((k_push_steps<W, A, G>), dim3(g), dim3(MP_STEPS_THREADS), 0, st, a.tets, a.soa, a.src,    \
+                       a.perm, a.n, a.qm, a.dt, a.den, nn, k_steps, a.part, a.cap)
 #define MP_STEPS_W(W, A)                  \
     do {                                  \
         if (a.perm) MP_STEPS(W, A, true); \
