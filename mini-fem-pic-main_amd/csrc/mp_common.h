@@ -327,8 +327,8 @@ __device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, dou
 // Multi-step kernels: one LDS hash per workgroup keyed by (step, node) = the offset of the
 // contribution in the launch's densities (step k of a launch at k * nodes per step), so the
 // steps need no workgroup barrier between them and lanes may be at different steps; the table
-// is flushed (one global atomic per occupied slot) once, at the end of the launch, which holds
-// at most kStepMaxLaunch steps.  A contribution that finds no slot within kStepProbe probes
+// is flushed (one global atomic per occupied slot) once, by the workgroup's last wave to
+// finish (steps_finish), at the end of a launch of at most kStepMaxLaunch steps.  A contribution that finds no slot within kStepProbe probes
 // (more distinct keys than the table holds, e.g. unsorted particles) is added to HBM directly.
 // The table has 1 << BITS slots (12 B each).
 constexpr int kStepBits = 11;
@@ -350,20 +350,39 @@ __device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __
     unsafeAtomicAdd(den + key, w);
 }
 
-// Every thread of the workgroup calls it: global atomics for the occupied slots, which are
-// then emptied; barriers on both sides.
-template <int BITS = kStepBits>
-__device__ __forceinline__ void step_flush(uint32_t* hk, double* hv, double* __restrict__ den) {
-    __syncthreads();
-    for (int s = threadIdx.x; s < (1 << BITS); s += blockDim.x) {
-        const uint32_t key = hk[s];
-        if (key != 0xFFFFFFFFu) {
-            unsafeAtomicAdd(den + key, hv[s]);
-            hk[s] = 0xFFFFFFFFu;
-            hv[s] = 0.0;
+// The end of a multi-step workgroup without a barrier: each wave posts its counters in LDS
+// (shc, 6 per wave) and leaves; the last wave to arrive (the done counter) flushes the
+// (step, node) hash (HASH) and writes the workgroup's six tallies, as tally_counts does.  Waves
+// that finish early free their slots instead of waiting for the workgroup's slowest lane.
+// Every lane of every wave calls it once, after its last use of the hash.
+template <int BITS, bool HASH>
+__device__ __forceinline__ void steps_finish(uint32_t* hk, double* hv, double* __restrict__ den,
+                                             int32_t* __restrict__ part, int32_t* shc, int32_t* done, int a,
+                                             int l, int x, int m, int k, int r) {
+    const int v[6] = {wave_sum(a), wave_sum(l), wave_sum(x), wave_sum(m), wave_sum(k), wave_sum(r)};
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 6; q++) shc[w * 6 + q] = v[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    int prev = 0;
+    if (lane == 0) prev = atomicAdd(done, 1);
+    prev = __builtin_amdgcn_readfirstlane(prev);
+    if (prev != nw - 1) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (HASH) {
+        for (int s = lane; s < (1 << BITS); s += kWave) {
+            const uint32_t key = hk[s];
+            if (key != 0xFFFFFFFFu) unsafeAtomicAdd(den + key, hv[s]);
         }
     }
-    __syncthreads();
+    if (lane < 6) {
+        int sum = 0;
+        for (int q = 0; q < nw; q++) sum += shc[q * 6 + lane];
+        part[(int64_t)blockIdx.x * 6 + lane] = sum;
+    }
 }
 
 // K steps of one particle whose tet record t is in registers, each lane on its own: one
