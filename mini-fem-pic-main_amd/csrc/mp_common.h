@@ -328,8 +328,9 @@ __device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, dou
 // contribution in the launch's densities (step k of a launch at k * nodes per step), so the
 // steps need no workgroup barrier between them and lanes may be at different steps; the table
 // is flushed (one global atomic per occupied slot) once, by the workgroup's last wave to
-// finish (steps_finish), at the end of a launch of at most kStepMaxLaunch steps.  A contribution that finds no slot within kStepProbe probes
-// (more distinct keys than the table holds, e.g. unsorted particles) is added to HBM directly.
+// finish (steps_finish), at the end of a launch of at most kStepMaxLaunch steps.  A
+// contribution that finds no slot within kStepProbe probes (more distinct keys than the table
+// holds, e.g. unsorted particles) is added to HBM directly.
 // The table has 1 << BITS slots (12 B each).
 constexpr int kStepBits = 11;
 constexpr int kStepSlots = 1 << kStepBits;
