@@ -709,7 +709,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
         int32_t chunk = n_steps - done;
         if (si > 0) chunk = (int32_t)std::min<int64_t>(chunk, si - (gather ? 0 : c->since_sort));
         chunk = std::min<int32_t>(chunk, mpd::kStepMaxLaunch);
-        int rc = reserve_tally(c, c->n_slots, 256);
+        int rc = reserve_tally(c, c->n_slots, mpd::steps_block());
         if (rc) return rc;
         mpd::PushArgs a;
         a.tets = c->tets;

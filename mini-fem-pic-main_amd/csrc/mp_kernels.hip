@@ -1568,6 +1568,8 @@ static hipError_t launch_push_defer(const PushArgs& a, hipStream_t st) {
     return launch_tally_finish(a.part, gw, a.ctr, st);
 }
 
+int steps_block() { return MP_STEPS_THREADS; }
+
 hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStream_t st) {
     if (a.n <= 0 || k_steps <= 0) return hipSuccess;
     // (step, node) keys are 32-bit offsets below the empty-slot marker

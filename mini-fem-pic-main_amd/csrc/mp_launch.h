@@ -52,6 +52,8 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st);
 // k_steps <= kStepMaxLaunch steps in one launch (atomic scatter; aggregation 0 = direct
 // atomics, any other = the (step, node) LDS hash); step k scatters into a.den + k * nn.
 hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStream_t st);
+// workgroup size of launch_push_steps (its per-block tallies: tally_ints(n, steps_block()))
+int steps_block();
 hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, int k_steps, double spwt,
                                   hipStream_t st);
 hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
