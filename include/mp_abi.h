@@ -67,7 +67,7 @@ typedef struct mp_options {
     int32_t wall_mode;             /* MP_WALL_ABSORB / MP_WALL_REFLECT                 */
     int32_t block_size;            /* threads per workgroup of the push kernel         */
     int32_t threads_per_particle;  /* 1 or 4 (4: one lane per barycentric coordinate)  */
-    int32_t particles_per_thread;  /* 1, 2 or 4: rounds of block_size particles per thread */
+    int32_t particles_per_thread;  /* 1, 2, 4 or 8: rounds of block_size particles per thread */
     int32_t walk_cap;              /* max face moves per particle per step             */
     int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
                                       2 per workgroup by node (LDS hash), 3 persistent

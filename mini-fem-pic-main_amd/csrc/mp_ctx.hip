@@ -338,7 +338,8 @@ bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
            (o.threads_per_particle == 1 || (o.threads_per_particle == 4 && o.particles_per_thread == 1)) &&
-           (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4) &&
+           (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4 ||
+            o.particles_per_thread == 8) &&
            o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
            o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&

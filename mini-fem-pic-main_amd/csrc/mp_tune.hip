@@ -190,7 +190,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
         sweep([](mp_options& o, int v) { o.defer_walk = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
-        sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4});
+        sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4, 8});
         sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
         sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.compact_crossers = v; }, {0, 1});

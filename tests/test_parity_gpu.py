@@ -236,6 +236,33 @@ def test_atomic_aggregation_modes(mp, orc, agg):
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
 
 
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+@pytest.mark.parametrize("ppt,block", [(2, 256), (4, 128), (8, 64)])
+def test_particles_per_thread_walls_and_cap(mp, orc, wall, ppt, block):
+    """Several rounds of particles per thread with the LDS hash in atomic mode: 6 steps without
+    a sort, so lost slots stay in place between live ones, with a walk cap of 3 that the fast
+    particles hit and a ragged last workgroup; cells, positions, velocities and counters
+    bit-exact, every step's density to the atomic bar."""
+    mesh, ef, pos, vel, cell = box_case(mp, seed=97 + wall, n=50003)
+    dt = 1.2e-6
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=3)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(particles_per_thread=ppt, scatter_agg=2, block_size=block, sort_interval=0, wall_mode=wall,
+                       walk_cap=3)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        for _ in range(6):
+            mv.step(QM, dt, SPWT, mp.SCATTER_ATOMIC)
+            check_den(mv.density(), run.step(QM, dt, SPWT), mp.SCATTER_ATOMIC, mp)
+        gp, gv, gc, gid = mv.get_particles()
+        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
+        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
+        c = mv.counters()
+        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
+            assert c[key] == run.stats[key], key
+        assert c["cap_hits"] > 0 and c["lost"] > c["cap_hits"] * (1 - wall) and (wall == 0 or c["reflections"] > 0)
+
+
 @pytest.mark.parametrize("ppt,agg,block", [(2, 2, 256), (4, 4, 128), (2, 4, 64), (4, 0, 256)])
 def test_particles_per_thread_matches_oracle(mp, orc, ppt, agg, block):
     """Several particles per thread (and per-wave aggregation, smaller blocks) change only the
@@ -445,7 +472,7 @@ def test_autotune_leaves_state_and_keeps_parity(mp, orc):
             assert a.tobytes() == b.tobytes()
         assert before[1].tobytes() == after[1].tobytes() and before[2] == after[2]
         assert mv.options == best
-        assert best["threads_per_particle"] in (1, 4) and best["particles_per_thread"] in (1, 2, 4)
+        assert best["threads_per_particle"] in (1, 4) and best["particles_per_thread"] in (1, 2, 4, 8)
         assert best["block_size"] in (64, 128, 256) and 0 <= best["scatter_agg"] <= 4
         # the reported time is one of the chosen options' timings, within 1% of the fastest trial
         # or from the final re-check against the starting options
