@@ -188,7 +188,9 @@ int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
             c->vals[k] = nullptr;
         }
         c->rec_cap = 0;
-        const int64_t nc = std::max<int64_t>(m + m / 4, 4096);
+        // doubling: the pid-indexed record count grows with every injection, and each
+        // reallocation's free synchronises the device
+        const int64_t nc = std::max<int64_t>(2 * m, 4096);
         for (int k = 0; k < 2; k++)
             if (dalloc(&c->keys[k], nc) != hipSuccess || dalloc(&c->vals[k], nc) != hipSuccess)
                 return set_err(c, MP_ERR_ALLOC, "deterministic scatter buffers: allocation failed");
@@ -425,7 +427,8 @@ void mp_destroy(mp_ctx* c) {
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
-                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps, c->perm};
+                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps, c->perm, c->ef_stage,
+                    c->live_cnt};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     mpstage::stager_free(c->stage);
@@ -460,12 +463,12 @@ int mp_upload_field(mp_ctx* c, const double* ef, int32_t n_tets) {
     if (!c) return MP_ERR_ARG;
     if (!ef || n_tets != c->n_tets) return set_err(c, MP_ERR_ARG, "mp_upload_field: ef must be [n_tets][3]");
     HIP_TRY(c, hipSetDevice(c->device));
-    double* tmp = nullptr;
-    HIP_TRY(c, dalloc(&tmp, 3 * (int64_t)n_tets));
-    hipError_t e = hipMemcpyAsync(tmp, ef, sizeof(double) * 3 * (size_t)n_tets, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = mpd::launch_set_field(c->tets, tmp, n_tets, c->stream);
+    // the staging buffer is kept: a free per upload would synchronise the device every step
+    if (!c->ef_stage) HIP_TRY(c, dalloc(&c->ef_stage, 3 * (int64_t)n_tets));
+    hipError_t e =
+        hipMemcpyAsync(c->ef_stage, ef, sizeof(double) * 3 * (size_t)n_tets, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = mpd::launch_set_field(c->tets, c->ef_stage, n_tets, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dfree(tmp);
     if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_upload_field: ") + hipGetErrorString(e));
     c->have_field = true;
     return MP_OK;
@@ -777,14 +780,13 @@ int mp_sort_now(mp_ctx* c) {
 int mp_num_active(mp_ctx* c, int64_t* n_active) {
     if (!c || !n_active) return MP_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
-    std::vector<int32_t> cell((size_t)c->n_slots);
-    if (c->n_slots > 0)
-        HIP_TRY(c, hipMemcpyAsync(cell.data(), c->soa[c->cur].cell, sizeof(int32_t) * cell.size(),
-                                  hipMemcpyDeviceToHost, c->stream));
+    // counted on the device: one 8 B read-back instead of the whole cell array
+    if (!c->live_cnt) HIP_TRY(c, dalloc(&c->live_cnt, 1));
+    HIP_TRY(c, mpd::launch_count_live(c->soa[c->cur].cell, c->n_slots, c->live_cnt, c->stream));
+    unsigned long long* host = reinterpret_cast<unsigned long long*>(c->pinned_i32 + 2);
+    HIP_TRY(c, hipMemcpyAsync(host, c->live_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    int64_t k = 0;
-    for (int32_t v : cell) k += v >= 0;
-    *n_active = k;
+    *n_active = (int64_t)*host;
     return MP_OK;
 }
 

@@ -35,6 +35,8 @@ struct mp_ctx {
     int64_t cmp_n = 0;
     int32_t* perm = nullptr;  // gather sort: source slot of each sorted slot
     int64_t perm_n = 0;
+    double* ef_stage = nullptr;  // the field's H2D staging (3 * n_tets), kept between uploads
+    unsigned long long* live_cnt = nullptr;  // mp_num_active's device count
     // counting-sort and scan scratch
     int32_t* cell_cnt = nullptr;
     int32_t* cell_cur = nullptr;

@@ -1387,6 +1387,17 @@ __global__ void k_flag_soa(const int32_t* __restrict__ cell, int64_t n, int32_t*
     if (i < n) flag[i] = cell[i] >= 0 ? 1 : 0;
 }
 
+// Live slots (cell >= 0), one atomic per wave; count was cleared by k_fill_u64.
+__global__ void k_count_live(const int32_t* __restrict__ cell, int64_t n, unsigned long long* __restrict__ count) {
+    unsigned long long k = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        k += cell[i] >= 0;
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, kWave);
+    if (lane_id() == 0 && k) atomicAdd(count, k);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, unsigned long long v) { *p = v; }
+
 // Copy particle i to slot dst_slot(i): off[i] (stable compaction) or off[pid] (pid order).
 template <bool BY_PID>
 __global__ void k_move_soa(Soa src, int64_t n, const int32_t* __restrict__ off, Soa dst) {
@@ -1788,6 +1799,15 @@ hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n,
     if (n > (int64_t)INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cell_rank, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, cell, run, n, shift,
                        nb, cursor, perm);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_live(const int32_t* cell, int64_t n, unsigned long long* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, count, 0ull);
+    if (n > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>(1024, grid_for(n, 256));
+        hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, st, cell, n, count);
+    }
     return hipGetLastError();
 }
 

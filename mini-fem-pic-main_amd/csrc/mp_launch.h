@@ -78,6 +78,8 @@ int64_t scan_scratch_ints(int64_t n);
 hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
 // stable compaction / pid reordering of the resident particles
 hipError_t launch_flag_soa(const int32_t* cell, int64_t n, int32_t* flag, hipStream_t st);
+// *count = number of cell[i] >= 0, i < n (count is cleared first)
+hipError_t launch_count_live(const int32_t* cell, int64_t n, unsigned long long* count, hipStream_t st);
 hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool by_pid, const Soa& dst,
                            hipStream_t st);
 hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st);
