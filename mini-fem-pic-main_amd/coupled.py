@@ -237,6 +237,7 @@ class CoupledRun:
         self.move = move if move is not None else GpuMover(mesh, self.q_over_m, params.dt, spwt, mode)
         self.resident = getattr(self.move, "resident", False)
         self.parts = np.zeros(0, mp.PARTICLE_DTYPE)
+        self._buf = None  # host particle storage with spare capacity (a std::vector's growth)
         self.n_parts = 0
         self.rem = 0.0
         self.ef = np.zeros((mesh.n_tets, 3))  # zero until the first solve
@@ -250,7 +251,18 @@ class CoupledRun:
         if self.resident:
             self.parts = new
         else:
-            self.parts = np.concatenate([self.parts, new]) if len(self.parts) else new.copy()
+            # append in amortised O(1), as the reference's vector insert does: the survivors are
+            # a prefix view of the buffer after the move (it compacts in place)
+            n, k = len(self.parts), len(new)
+            buf = self._buf
+            if buf is None or n + k > len(buf):
+                buf = np.empty(max(2 * (n + k), 1024), mp.PARTICLE_DTYPE)
+                buf[:n] = self.parts
+                self._buf = buf
+            elif n and self.parts.ctypes.data != buf.ctypes.data:
+                buf[:n] = self.parts
+            buf[n:n + k] = new
+            self.parts = buf[:n + k]
         self.n_parts += len(new)
         return len(new)
 
