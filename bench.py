@@ -476,8 +476,9 @@ def main():
     ap.add_argument("--coupled-steps", type=int, default=200, help="steps of the coupled PIC loop (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run here uses a single "
-              "MI355X; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the mover runs on a single "
+              "MI355X; several GPUs would each take a slab of cell-sort buckets and exchange the particles "
+              "that cross slab faces once per step)", file=sys.stderr)
         sys.exit(2)
 
     t = time.perf_counter()
