@@ -96,6 +96,11 @@ typedef struct mp_options {
                                       writes only a permutation and the next multi-step
                                       launch reads each particle through it into the other
                                       buffer; 0 (default): the sort copies the records */
+    int32_t sort_order;            /* cell sort key: 0 (default) cell >> sort_shift; 1 the
+                                      tet's rank along a Morton curve through the tet
+                                      centroids, >> sort_shift (buckets compact in space:
+                                      fewer distinct nodes per workgroup where particles are
+                                      sparse, a few per tet) */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
@@ -171,6 +176,9 @@ int  mp_num_active(mp_ctx* ctx, int64_t* n_active);
  * cell[i] (-1 = lost) and pid[i] of slot i.  Either pointer may be NULL.  *n_inout: capacity
  * in, number of slots out; MP_ERR_ARG (with the count) when the capacity is short (blocks). */
 int  mp_get_slots(mp_ctx* ctx, int32_t* cell, int32_t* pid, int64_t* n_inout);
+/* Diagnostics (no reference counterpart): rank[e] = position of tet e along the Morton curve
+ * of sort_order 1 (a permutation of 0..n_tets-1; blocks). */
+int  mp_get_tet_order(mp_ctx* ctx, int32_t* rank);
 /* Download the particles still in the domain in ascending pid order (blocks).
  * Any output pointer may be NULL.  *n_inout: capacity in, count out. */
 int  mp_get_particles(mp_ctx* ctx, double* pos, double* vel, int32_t* cell, int64_t* pid,

@@ -321,7 +321,8 @@ int batch_sort(mp_batch* b) {
     const Soa& src = b->soa[b->cur];
     const Soa& dst = b->soa[1 - b->cur];
     HIP_TRY(c, mpd::launch_fill_i32(b->cnt, nkeys + 1, 0, c->stream));
-    HIP_TRY(c, mpd::launch_cell_hist(src.cell, src.run, b->n_slots, shift, (int32_t)nb, b->cnt, c->stream));
+    const int32_t* trank = c->opt.sort_order == 1 ? c->tet_rank : nullptr;
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, src.run, b->n_slots, shift, (int32_t)nb, b->cnt, c->stream, trank));
     HIP_TRY(c, mpd::launch_excl_scan(b->cnt, nkeys + 1, b->cursor, c->scan_tmp, c->stream));
     hipLaunchKernelGGL(mpd::k_gather_stride, dim3(mpd::grid256(b->R + 1)), dim3(256), 0, c->stream, b->cursor, nb,
                        b->R + 1, b->runs_d);
@@ -329,7 +330,7 @@ int batch_sort(mp_batch* b) {
     hipLaunchKernelGGL(mpd::k_set_count, dim3(1), dim3(64), 0, c->stream, b->cursor + nkeys, b->n_dev);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(b->pinned, b->runs_d, sizeof(int32_t) * (size_t)(b->R + 1), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, mpd::launch_cell_scatter(src, b->n_slots, shift, (int32_t)nb, b->cursor, dst, c->stream));
+    HIP_TRY(c, mpd::launch_cell_scatter(src, b->n_slots, shift, (int32_t)nb, b->cursor, dst, c->stream, trank));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     b->cur = 1 - b->cur;
     for (int32_t r = 0; r <= b->R; r++) b->run_start[r] = b->pinned[r];

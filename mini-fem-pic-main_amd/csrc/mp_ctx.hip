@@ -9,6 +9,7 @@
 // The device-resident path (mp_step) keeps particles in HBM as structure of arrays with a
 // particle id (pid) per particle; pid order is the order a serial MoveParticles would see.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -110,6 +111,7 @@ mp_options default_options() {
     o.compact_crossers = 0;
     o.defer_walk = 0;
     o.gather_sort = 0;
+    o.sort_order = 0;
     return o;
 }
 
@@ -272,6 +274,9 @@ bool sort_due(const mp_ctx* c) {
     return si > 0 && (c->need_sort || c->since_sort >= si);
 }
 
+// The tet order of the cell sort's key (sort_order 1), or none (cell order).
+const int32_t* sort_rank(const mp_ctx* c) { return c->opt.sort_order == 1 ? c->tet_rank : nullptr; }
+
 int sort_now(mp_ctx* c) {
     if (c->opt.keep_pid_order) return compact_now(c);
     c->since_sort = 0;
@@ -286,11 +291,13 @@ int sort_now(mp_ctx* c) {
     HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
     const Soa& src = c->soa[c->cur];
     const Soa& dst = c->soa[1 - c->cur];
-    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream,
+                                     sort_rank(c)));
     HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
-    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, (int32_t)nb, c->cell_cur, dst, c->stream));
+    HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, (int32_t)nb, c->cell_cur, dst, c->stream,
+                                        sort_rank(c)));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->cur = 1 - c->cur;
     c->n_slots = *c->pinned_i32;
@@ -317,12 +324,13 @@ int rank_now(mp_ctx* c, int64_t* n_live) {
     }
     const Soa& src = c->soa[c->cur];
     HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
-    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream));
+    HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream,
+                                     sort_rank(c)));
     HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, mpd::launch_cell_rank(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cur, c->perm,
-                                     c->stream));
+                                     c->stream, sort_rank(c)));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     *n_live = *c->pinned_i32;
     return MP_OK;
@@ -346,12 +354,74 @@ bool good_options(const mp_options& o) {
            o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
            (o.compact_crossers == 0 || o.compact_crossers == 1) && (o.defer_walk == 0 || o.defer_walk == 1) &&
-           (o.gather_sort == 0 || o.gather_sort == 1);
+           (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1);
 }
 
 }  // namespace
 
 extern "C" {
+
+// Rank of every tet along a Morton (Z-order) curve through the tet centroids, 10 bits per
+// axis over the centroids' bounding box, ties by tet id.  The centroid is the point where all
+// four barycentrics are 1/4: lambda_i = ((alpha_i - x beta_i) + y gamma_i - z delta_i) / (6 V),
+// so rows i = 1..3 give a 3 x 3 system, solved by Cramer's rule.
+static std::vector<int32_t> morton_tet_rank(const mp_tetra* tets, int32_t n) {
+    std::vector<double> cx((size_t)n), cy((size_t)n), cz((size_t)n);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int32_t e = 0; e < n; e++) {
+        const mp_tetra& t = tets[e];
+        double a[3][3], b[3];
+        for (int r = 0; r < 3; r++) {
+            a[r][0] = -t.beta[r + 1];
+            a[r][1] = t.gamma[r + 1];
+            a[r][2] = -t.delta[r + 1];
+            b[r] = 1.5 * t.volume - t.alpha[r + 1];
+        }
+        auto det3 = [](const double m[3][3]) {
+            return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                   m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                   m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        };
+        const double d = det3(a);
+        double x[3] = {0.0, 0.0, 0.0};
+        if (d != 0.0) {
+            for (int k = 0; k < 3; k++) {
+                double m[3][3];
+                for (int r = 0; r < 3; r++)
+                    for (int q = 0; q < 3; q++) m[r][q] = q == k ? b[r] : a[r][q];
+                x[k] = det3(m) / d;
+            }
+        }
+        cx[e] = x[0];
+        cy[e] = x[1];
+        cz[e] = x[2];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], x[k]);
+            hi[k] = std::max(hi[k], x[k]);
+        }
+    }
+    auto spread = [](uint32_t v) {
+        v &= 0x3FFu;
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    auto quant = [](double v, double l, double h) {
+        const double f = h > l ? (v - l) / (h - l) : 0.0;
+        return (uint32_t)std::min(1023.0, std::max(0.0, std::floor(f * 1023.0)));
+    };
+    std::vector<std::pair<uint32_t, int32_t>> code((size_t)n);
+    for (int32_t e = 0; e < n; e++)
+        code[e] = {spread(quant(cx[e], lo[0], hi[0])) | (spread(quant(cy[e], lo[1], hi[1])) << 1) |
+                       (spread(quant(cz[e], lo[2], hi[2])) << 2),
+                   e};
+    std::sort(code.begin(), code.end());
+    std::vector<int32_t> rank((size_t)n);
+    for (int32_t k = 0; k < n; k++) rank[code[k].second] = k;
+    return rank;
+}
 
 int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, int32_t n_nodes, mp_ctx** out) {
     if (!out) return set_err(nullptr, MP_ERR_ARG, "mp_create: out is null");
@@ -416,6 +486,12 @@ int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, i
         hipMemset(c->den_out, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||
         hipMemset(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters) != hipSuccess)
         return bail("mp_create: upload failed");
+    {
+        const std::vector<int32_t> rank = morton_tet_rank(tets, n_tets);
+        if (dalloc(&c->tet_rank, n_tets) != hipSuccess ||
+            hipMemcpy(c->tet_rank, rank.data(), sizeof(int32_t) * rank.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail("mp_create: tet order upload failed");
+    }
     *out = c;
     return MP_OK;
 }
@@ -428,7 +504,7 @@ void mp_destroy(mp_ctx* c) {
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
                     c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps, c->perm, c->ef_stage,
-                    c->live_cnt};
+                    c->live_cnt, c->tet_rank};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     mpstage::stager_free(c->stage);
@@ -800,6 +876,15 @@ int mp_get_slots(mp_ctx* c, int32_t* cell, int32_t* pid, int64_t* n_inout) {
     const size_t bytes = sizeof(int32_t) * (size_t)c->n_slots;
     if (cell && bytes) HIP_TRY(c, hipMemcpyAsync(cell, s.cell, bytes, hipMemcpyDeviceToHost, c->stream));
     if (pid && bytes) HIP_TRY(c, hipMemcpyAsync(pid, s.pid, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_get_tet_order(mp_ctx* c, int32_t* rank) {
+    if (!c || !rank) return MP_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(rank, c->tet_rank, sizeof(int32_t) * (size_t)c->n_tets, hipMemcpyDeviceToHost,
+                              c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MP_OK;
 }

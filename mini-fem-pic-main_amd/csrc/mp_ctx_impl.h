@@ -36,6 +36,7 @@ struct mp_ctx {
     int32_t* perm = nullptr;  // gather sort: source slot of each sorted slot
     int64_t perm_n = 0;
     double* ef_stage = nullptr;  // the field's H2D staging (3 * n_tets), kept between uploads
+    int32_t* tet_rank = nullptr;  // Morton rank of each tet's centroid (sort_order 1)
     unsigned long long* live_cnt = nullptr;  // mp_num_active's device count
     // counting-sort and scan scratch
     int32_t* cell_cnt = nullptr;

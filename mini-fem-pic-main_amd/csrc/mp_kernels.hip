@@ -1079,8 +1079,14 @@ __device__ __forceinline__ int sort_slot(SortSlots& t, int32_t key) {
 
 // Bucket keys of a thread's kSortItems slots (-1: lost or past n); every load is issued
 // before the first key is formed.
+// The sort position of cell c: c itself, or its rank along a space-filling curve (trank).
+__device__ __forceinline__ int32_t sort_pos(const int32_t* __restrict__ trank, int32_t c) {
+    return trank ? trank[c] : c;
+}
+
 __device__ __forceinline__ void tile_keys(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run,
-                                          int64_t n, int shift, int32_t nb, int64_t i0, int32_t* key) {
+                                          const int32_t* __restrict__ trank, int64_t n, int shift, int32_t nb,
+                                          int64_t i0, int32_t* key) {
     int32_t c[kSortItems], r[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
@@ -1089,7 +1095,7 @@ __device__ __forceinline__ void tile_keys(const int32_t* __restrict__ cell, cons
         r[j] = (run && i < n) ? (int32_t)run[i] : 0;
     }
 #pragma unroll
-    for (int j = 0; j < kSortItems; j++) key[j] = c[j] >= 0 ? r[j] * nb + (c[j] >> shift) : -1;
+    for (int j = 0; j < kSortItems; j++) key[j] = c[j] >= 0 ? r[j] * nb + (sort_pos(trank, c[j]) >> shift) : -1;
 }
 
 __device__ __forceinline__ void sort_slots_clear(SortSlots& t) {
@@ -1135,8 +1141,8 @@ constexpr int kHistThreads = 1024;
 static_assert(kSortHash >= 2 * kHistThreads, "the LDS hash of a workgroup stays at most half full");
 
 __global__ void __launch_bounds__(kHistThreads)
-k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift,
-            int32_t nb, int32_t* __restrict__ cnt) {
+k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, const int32_t* __restrict__ trank,
+            int64_t n, int shift, int32_t nb, int32_t* __restrict__ cnt) {
     __shared__ SortSlots t;
     for (int q = threadIdx.x; q < kSortHash; q += kHistThreads) {
         t.key[q] = -1;
@@ -1146,7 +1152,7 @@ k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, 
     const int32_t c = (i < n) ? cell[i] : -1;
     const int32_t r = (run && i < n) ? (int32_t)run[i] : 0;
     __syncthreads();
-    if (c >= 0) atomicAdd(&t.cnt[sort_slot(t, r * nb + (c >> shift))], 1);
+    if (c >= 0) atomicAdd(&t.cnt[sort_slot(t, r * nb + (sort_pos(trank, c) >> shift))], 1);
     __syncthreads();
     for (int q = threadIdx.x; q < kSortHash; q += kHistThreads)
         if (t.key[q] >= 0) atomicAdd(cnt + t.key[q], t.cnt[q]);
@@ -1176,7 +1182,8 @@ __device__ __forceinline__ void stage_field(uint64_t* __restrict__ buf, const in
 // and atomics, so the HBM reads overlap the atomics' round trip instead of following it; they
 // leave through LDS in bucket order (stage_field), field by field.
 __global__ void __launch_bounds__(kSortThreads)
-k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ cursor, Soa dst) {
+k_cell_scatter(Soa src, const int32_t* __restrict__ trank, int64_t n, int shift, int32_t nb,
+               int32_t* __restrict__ cursor, Soa dst) {
     __shared__ SortSlots t;
     __shared__ int32_t gidx[kSortTile];
     __shared__ uint64_t buf[2][kSortTile];
@@ -1206,7 +1213,8 @@ k_cell_scatter(Soa src, int64_t n, int shift, int32_t nb, int32_t* __restrict__ 
     int slot[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
-        const int32_t key = c[j] >= 0 ? (src.run ? (int32_t)run[j] * nb : 0) + (c[j] >> shift) : -1;
+        const int32_t key =
+            c[j] >= 0 ? (src.run ? (int32_t)run[j] * nb : 0) + (sort_pos(trank, c[j]) >> shift) : -1;
         sort_take(t, key, slot[j], rank[j]);
     }
     __syncthreads();
@@ -1276,12 +1284,12 @@ __device__ __forceinline__ void sort_claim(SortSlots& t, int32_t* __restrict__ c
 // The rank pass of a gather sort: k_cell_scatter's bucket ranges, but only the particle's
 // source slot is written (perm[d] = i).  The launch that consumes perm moves the records.
 __global__ void __launch_bounds__(kSortThreads)
-k_cell_rank(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, int64_t n, int shift, int32_t nb,
-            int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+k_cell_rank(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, const int32_t* __restrict__ trank,
+            int64_t n, int shift, int32_t nb, int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
     __shared__ SortSlots t;
     const int64_t i0 = xcd_block() * kSortTile + threadIdx.x;
     int32_t key[kSortItems];
-    tile_keys(cell, run, n, shift, nb, i0, key);
+    tile_keys(cell, run, trank, n, shift, nb, i0, key);
     sort_slots_clear(t);
     __syncthreads();
     int slot[kSortItems], rank[kSortItems];
@@ -1778,26 +1786,26 @@ static unsigned sort_grid(int64_t n) {
 }
 
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cnt, hipStream_t st) {
+                            int32_t* cnt, hipStream_t st, const int32_t* trank) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, kHistThreads)), dim3(kHistThreads), 0, st, cell, run, n, shift,
-                       nb, cnt);
+    hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, kHistThreads)), dim3(kHistThreads), 0, st, cell, run, trank, n,
+                       shift, nb, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
-                               const Soa& dst, hipStream_t st) {
+                               const Soa& dst, hipStream_t st, const int32_t* trank) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, src, n, shift,
+    hipLaunchKernelGGL(k_cell_scatter, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, src, trank, n, shift,
                        nb, cursor, dst);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cursor, int32_t* perm, hipStream_t st) {
+                            int32_t* cursor, int32_t* perm, hipStream_t st, const int32_t* trank) {
     if (n <= 0) return hipSuccess;
     if (n > (int64_t)INT32_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cell_rank, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, cell, run, n, shift,
+    hipLaunchKernelGGL(k_cell_rank, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, cell, run, trank, n, shift,
                        nb, cursor, perm);
     return hipGetLastError();
 }

@@ -83,15 +83,16 @@ hipError_t launch_count_live(const int32_t* cell, int64_t n, unsigned long long*
 hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool by_pid, const Soa& dst,
                            hipStream_t st);
 hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st);
-// Counting sort by key = run * nb + (cell >> shift) (run term only when the Soa has runs).
+// Counting sort by key = run * nb + (pos(cell) >> shift) (run term only when the Soa has runs),
+// pos(c) = trank[c] when a tet order is given, else c.
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cnt, hipStream_t st);
+                            int32_t* cnt, hipStream_t st, const int32_t* trank = nullptr);
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
-                               const Soa& dst, hipStream_t st);
+                               const Soa& dst, hipStream_t st, const int32_t* trank = nullptr);
 // The rank pass alone: perm[d] = source slot of the particle that belongs at slot d (cursor =
 // bucket offsets, advanced); lost particles get no slot.  n < 2^31.
 hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cursor, int32_t* perm, hipStream_t st);
+                            int32_t* cursor, int32_t* perm, hipStream_t st, const int32_t* trank = nullptr);
 hipError_t launch_flag_aos(const mp_particle* p, int64_t n, int32_t* flag, hipStream_t st);
 hipError_t launch_check_cells_aos(const mp_particle* p, int64_t n, int32_t n_tets, int32_t* first_bad,
                                   hipStream_t st);

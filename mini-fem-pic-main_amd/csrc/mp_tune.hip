@@ -182,12 +182,14 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
         sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 2});
         sweep([](mp_options& o, int v) { o.gather_sort = v; }, {0, 1});
+        sweep([](mp_options& o, int v) { o.sort_order = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 12, 16, 20, 24, 32});
     } else {
         sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
         if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
+        sweep([](mp_options& o, int v) { o.sort_order = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.defer_walk = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
         sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4, 8});

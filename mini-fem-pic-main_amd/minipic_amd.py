@@ -49,7 +49,7 @@ class MpOptions(C.Structure):
                 ("threads_per_particle", C.c_int32), ("particles_per_thread", C.c_int32),
                 ("walk_cap", C.c_int32), ("scatter_agg", C.c_int32), ("sort_shift", C.c_int32),
                 ("stage_tets", C.c_int32), ("keep_pid_order", C.c_int32), ("compact_crossers", C.c_int32),
-                ("defer_walk", C.c_int32), ("gather_sort", C.c_int32)]
+                ("defer_walk", C.c_int32), ("gather_sort", C.c_int32), ("sort_order", C.c_int32)]
 
 
 class MpTuneTrial(C.Structure):
@@ -90,6 +90,7 @@ SIGNATURES = {
     "mp_sync": (C.c_int, [_P]),
     "mp_num_active": (C.c_int, [_P, C.POINTER(_I64)]),
     "mp_get_slots": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.POINTER(_I64)]),
+    "mp_get_tet_order": (C.c_int, [_P, C.c_void_p]),
     "mp_get_particles": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(_I64)]),
     "mp_get_density": (C.c_int, [_P, _P]),
     "mp_get_counters": (C.c_int, [_P, C.POINTER(MpCounters)]),
@@ -549,6 +550,12 @@ class Mover:
         cell, pid = np.empty(n.value, np.int32), np.empty(n.value, np.int32)
         self._check(lib().mp_get_slots(self._h, _ptr(cell), _ptr(pid), C.byref(n)))
         return cell, pid
+
+    def tet_order(self, n_tets):
+        """Rank of every tet along the Morton curve of sort_order 1 (int32, a permutation)."""
+        rank = np.empty(n_tets, np.int32)
+        self._check(lib().mp_get_tet_order(self._h, _ptr(rank)))
+        return rank
 
     def get_particles(self):
         """Particles still in the domain, ascending pid: (pos, vel, cell, pid)."""

@@ -695,14 +695,15 @@ def test_node_hash_overflow_unsorted_large_mesh(mp, orc, opts):
         assert np.array_equal(gc, run.cell) and gp.tobytes() == run.pos.tobytes()
 
 
+@pytest.mark.parametrize("order", [0, 1], ids=["cell", "morton"])
 @pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "deterministic"])
-def test_batch_step_many_matches_per_run_oracle(mp, orc, mode):
+def test_batch_step_many_matches_per_run_oracle(mp, orc, mode, order):
     """mp_batch_step_many: every run's density at every step, its particles and the counters
     against its own serial MoveParticles; sort every 3 steps, 7 steps (launches of 3, 3, 1)."""
     mesh, ef, runs = _ensemble(mp)
     oracles = [orc.Run(mesh.tets, ef, mesh.node_volume, r["pos"], r["vel"], r["cell"]) for r in runs]
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=3, sort_shift=2)
+        mv.set_options(sort_interval=3, sort_shift=2, sort_order=order)
         mv.upload_field(ef)
         with mp.Batch(mv, len(runs)) as b:
             for k, r in enumerate(runs):
@@ -753,8 +754,9 @@ def test_autotune_multi_step_keeps_state_and_parity(mp, orc):
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
 
 
+@pytest.mark.parametrize("order", [0, 1], ids=["cell", "morton"])
 @pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-def test_step_many_gather_sort(mp, orc, wall):
+def test_step_many_gather_sort(mp, orc, wall, order):
     """The gather sort (k_cell_rank writes only the permutation, the next k_push_steps launch
     moves the records) against the copy sort and the oracle: a sort every 3 steps, particles
     injected between two calls (sorted in before the next launch), every step's density, the
@@ -770,7 +772,7 @@ def test_step_many_gather_sort(mp, orc, wall):
         for gather in (1, 0):
             mv = mp.Mover.from_mesh(mesh)
             movers.append(mv)
-            mv.set_options(sort_interval=3, wall_mode=wall, walk_cap=6, gather_sort=gather)
+            mv.set_options(sort_interval=3, wall_mode=wall, walk_cap=6, gather_sort=gather, sort_order=order)
             assert mv.options["gather_sort"] == gather
             mv.upload_field(ef)
             mv.set_particles(pos[:half], vel[:half], cell[:half])
@@ -810,16 +812,34 @@ def test_step_many_gather_sort(mp, orc, wall):
             mv.close()
 
 
-def check_bucket_order(mv, shift, live_pids):
+def check_bucket_order(mv, shift, live_pids, rank=None):
     cell, pid = mv.slots()
     assert (cell >= 0).all(), "a cell sort drops the lost particles"
-    assert (np.diff(cell >> shift) >= 0).all(), "slots not in bucket order"
+    pos = cell if rank is None else rank[cell]
+    assert (np.diff(pos >> shift) >= 0).all(), "slots not in bucket order"
     assert np.array_equal(np.sort(pid), np.sort(live_pids))
 
 
+def test_tet_order_is_a_compact_curve(mp):
+    """sort_order 1's tet rank: a permutation of the tets, and consecutive ranks are spatial
+    neighbours (a wrong centroid from the barycentric coefficients would scatter them)."""
+    mesh = mp.Mesh.box_mesh(12, 0.1)
+    with mp.Mover.from_mesh(mesh) as mv:
+        rank = mv.tet_order(mesh.n_tets)
+    assert np.array_equal(np.sort(rank), np.arange(mesh.n_tets))
+    cent = mesh.node_pos[mesh.tets["con"]].mean(1)
+    order = np.argsort(rank)
+    step = np.linalg.norm(np.diff(cent[order], axis=0), axis=1)
+    h = 0.1 / 12
+    # tets of one hex sit 0.35-0.6 h apart; a wrong centroid would give a near-random order
+    # (mean step ~8 h on this box)
+    assert np.median(step) < 1.0 * h and np.mean(step) < 2.0 * h
+
+
+@pytest.mark.parametrize("order", [0, 1], ids=["cell", "morton"])
 @pytest.mark.parametrize("shift", [0, 3])
 @pytest.mark.parametrize("gather", [0, 1], ids=["copy", "gather"])
-def test_cell_sort_orders_buckets(mp, shift, gather):
+def test_cell_sort_orders_buckets(mp, shift, gather, order):
     """The cell sort's result itself (parity holds for any permutation): from the host's random
     order, every live particle once, in non-decreasing cell >> shift; then a few steps later,
     from near-sorted order, with the lost particles dropped.  With the gather sort the sort
@@ -827,13 +847,14 @@ def test_cell_sort_orders_buckets(mp, shift, gather):
     changes cell after it."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=91, n=300001)
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=4, sort_shift=shift, gather_sort=gather)
+        mv.set_options(sort_interval=4, sort_shift=shift, gather_sort=gather, sort_order=order)
+        rank = mv.tet_order(mesh.n_tets) if order else None
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         c0, p0 = mv.slots()
         assert np.array_equal(p0, np.arange(len(pos))) and np.array_equal(c0, cell)
         mv.sort_now()
-        check_bucket_order(mv, shift, np.arange(len(pos)))
+        check_bucket_order(mv, shift, np.arange(len(pos)), rank)
         mv.step_many(QM, 4e-7, SPWT, 0, 3)  # no sort: 3 of the 4 steps of the interval
         live = mv.get_particles()[3]
         assert 0 < len(live) < len(pos)
@@ -841,7 +862,7 @@ def test_cell_sort_orders_buckets(mp, shift, gather):
             mv.step_many(QM, 0.0, SPWT, 0, 2)  # the 4th step, then the sort and one more step
         else:
             mv.sort_now()
-        check_bucket_order(mv, shift, live)
+        check_bucket_order(mv, shift, live, rank)
 
 
 def test_step_many_long_and_empty(mp, orc):
