@@ -208,7 +208,7 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
                         "registers"}
 
 
-def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order=1):
+def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order=0):
     """Contention stress test: a 100^3 six-tet box (side 1 m, 6,000,000 tets, 1,030,301 nodes),
     n particles from a 3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2, seeded centres in
     [0.3, 0.7]^3) so a few thousand nodes take most of the scatter, Maxwellian velocities
@@ -476,7 +476,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=50, help="steps of the adjoint comparison (0: skip)")
     ap.add_argument("--contention-steps", type=int, default=50, help="timed steps of the 200M contention test (0: skip)")
     ap.add_argument("--contention-particles", type=int, default=200_000_000)
-    ap.add_argument("--contention-sort-order", type=int, default=1,
+    ap.add_argument("--contention-sort-order", type=int, default=0,
                     help="contention line: cell sort by cell id (0) or along the Morton curve (1)")
     ap.add_argument("--coupled-steps", type=int, default=200, help="steps of the coupled PIC loop (0: skip)")
     args = ap.parse_args()
