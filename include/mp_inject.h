@@ -41,6 +41,20 @@ int    mp_inject_count(const mp_mesh* m, double plasma_den, double ion_velocity,
 int    mp_inject_ions(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_velocity, double dt,
                       double spwt, double q_over_m, const double* ef, double* rem, mp_particle* out,
                       int64_t cap, int64_t* n_new);
+/* InjectIons with a thermal spread (an extension: the reference injects the cold beam of
+ * src/mini-pic.cpp:245-246).  Everything as mp_inject_ions (same counts, same two position
+ * draws per particle from the same stream), and before the half-step rewind each velocity
+ * component gets v_thermal * z: vel[i] = normal[i]*ion_velocity + v_thermal*z[i].  v_thermal is
+ * the Maxwellian's per-component standard deviation sqrt(kT/m) in m/s.  z[0..2] of the k-th
+ * particle the stream injects (k = 0, 1, ... over the stream's life) are Box-Muller normals
+ * from draws 0..3 of item k of counter-based SplitMix64 (draw j of item k is
+ * mix64(s + (16k + j + 1) * 0x9E3779B97F4A7C15) >> 11, times 2^-53) with
+ * s = mix64(seed ^ 0x7468726D616C0000), seed as given to mp_rng_create; the normals use a
+ * generator of their own, so the positions are the cold beam's for any v_thermal.
+ * v_thermal = 0 injects exactly what mp_inject_ions does. */
+int    mp_inject_ions_thermal(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_velocity,
+                              double v_thermal, double dt, double spwt, double q_over_m, const double* ef,
+                              double* rem, mp_particle* out, int64_t cap, int64_t* n_new);
 
 #ifdef __cplusplus
 }

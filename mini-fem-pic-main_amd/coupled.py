@@ -23,7 +23,9 @@ import minipic_amd as mp
 
 @dataclass
 class Parameters:
-    """Parameters (src/parameters.h:18-38) with the reference's defaults."""
+    """Parameters (src/parameters.h:18-38) with the reference's defaults, plus ion_temperature
+    (kelvin; an extension, key `ion_temperature`): 0 injects the reference's cold beam, T > 0
+    adds a Maxwellian of per-component deviation sqrt(kT/m) to it."""
     plasma_den: float = 1e20
     ion_velocity: float = 7000.0
     electron_temperature: float = 24000.0
@@ -34,6 +36,7 @@ class Parameters:
     invert_normals: bool = False
     plasma_species: str = "Deuterium"
     fesolver_method: int = mp.FE_NONLINEAR
+    ion_temperature: float = 0.0
 
     @classmethod
     def from_file(cls, path):
@@ -49,7 +52,8 @@ class Parameters:
                 if not m:
                     raise ValueError(f"Invalid input line: '{line}'")
                 key, value = m.group(1), m.group(2)
-                if key in ("plasma_den", "ion_velocity", "electron_temperature", "wall_potential", "dt"):
+                if key in ("plasma_den", "ion_velocity", "electron_temperature", "wall_potential", "dt",
+                           "ion_temperature"):
                     setattr(p, key, float(value))
                 elif key == "max_iter":
                     p.max_iter = int(value)
@@ -231,6 +235,7 @@ class CoupledRun:
         self.trace = Trace()
         self.spwt = spwt
         self.q_over_m = charge / mass
+        self.v_thermal = mp.thermal_speed(params.ion_temperature, mass) if params.ion_temperature > 0 else 0.0
         self.fe = mp.FESolver(mesh, n0=params.plasma_den, phi0=0.0, kTe=mp.KB * params.electron_temperature,
                               wall_potential=params.wall_potential, flags=fe_flags)
         self.rng = mp.Rng(seed)
@@ -247,7 +252,7 @@ class CoupledRun:
     def inject(self):
         p = self.params
         new, self.rem = mp.inject_ions(self.mesh, self.rng, p.plasma_den, p.ion_velocity, p.dt, self.spwt,
-                                       self.q_over_m, self.ef, self.rem)
+                                       self.q_over_m, self.ef, self.rem, self.v_thermal)
         if self.resident:
             self.parts = new
         else:

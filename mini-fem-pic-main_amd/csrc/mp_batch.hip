@@ -229,6 +229,9 @@ struct mp_batch {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int64_t graph_slots[2] = {-1, -1};
+    // the options a graph's launch was captured with (the kernel instance and its walk cap)
+    int32_t graph_wall[2] = {-1, -1};
+    int32_t graph_cap[2] = {-1, -1};
     // per-step densities of the last mp_batch_step_many, [steps][R * nn]
     double* den_steps = nullptr;
     int64_t den_steps_n = 0;
@@ -535,7 +538,10 @@ int mp_batch_step(mp_batch* b, int32_t mode) {
     const int64_t nk = (int64_t)b->R * b->nn;
     if (!b->use_graph) return launch_atomic_step(b, b->n_dev, 0, 0, 0, nk);
     const int k = b->cur;
-    if (!b->exec[k] || b->graph_slots[k] < b->n_slots) {
+    // a graph replays the launch as captured: capture again when the slots outgrew its grid or
+    // mp_set_options changed the wall mode or the walk cap since
+    if (!b->exec[k] || b->graph_slots[k] < b->n_slots || b->graph_wall[k] != c->opt.wall_mode ||
+        b->graph_cap[k] != c->opt.walk_cap) {
         if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
         if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
         b->exec[k] = nullptr;
@@ -552,6 +558,8 @@ int mp_batch_step(mp_batch* b, int32_t mode) {
         b->graph[k] = g;
         HIP_TRY(c, hipGraphInstantiate(&b->exec[k], g, nullptr, nullptr, 0));
         b->graph_slots[k] = b->n_slots;
+        b->graph_wall[k] = c->opt.wall_mode;
+        b->graph_cap[k] = c->opt.walk_cap;
     }
     HIP_TRY(c, hipGraphLaunch(b->exec[k], c->stream));
     return MP_OK;

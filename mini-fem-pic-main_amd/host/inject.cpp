@@ -3,14 +3,29 @@
 // (built with -ffp-contract=off).
 #include "mp_inject.h"
 
+#include <cmath>
 #include <random>
 #include <string>
 
+#include "synth_rng.h"
+
+// The reference's stream (mt19937 + uniform_real_distribution<double>) for positions, plus the
+// thermal extension's counter-based normals: the k-th particle this stream injects takes
+// normals 0..2 of item k under the stream's seed (their own generator, so the position draws
+// stay the reference's whatever the temperature).
 struct mp_rng {
     std::mt19937 gen;
     std::uniform_real_distribution<double> dist{0, 1.0};
-    explicit mp_rng(uint32_t seed) : gen(seed) {}
+    uint64_t seed;
+    int64_t injected = 0;
+    explicit mp_rng(uint32_t s) : gen(s), seed(mprng::mix64((uint64_t)s ^ 0x7468726D616C0000ull)) {}
     double next() { return dist(gen); }
+    void thermal(double z[3]) {
+        double z3;
+        mprng::normal2(seed, injected, 0, z[0], z[1]);
+        mprng::normal2(seed, injected, 2, z[2], z3);
+        injected++;
+    }
 };
 
 namespace {
@@ -61,10 +76,11 @@ int mp_inject_count(const mp_mesh* m, double plasma_den, double ion_velocity, do
     return MP_OK;
 }
 
-int mp_inject_ions(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_velocity, double dt, double spwt,
-                   double q_over_m, const double* ef, double* rem, mp_particle* out, int64_t cap,
-                   int64_t* n_new) {
+int mp_inject_ions_thermal(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_velocity,
+                           double v_thermal, double dt, double spwt, double q_over_m, const double* ef,
+                           double* rem, mp_particle* out, int64_t cap, int64_t* n_new) {
     if (!m || !r || !ef || !rem || !n_new) return fail("mp_inject_ions: null argument");
+    if (!(v_thermal >= 0) || std::isinf(v_thermal)) return fail("mp_inject_ions: v_thermal must be finite and >= 0");
     int64_t need = 0;
     int rc = mp_inject_count(m, plasma_den, ion_velocity, dt, spwt, *rem, &need);
     if (rc) return rc;
@@ -90,7 +106,12 @@ int mp_inject_ions(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_ve
             }
             for (int i = 0; i < 3; i++) {
                 part.pos[i] = a * face.u[i] + b * face.v[i] + p0[i];
-                part.vel[i] = face.normal[i] * ion_velocity;  // cold beam
+                part.vel[i] = face.normal[i] * ion_velocity;  // the reference's cold beam
+            }
+            if (v_thermal > 0) {
+                double z[3];
+                r->thermal(z);
+                for (int i = 0; i < 3; i++) part.vel[i] = part.vel[i] + v_thermal * z[i];
             }
             part.cell_index = face.cell_con;
             const double* e = ef + 3 * (size_t)face.cell_con;
@@ -100,6 +121,12 @@ int mp_inject_ions(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_ve
     }
     *n_new = k;
     return MP_OK;
+}
+
+int mp_inject_ions(const mp_mesh* m, mp_rng* r, double plasma_den, double ion_velocity, double dt, double spwt,
+                   double q_over_m, const double* ef, double* rem, mp_particle* out, int64_t cap,
+                   int64_t* n_new) {
+    return mp_inject_ions_thermal(m, r, plasma_den, ion_velocity, 0.0, dt, spwt, q_over_m, ef, rem, out, cap, n_new);
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@
 #include "mp_mesh.h"
 
 #include "fe_geom.h"
+#include "synth_rng.h"
 
 #include <algorithm>
 #include <cmath>
@@ -207,24 +208,9 @@ void compute_tables(mp_mesh& m) {
 // Axis order of tet p inside a hex: path corner(0,0,0) -> +e[o0] -> +e[o1] -> +e[o2].
 const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
-inline uint64_t mix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-inline double draw(uint64_t seed, int64_t p, int j) {
-    uint64_t z = mix64(seed + ((uint64_t)p * 16u + (uint64_t)j + 1u) * 0x9E3779B97F4A7C15ull);
-    return (double)(z >> 11) * (1.0 / 9007199254740992.0);  // [0,1)
-}
-// Box-Muller pair from draws j, j+1 of particle p.
-inline void normal2(uint64_t seed, int64_t p, int j, double& z0, double& z1) {
-    double u1 = 1.0 - draw(seed, p, j);  // (0,1]
-    double u2 = draw(seed, p, j + 1);
-    double r = std::sqrt(-2.0 * std::log(u1));
-    double a = 2.0 * M_PI * u2;
-    z0 = r * std::cos(a);
-    z1 = r * std::sin(a);
-}
+using mprng::draw;
+using mprng::mix64;
+using mprng::normal2;
 
 }  // namespace
 

@@ -147,6 +147,8 @@ SIGNATURES = {
     "mp_rng_uniform": (C.c_int, [_P, _P, _I64]),
     "mp_inject_count": (C.c_int, [_P, _D, _D, _D, _D, _D, C.POINTER(_I64)]),
     "mp_inject_ions": (C.c_int, [_P, _P, _D, _D, _D, _D, _D, _P, C.POINTER(_D), _P, _I64, C.POINTER(_I64)]),
+    "mp_inject_ions_thermal": (C.c_int, [_P, _P, _D, _D, _D, _D, _D, _D, _P, C.POINTER(_D), _P, _I64,
+                                         C.POINTER(_I64)]),
     "mp_mesh_num_inlet_faces": (_I32, [_P]),
     "mp_mesh_inlet_faces": (_P, [_P]),
     "mp_mesh_locate_box": (C.c_int, [_P, _P, _I64, _P]),
@@ -414,8 +416,16 @@ def write_particles_vtp(path, pos):
         raise MpError(rc, lib().mp_output_last_error().decode())
 
 
-def inject_ions(mesh, rng, plasma_den, ion_velocity, dt, spwt, q_over_m, ef, rem):
-    """InjectIons (src/mini-pic.cpp:208-265): (new Particle records, carried fraction)."""
+def thermal_speed(temperature_k, mass):
+    """Per-component standard deviation sqrt(kT/m) (m/s) of a Maxwellian at temperature_k
+    kelvin for particles of mass kg (k = KB eV/K times QE J/eV, the reference's constants)."""
+    return float(np.sqrt(KB * temperature_k * QE / mass))
+
+
+def inject_ions(mesh, rng, plasma_den, ion_velocity, dt, spwt, q_over_m, ef, rem, v_thermal=0.0):
+    """InjectIons (src/mini-pic.cpp:208-265): (new Particle records, carried fraction).
+    v_thermal > 0 adds a Maxwellian spread of that per-component deviation (m/s) to the cold
+    beam (mp_inject_ions_thermal; the reference's own injection is v_thermal = 0)."""
     n = C.c_int64()
     rc = lib().mp_inject_count(mesh._h, plasma_den, ion_velocity, dt, spwt, rem, C.byref(n))
     if rc != MP_OK:
@@ -424,8 +434,8 @@ def inject_ions(mesh, rng, plasma_den, ion_velocity, dt, spwt, q_over_m, ef, rem
     ef = _f64(ef, (-1, 3))
     r = C.c_double(rem)
     got = C.c_int64()
-    rc = lib().mp_inject_ions(mesh._h, rng._h, plasma_den, ion_velocity, dt, spwt, q_over_m, _ptr(ef), C.byref(r),
-                              _ptr(out), len(out), C.byref(got))
+    rc = lib().mp_inject_ions_thermal(mesh._h, rng._h, plasma_den, ion_velocity, v_thermal, dt, spwt, q_over_m,
+                                      _ptr(ef), C.byref(r), _ptr(out), len(out), C.byref(got))
     if rc != MP_OK:
         raise MpError(rc, "mp_inject_ions failed")
     return out[:got.value], float(r.value)

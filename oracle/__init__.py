@@ -182,6 +182,34 @@ class DenseFE:
         return uh, ef
 
 
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix_draws(seed, items, j):
+    """Draw j of each item under seed: mix64(seed + (16 item + j + 1) * 0x9E3779B97F4A7C15) >> 11,
+    times 2^-53 (SplitMix64's output function on a Weyl sequence; uint64 arithmetic wraps)."""
+    items = np.asarray(items, np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (items * np.uint64(16) + np.uint64(j + 1)) * np.uint64(0x9E3779B97F4A7C15)
+        return (_mix64(z) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def thermal_normals(rng_seed, k0, n):
+    """z[n][3] of the injected particles k0 .. k0+n-1 of a stream created with rng_seed: Box-Muller
+    on draws (0, 1) and (2, 3) of item k under s = mix64(rng_seed ^ 0x7468726D616C0000), the
+    third normal being the first of the second pair."""
+    with np.errstate(over="ignore"):
+        s = int(_mix64(np.uint64(rng_seed) ^ np.uint64(0x7468726D616C0000)))
+    k = np.arange(k0, k0 + n, dtype=np.uint64)
+    u = [splitmix_draws(s, k, j) for j in range(4)]
+    r1, a1 = np.sqrt(-2.0 * np.log(1.0 - u[0])), 2.0 * np.pi * u[1]
+    r2, a2 = np.sqrt(-2.0 * np.log(1.0 - u[2])), 2.0 * np.pi * u[3]
+    return np.stack([r1 * np.cos(a1), r1 * np.sin(a1), r2 * np.cos(a2)], axis=1)
+
+
 class AosMover:
     """MoveParticles (src/mini-pic.cpp:268-320) serially on the host: the reference kernel body
     over the Particle records, stable erase of the lost ones, den = raw * (spwt / node volume).

@@ -61,6 +61,65 @@ def test_inject_ions_known_answer(mp, cp, tmp_path):
     assert np.all(parts["pos"][:, 0] == 0.0) and np.all((parts["pos"] >= 0) & (parts["pos"] <= 0.1))
 
 
+def test_thermal_injection_zero_spread_is_cold_beam(mp, cp, tmp_path):
+    """mp_inject_ions_thermal with v_thermal = 0 injects the reference's cold beam bit for bit,
+    and its position draws leave the mt19937 stream where the cold beam leaves it."""
+    mesh = cp.box_case(4, 0.1, str(tmp_path))
+    ef = np.random.default_rng(5).normal(0.0, 50.0, (mesh.n_tets, 3))
+    r1, r2 = mp.Rng(7), mp.Rng(7)
+    a, ra = mp.inject_ions(mesh, r1, 1e10, 7000.0, 5e-7, 1.0, QM, ef, 0.5)
+    b, rb = mp.inject_ions(mesh, r2, 1e10, 7000.0, 5e-7, 1.0, QM, ef, 0.5, v_thermal=0.0)
+    assert len(a) > 1000 and a.tobytes() == b.tobytes() and ra == rb
+    assert r1.uniform(4).tobytes() == r2.uniform(4).tobytes()
+
+
+def test_thermal_injection_maxwellian(mp, orc, cp, tmp_path):
+    """The 1 eV Maxwellian of the coupled workload (sigma = sqrt(kT/m) = 2455.5 m/s for 16 amu):
+    positions, cells and the carried fraction are the cold beam's; the velocity differs from the
+    cold beam's by v_thermal * z with z the oracle's SplitMix64 Box-Muller normals of the k-th
+    injected particle (continuing across calls); and those normals are standard normal: mean,
+    variance, pairwise correlation within 5 standard errors and a Kolmogorov-Smirnov test."""
+    from scipy import stats
+
+    mesh = cp.box_case(12, 0.1, str(tmp_path))
+    vth = mp.thermal_speed(1.0 / mp.KB, 16 * mp.AMU)
+    assert abs(vth - 2455.5) < 0.5
+    ef = np.random.default_rng(6).normal(0.0, 50.0, (mesh.n_tets, 3))
+    plasma_den, v0, dt, spwt = 1e13, 7000.0, 5e-7, 200.0   # ~1.75M ions a call
+    cold_rng, hot_rng = mp.Rng(3), mp.Rng(3)
+    k0 = 0
+    zs = []
+    for call in range(2):
+        cold, rc_ = mp.inject_ions(mesh, cold_rng, plasma_den, v0, dt, spwt, QM, ef, 0.0)
+        hot, rh = mp.inject_ions(mesh, hot_rng, plasma_den, v0, dt, spwt, QM, ef, 0.0, v_thermal=vth)
+        assert len(hot) == len(cold) > 100000 and rh == rc_
+        assert hot["pos"].tobytes() == cold["pos"].tobytes()
+        assert np.array_equal(hot["cell_index"], cold["cell_index"])
+        z = orc.thermal_normals(3, k0, len(hot))
+        k0 += len(hot)
+        np.testing.assert_allclose(hot["vel"] - cold["vel"], vth * z, rtol=0, atol=1e-8)
+        zs.append(z)
+    z = np.concatenate(zs)
+    n = len(z)
+    assert np.all(np.abs(z.mean(axis=0)) < 5 / np.sqrt(n))
+    assert np.all(np.abs(z.var(axis=0) - 1) < 5 * np.sqrt(2 / n))
+    c = np.corrcoef(z.T)
+    assert np.all(np.abs(c[np.triu_indices(3, 1)]) < 5 / np.sqrt(n))
+    assert stats.kstest(z.ravel(), "norm").pvalue > 1e-4
+    # the draws behind the normals are uniform on [0, 1): a plausible indexing slip (two items
+    # sharing draws) would correlate consecutive particles
+    assert abs(np.corrcoef(z[:-1, 0], z[1:, 0])[0, 1]) < 5 / np.sqrt(n)
+
+
+def test_splitmix_draws_match_synthetic_generator(mp, orc):
+    """The oracle's SplitMix64 restatement reproduces the library's seeded synthetic positions
+    (draws 0..2 of particle p scaled into the box) bit for bit."""
+    pos, _ = mp.synth_uniform(12345, 1000, [0, 0, 0], [1, 1, 1], 1.0, [0, 0, 0])
+    p = np.arange(1000)
+    want = np.stack([orc.splitmix_draws(12345, p, j) for j in range(3)], axis=1)
+    assert pos.tobytes() == want.tobytes()
+
+
 def test_fe_stiffness_closed_form(mp, cp, tmp_path):
     """K_PQ = sum_e 8 detJ_e (grad N_a . grad N_b) (2x2x2 unit weights over the element), with the
     gradients taken from the tet tables (grad lc_i = (-beta_i, gamma_i, -delta_i) / 6V) and

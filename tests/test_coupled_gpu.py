@@ -21,10 +21,18 @@ def cp():
     return coupled
 
 
-def _setup(mp, orc, cp, tmp_path, fe_flags):
+ION_T = {"cold-beam": 0.0, "thermal-1eV": 11604.518121745585}  # K; 1 eV = 1 / KB kelvin
+IN_FLIGHT = {"cold-beam": (45000, 52000), "thermal-1eV": (35000, 42000)}  # ions after 200 steps
+
+
+def _params(cp, beam="cold-beam"):
+    return cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                         dt=5e-7, max_iter=200, ion_temperature=ION_T[beam])
+
+
+def _setup(mp, orc, cp, tmp_path, fe_flags, beam="cold-beam"):
     mesh = cp.box_case(12, 0.1, str(tmp_path))
-    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
-                      dt=5e-7, max_iter=200)
+    p = _params(cp, beam)
     gpu = cp.GpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
     run = cp.CoupledRun(mesh, p, move=gpu, fe_flags=fe_flags)
     cpu = orc.AosMover(mesh.tets, mesh.node_volume, QM, p.dt, 2e2)
@@ -46,14 +54,17 @@ def _lockstep(run, cpu, steps, done):
         done.append(ts)
 
 
-def test_coupled_loop_200_steps_bit_exact(mp, orc, cp, tmp_path):
-    """200 steps (about 50k ions in flight once the beam crosses the box), F1 cleared per solve
-    so the Newton iteration converges throughout."""
-    run, gpu, cpu = _setup(mp, orc, cp, tmp_path, mp.FE_RESET_F1)
+@pytest.mark.parametrize("beam", list(ION_T))
+def test_coupled_loop_200_steps_bit_exact(mp, orc, cp, tmp_path, beam):
+    """200 steps (about 50k ions in flight once the cold beam crosses the box, about 38k with the
+    1 eV Maxwellian, which loses ions through the side walls), F1 cleared per solve so the Newton
+    iteration converges throughout."""
+    run, gpu, cpu = _setup(mp, orc, cp, tmp_path, mp.FE_RESET_F1, beam)
     try:
         done = []
         _lockstep(run, cpu, 200, done)
-        assert len(done) == 200 and 45000 < run.n_parts < 52000
+        lo, hi = IN_FLIGHT[beam]
+        assert len(done) == 200 and lo < run.n_parts < hi
     finally:
         gpu.close()
 
@@ -72,13 +83,13 @@ def test_coupled_loop_reference_solver_fails_alike(mp, orc, cp, tmp_path):
         gpu.close()
 
 
-def test_coupled_loop_resident_mover_200_steps(mp, orc, cp, tmp_path):
+@pytest.mark.parametrize("beam", list(ION_T))
+def test_coupled_loop_resident_mover_200_steps(mp, orc, cp, tmp_path, beam):
     """The same loop with the ions resident in HBM (only the injected ions cross PCIe, cell
     sort every 10 steps): density bit-exact every step, and the ions themselves (ascending pid
     = the serial vector's order) bit-exact every 25 steps."""
     mesh = cp.box_case(12, 0.1, str(tmp_path))
-    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
-                      dt=5e-7, max_iter=200)
+    p = _params(cp, beam)
     gpu = cp.ResidentGpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC, sort_interval=10)
     run = cp.CoupledRun(mesh, p, move=gpu, fe_flags=mp.FE_RESET_F1)
     cpu = orc.AosMover(mesh.tets, mesh.node_volume, QM, p.dt, 2e2)
@@ -96,6 +107,7 @@ def test_coupled_loop_resident_mover_200_steps(mp, orc, cp, tmp_path):
                 assert np.array_equal(gc, ref["cell_index"]), f"step {ts}: cells differ"
                 assert gp.tobytes() == ref["pos"].tobytes() and gv.tobytes() == ref["vel"].tobytes()
             run.solve()
-        assert 45000 < run.n_parts < 52000
+        lo, hi = IN_FLIGHT[beam]
+        assert lo < run.n_parts < hi
     finally:
         gpu.close()

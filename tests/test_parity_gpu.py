@@ -434,6 +434,37 @@ def test_batch_matches_per_run_oracle(mp, orc, mode, graph):
             assert c["lost"] == sum(o.stats["lost"] for o in oracles) > 0
 
 
+def test_batch_graph_follows_option_changes(mp, orc):
+    """The batch's captured step replays the launch it captured; an mp_set_options between graph
+    steps (absorbing -> reflecting walls, then a walk cap of 2) must take effect at the next step:
+    every run still equals its own oracle switched alike."""
+    mesh, ef, runs = _ensemble(mp, n_runs=3, seed=37)
+    oracles = [orc.Run(mesh.tets, ef, mesh.node_volume, r["pos"], r["vel"], r["cell"]) for r in runs]
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=0)
+        mv.upload_field(ef)
+        with mp.Batch(mv, len(runs)) as b:
+            b.set_graph(True)
+            for k, r in enumerate(runs):
+                r = dict(r, dt=4 * r["dt"])  # fast enough to reach the walls and to walk several tets
+                runs[k] = r
+                b.set_run(k, r["pos"], r["vel"], r["cell"], r["qm"], r["dt"], r["spwt"])
+            for wall, cap in ((0, 4096), (1, 4096), (1, 2), (0, 4096)):
+                mv.set_options(wall_mode=wall, walk_cap=cap)
+                for o in oracles:
+                    o.wall_mode, o.walk_cap = wall, cap
+                for _ in range(2):
+                    b.step(mp.SCATTER_ATOMIC)
+                    for k, r in enumerate(runs):
+                        check_den(b.density(k), oracles[k].step(r["qm"], r["dt"], r["spwt"]), 0, mp)
+            for k in range(len(runs)):
+                gp, gv, gc, gid = b.get_particles(k)
+                assert np.array_equal(gid, oracles[k].pid) and gp.tobytes() == oracles[k].pos.tobytes()
+            c = b.counters()
+            assert c["reflections"] == sum(o.stats["reflections"] for o in oracles) > 0
+            assert c["cap_hits"] == sum(o.stats["cap_hits"] for o in oracles) > 0
+
+
 def test_batch_sequential_equals_batched(mp):
     mesh, ef, runs = _ensemble(mp, n_runs=4, seed=5)
     out = {}
