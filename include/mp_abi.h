@@ -275,6 +275,21 @@ int  mp_adjoint(mp_ctx* ctx, const double* pos0, const double* vel0, const int32
                 const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,
                 double* dL_ddrift);
 
+/* ---- drop-in step with no allocation (no reference counterpart) ------------------------ */
+/* For callers that cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The MoveParticles step of mp_step_aos, atomic
+ * scatter only, over Particle records that the caller already holds in DEVICE memory:
+ * parts[0..n) are pushed, walked and scattered in place (lc written, cell_index = -1 for the
+ * lost), the survivors are copied in order to kept[0..*n_kept) (device, capacity n, must not
+ * overlap parts), and den_out[n_nodes] (device) = raw * (spwt / node_volume).  The step's
+ * scratch (keep flags, prefix sums, tallies) is one fixed device array per GPU of 32.4 MiB,
+ * held until the process ends; n must not exceed MP_NOALLOC_MAX_PARTICLES (MP_ERR_ARG beyond).
+ * Blocks until done.  Calls that overlap on one GPU share that array and race: serialise
+ * them across host threads and contexts. */
+#define MP_NOALLOC_MAX_PARTICLES (1 << 22)
+int  mp_step_aos_noalloc(mp_ctx* ctx, mp_particle* parts, int64_t n, mp_particle* kept, int64_t* n_kept,
+                         double q_over_m, double dt, double spwt, double* den_out, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

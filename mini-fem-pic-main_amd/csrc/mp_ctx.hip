@@ -480,6 +480,15 @@ int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, i
         return bail("Failed to allocate the mesh on the device");
     if (hipHostMalloc((void**)&c->pinned_i32, 16 * sizeof(int32_t), 0) != hipSuccess)
         return bail("mp_create: pinned allocation failed");
+    {
+        // the fixed scratch of mp_step_aos_noalloc on this device: its address, once per device
+        const hipError_t e = mpna::scratch_setup(dev);
+        if (e != hipSuccess) {
+            mp_destroy(c);
+            return set_err(nullptr, MP_ERR_HIP, std::string("mp_create: no-alloc scratch lookup: ") +
+                                                    hipGetErrorString(e));
+        }
+    }
     if (hipMemcpy(c->tets, h.data(), sizeof(DevTet) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->node_vol, node_volume, sizeof(double) * (size_t)n_nodes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->den_raw, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||

@@ -97,6 +97,12 @@ hipError_t dalloc(T** p, int64_t n) {
 
 }  // namespace mpctx
 
+namespace mpna {
+// Takes the no-alloc step's scratch address on `device` (current), once per device
+// (mp_noalloc.hip); mp_create calls it.
+hipError_t scratch_setup(int device);
+}  // namespace mpna
+
 #define HIP_TRY(ctx, expr)                                                                  \
     do {                                                                                    \
         hipError_t e__ = (expr);                                                            \

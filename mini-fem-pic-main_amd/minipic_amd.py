@@ -26,6 +26,7 @@ SCATTER_DETERMINISTIC = 1
 TUNE_MULTI_STEP = 0x100  # OR into autotune's mode: time candidates through mp_step_many
 WALL_ABSORB = 0
 WALL_REFLECT = 1
+NOALLOC_MAX_PARTICLES = 1 << 22  # MP_NOALLOC_MAX_PARTICLES
 NODE_NORMAL, NODE_OPEN, NODE_INLET, NODE_FIXED = 0, 1, 2, 3
 
 # physical constants of the reference (src/FESolver.h:18-21)
@@ -82,6 +83,7 @@ SIGNATURES = {
     "mp_set_sort_interval": (C.c_int, [_P, _I32]),
     "mp_upload_field": (C.c_int, [_P, _P, _I32]),
     "mp_step_aos": (C.c_int, [_P, _P, C.POINTER(C.c_size_t), _D, _D, _D, _P, _I32]),
+    "mp_step_aos_noalloc": (C.c_int, [_P, _P, _I64, _P, C.POINTER(_I64), _D, _D, _D, _P, _I32]),
     "mp_set_particles": (C.c_int, [_P, _P, _P, _P, _I64]),
     "mp_add_particles": (C.c_int, [_P, _P, _P, _P, _I64]),
     "mp_step": (C.c_int, [_P, _D, _D, _D, _I32]),
@@ -509,6 +511,15 @@ class Mover:
         den = np.empty(self.n_nodes, np.float64)
         self._check(lib().mp_step_aos(self._h, _ptr(parts), C.byref(n), q_over_m, dt, spwt, _ptr(den), mode))
         return int(n.value), den
+
+    def step_aos_noalloc(self, parts_dev, n, kept_dev, q_over_m, dt, spwt, den_dev, mode=SCATTER_ATOMIC):
+        """mp_step_aos_noalloc over device addresses (ints): parts_dev holds n Particle records,
+        kept_dev receives the survivors, den_dev the n_nodes normalised densities.  Returns the
+        number kept."""
+        k = C.c_int64()
+        self._check(lib().mp_step_aos_noalloc(self._h, C.c_void_p(parts_dev), n, C.c_void_p(kept_dev), C.byref(k),
+                                              q_over_m, dt, spwt, C.c_void_p(den_dev), mode))
+        return int(k.value)
 
     def move_particles(self, parts, q_over_m, dt, spwt, mode=SCATTER_ATOMIC):
         """MoveParticles semantics on a numpy Particle array: returns (kept particles, den)."""
