@@ -69,6 +69,18 @@ def measured_traffic(name="k_push_soa_box60_20M_traffic.txt", key="traffic_bytes
     return None
 
 
+def traffic_fields(traffic, launch_s, profile):
+    """roofline fields from the committed profile's measured HBM bytes per launch: the bytes and
+    the HBM bandwidth they mean at this run's launch time, as a fraction of the peak."""
+    if traffic is None:
+        return {"traffic": None, "traffic_source": None, "measured_gbs": None, "measured_frac": None}
+    gbs = traffic / launch_s / 1e9
+    return {"traffic": traffic,
+            "traffic_source": f"profiles/{profile}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, "
+                              "calibrated on a copy kernel of known size (not measured in this run)",
+            "measured_gbs": gbs, "measured_frac": gbs / HBM_PEAK_GBS}
+
+
 def push_bytes(active_particle_steps, launches, n_tets, n_nodes):
     return 104.0 * active_particle_steps + launches * (192.0 * n_tets + 16.0 * n_nodes)
 
@@ -109,14 +121,13 @@ def run_gpu_many(mesh, ef, pos, vel, cell, steps, warmup, options, tune=True):
 
 
 def run_gpu(mesh, ef, pos, vel, cell, steps, warmup, mode, sort_interval, agg=2, shift=0, block=256, ppt=1,
-            tpp=1, tune=False, stage=0, pid_order=0, compact=0, defer=0):
+            tune=False, pid_order=0):
     # Synchronisation goes through the C ABI (hipStreamSynchronize on the context stream,
     # the only stream the mover uses); torch is not loaded: it bundles a second HIP runtime.
     tuned = None
     with mp.Mover.from_mesh(mesh) as mv:
         mv.set_options(sort_interval=sort_interval, scatter_agg=agg, sort_shift=shift, block_size=block,
-                       particles_per_thread=ppt, threads_per_particle=tpp, stage_tets=stage,
-                       keep_pid_order=pid_order, compact_crossers=compact, defer_walk=defer)
+                       particles_per_thread=ppt, keep_pid_order=pid_order)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         if tune:
@@ -452,24 +463,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["atomic", "deterministic"], default="atomic")
     ap.add_argument("--sort-interval", type=int, default=10)
-    ap.add_argument("--many-steps", type=int, default=100,
-                    help="timed steps of the multi-step (mp_step_many) line (0: skip)")
+    ap.add_argument("--many-steps", type=int, default=None,
+                    help="timed steps of the multi-step (mp_step_many) line (default: --steps; 0: skip)")
     ap.add_argument("--headline", choices=["multi", "per-step"], default="multi",
                     help="which atomic-mode line is value/roofline (the other is reported beside it)")
     ap.add_argument("--det-steps", type=int, default=20, help="timed steps of the deterministic-mode line (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--particles", type=int, default=N_PART)
-    ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 none, 1 wave/tet, 2 LDS/node")
+    ap.add_argument("--agg", type=int, default=2, help="atomic scatter aggregation: 0 one atomic per weight, 2 LDS hash per workgroup")
     ap.add_argument("--sort-shift", type=int, default=3, help="cell sort key = cell >> shift")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="timed steps of the batched ensemble (0: skip)")
     ap.add_argument("--block", type=int, default=256, help="push workgroup size (64, 128, 256)")
     ap.add_argument("--ppt", type=int, default=1, help="particles per thread (1, 2, 4)")
-    ap.add_argument("--tpp", type=int, default=1, help="threads per particle (1, 4)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use the options given)")
-    ap.add_argument("--stage", type=int, default=0, help="stage start tets in LDS (1) or gather per lane (0)")
     ap.add_argument("--det-pid-order", type=int, default=0, help="deterministic line: keep pid order (1) or cell sort (0)")
-    ap.add_argument("--compact", type=int, default=0, help="compact crossing particles before the walk (1) or not (0)")
-    ap.add_argument("--defer", type=int, default=0, help="walk crossing particles in a second kernel (1) or not (0)")
     ap.add_argument("--gather-sort", type=int, default=0,
                     help="multi-step line: the sort writes a permutation that the next launch gathers through (1), "
                          "or copies the records (0)")
@@ -480,6 +487,8 @@ def main():
                     help="contention line: cell sort by cell id (0) or along the Morton curve (1)")
     ap.add_argument("--coupled-steps", type=int, default=200, help="steps of the coupled PIC loop (0: skip)")
     args = ap.parse_args()
+    if args.many_steps is None:
+        args.many_steps = args.steps
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the mover runs on a single "
               "MI355X; several GPUs would each take a slab of cell-sort buckets and exchange the particles "
@@ -496,8 +505,7 @@ def main():
 
     mode = mp.SCATTER_ATOMIC if args.mode == "atomic" else mp.SCATTER_DETERMINISTIC
     r = run_gpu(mesh, ef, pos, vel, cell, args.steps, args.warmup, mode, args.sort_interval, args.agg,
-                args.sort_shift, args.block, args.ppt, args.tpp, tune=not args.no_tune, stage=args.stage,
-                compact=args.compact, defer=args.defer)
+                args.sort_shift, args.block, args.ppt, tune=not args.no_tune)
     log(f"gpu {args.mode}: {r['rate'] / 1e9:.3f} G particle-steps/s, push {r['push_ms'] / max(r['launches'], 1):.3f} ms/launch")
     avg_push_s = r["push_ms"] / 1e3 / max(r["launches"], 1)
     bytes_per_launch = push_bytes(r["particle_steps"], r["launches"], mesh.n_tets, mesh.n_nodes) / max(r["launches"], 1)
@@ -507,7 +515,8 @@ def main():
         "path": "mp_step: one MoveParticles step per call (k_push_soa)", "options": r["options"],
         "autotune": r["tuned"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     **traffic_fields(measured_traffic(), avg_push_s, "k_push_soa_box60_20M_traffic.txt"),
                      "kernel": "k_push_soa", "avg_launch_ms": avg_push_s * 1e3,
                      "bytes_per_launch": bytes_per_launch}}
     head, roof, path, head_steps = r, per_step["roofline"], per_step["path"], args.steps
@@ -530,11 +539,13 @@ def main():
             "options": m["options"], "autotune": m["tuned"], "steps_per_launch": m["steps_per_launch"],
             "roofline": {"bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": m_ach / HBM_PEAK_GBS,
-                         "traffic": None if tps is None else tps * m["steps_per_launch"],
+                         **traffic_fields(None if tps is None else tps * m["steps_per_launch"], m_launch_s,
+                                          "k_push_steps_box60_20M_traffic.txt"),
                          "kernel": "k_push_steps", "avg_launch_ms": m_launch_s * 1e3,
                          "bytes_per_launch": m_bytes,
-                         "note": "achieved counts the algorithmic bytes of every step the launch takes (the "
-                                 "state is read and written once per launch, so traffic is well below them)"}}
+                         "note": "achieved and frac count the algorithmic bytes of every step the launch takes "
+                                 "(an effective bandwidth: the state is read and written once per launch); "
+                                 "measured_gbs / measured_frac are the HBM bytes the launch really moves"}}
         if args.headline == "multi":
             head = {"rate": m["value"], "wall_s": m["ms_per_step"] * m["steps"] / 1e3, "counters": m["counters"],
                     "options": m["options"], "tuned": m["tuned"]}

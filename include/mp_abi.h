@@ -65,33 +65,16 @@ typedef struct mp_particle {
 typedef struct mp_options {
     int32_t sort_interval;         /* cell-sort + compact every k steps (0 = never)    */
     int32_t wall_mode;             /* MP_WALL_ABSORB / MP_WALL_REFLECT                 */
-    int32_t block_size;            /* threads per workgroup of the push kernel         */
-    int32_t threads_per_particle;  /* 1 or 4 (4: one lane per barycentric coordinate)  */
+    int32_t block_size;            /* threads per workgroup of the push kernel (64, 128, 256) */
     int32_t particles_per_thread;  /* 1, 2, 4 or 8: rounds of block_size particles per thread */
     int32_t walk_cap;              /* max face moves per particle per step             */
-    int32_t scatter_agg;           /* atomic-mode aggregation: 0 none, 1 per wave by tet,
-                                      2 per workgroup by node (LDS hash), 3 persistent
-                                      pipelined workgroups, LDS hash kept across chunks,
-                                      4 per wave by node (LDS hash, no workgroup barrier) */
+    int32_t scatter_agg;           /* atomic-mode aggregation: 0 one atomic per weight,
+                                      2 per workgroup by node (LDS hash)                */
     int32_t sort_shift;            /* cell sort key = cell >> sort_shift (0: exact cell) */
-    int32_t stage_tets;            /* 1: each workgroup loads its distinct start tets into
-                                      LDS once (TPP=1, PPT=1, aggregation 0 or 2, or the
-                                      deterministic scatter); 0 (default): every lane
-                                      gathers its own */
     int32_t keep_pid_order;        /* 1: the periodic pass is a stable compaction that keeps the
                                       particles in pid order instead of a cell sort; the
                                       deterministic records are then written contiguously by
                                       slot.  0 (default): cell sort */
-    int32_t compact_crossers;      /* 1: each workgroup queues its crossing particles in LDS
-                                      after the first walk iteration and walks them densely
-                                      packed (TPP=1, PPT=1, aggregation 2 or the
-                                      deterministic scatter); 0 (default): every lane walks */
-    int32_t defer_walk;            /* 1: the push kernel finishes only the particles its start
-                                      tet decides; those that cross into a neighbour are queued
-                                      per wave and walked, densely packed, by a second kernel
-                                      (TPP=1, PPT=1, no staging or compaction, aggregation
-                                      0, 1, 2, 4 or the deterministic scatter); 0 (default):
-                                      every lane walks in the push kernel */
     int32_t gather_sort;           /* mp_step_many with the cell sort: 1 the sort's rank pass
                                       writes only a permutation and the next multi-step
                                       launch reads each particle through it into the other
@@ -159,7 +142,7 @@ int  mp_step(mp_ctx* ctx, double q_over_m, double dt, double spwt, int32_t mode)
 /* n_steps consecutive mp_step calls under the field last uploaded (no reference counterpart:
  * for runs whose field stays frozen).  Atomic mode pushes every stretch of steps between two
  * periodic sorts in one launch, the particles kept in registers from step to step, so the
- * state crosses HBM once per launch; aggregation 0, 2 or 4 (1 and 3 run as 2), 256-thread
+ * state crosses HBM once per launch; aggregation 0 or 2, 256-thread
  * workgroups, one particle per thread.  Deterministic mode runs the steps one by one.
  * Particles, counters and sorts are those of n_steps mp_step calls; the normalised density
  * of every step is kept for mp_get_density_step, and the last one for mp_get_density. */
@@ -205,8 +188,8 @@ typedef struct mp_tune_trial {
     mp_options opt;
     double     ms_per_step;
 } mp_tune_trial;
-/* Choose threads_per_particle, particles_per_thread, block_size, scatter_agg (atomic mode),
- * sort_shift and sort_interval for the resident particles and the given step parameters:
+/* Choose particles_per_thread, block_size, scatter_agg (atomic mode), sort_shift,
+ * sort_order, sort_interval (and keep_pid_order in deterministic mode) for the resident particles and the given step parameters:
  * coordinate descent over the knobs, each candidate timed (HIP events) over a window of
  * max(interval, min_steps) steps that starts from the same saved state and holds one cell
  * sort per sort interval, as a long run does; a candidate replaces the incumbent only when

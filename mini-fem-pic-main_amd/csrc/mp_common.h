@@ -90,53 +90,10 @@ __device__ __forceinline__ void tally(int32_t* __restrict__ part, bool act, bool
     }
 }
 
-// Per-wave step tallies: lane 0 stores kTally int32 to part[global wave * kTally + k]; no
-// workgroup barrier.
-__device__ __forceinline__ void tally_wave(int32_t* __restrict__ part, int a, int l, int x, int m, int k,
-                                           int r) {
-    const int v[kTally] = {wave_sum(a), wave_sum(l), wave_sum(x), wave_sum(m), wave_sum(k), wave_sum(r)};
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (lane_id() == 0)
-#pragma unroll
-        for (int q = 0; q < kTally; q++) part[w * kTally + q] = v[q];
-}
-
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
-}
-
-// Atomic scatter with the wave's contributions grouped by final tet: one leader per distinct
-// tet adds the group's summed weights to its 4 nodes.  Cell-sorted particles put ~15 per
-// tet, so this turns 256 same-line atomics per wave into a few groups of 4.  Must be called
-// by every lane of the wave (the loop is wave-uniform).
-__device__ __forceinline__ void scatter_grouped(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                                bool ok, int32_t fc, double l0, double l1, double l2,
-                                                double l3) {
-    unsigned long long pending = __ballot(ok);
-    const int lane = lane_id();
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int32_t gc = __shfl(fc, leader, kWave);
-        const bool mine = ok && fc == gc;
-        const unsigned long long m = __ballot(mine);
-        pending &= ~m;
-        double a0 = l0, a1 = l1, a2 = l2, a3 = l3;
-        if (__popcll(m) > 1) {
-            a0 = wave_sum_f64(mine ? l0 : 0.0);
-            a1 = wave_sum_f64(mine ? l1 : 0.0);
-            a2 = wave_sum_f64(mine ? l2 : 0.0);
-            a3 = wave_sum_f64(mine ? l3 : 0.0);
-        }
-        if (lane == leader) {
-            const DevTet* t = tets + gc;
-            unsafeAtomicAdd(den + t->con[0], a0);
-            unsafeAtomicAdd(den + t->con[1], a1);
-            unsafeAtomicAdd(den + t->con[2], a2);
-            unsafeAtomicAdd(den + t->con[3], a3);
-        }
-    }
 }
 
 // Atomic scatter aggregated per workgroup by node: the block's (node, weight) pairs are summed
@@ -207,76 +164,6 @@ __device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, dou
     scatter_lds_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
 }
 
-// Atomic scatter aggregated per wave by node: like scatter_lds, in a wave-private LDS table of
-// 256 slots (>= the wave's 4 * 64 contributions, so every insert finds a slot), with no
-// workgroup barrier: LDS operations of one wave complete in issue order, and the wave
-// barriers keep the compiler from moving table accesses across the phases.  Every lane of the
-// wave must call it.
-constexpr int kWaveSlots = 256;
-
-__device__ __forceinline__ void lds_insert_wave(int32_t* hk, double* hv, int32_t node, double w) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 8);
-    for (;;) {
-        const int32_t prev = atomicCAS(&hk[h], -1, node);
-        if (prev == -1 || prev == node) {
-            atomicAdd(&hv[h], w);
-            return;
-        }
-        h = (h + 1) & (kWaveSlots - 1);
-    }
-}
-
-__device__ __forceinline__ void scatter_lds_wave_nodes(double* __restrict__ den, bool ok, int32_t n0, int32_t n1,
-                                                       int32_t n2, int32_t n3, double l0, double l1, double l2,
-                                                       double l3) {
-    __shared__ int32_t hk_all[256 / kWave][kWaveSlots];
-    __shared__ double hv_all[256 / kWave][kWaveSlots];
-    const int w = threadIdx.x / kWave;
-    int32_t* hk = hk_all[w];
-    double* hv = hv_all[w];
-    const int lane = lane_id();
-#pragma unroll
-    for (int q = 0; q < kWaveSlots / kWave; q++) {
-        hk[lane + q * kWave] = -1;
-        hv[lane + q * kWave] = 0.0;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (ok) {
-        lds_insert_wave(hk, hv, n0, l0);
-        lds_insert_wave(hk, hv, n1, l1);
-        lds_insert_wave(hk, hv, n2, l2);
-        lds_insert_wave(hk, hv, n3, l3);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int q = 0; q < kWaveSlots / kWave; q++) {
-        const int32_t node = hk[lane + q * kWave];
-        if (node >= 0) unsafeAtomicAdd(den + node, hv[lane + q * kWave]);
-    }
-    // the next call's initialisation must not overtake this flush's reads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void scatter_lds_wave(const DevTet* __restrict__ tets, double* __restrict__ den,
-                                                 bool ok, int32_t fc, double l0, double l1, double l2,
-                                                 double l3) {
-    int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-    if (ok) {
-        const DevTet* t = tets + fc;
-        n0 = t->con[0];
-        n1 = t->con[1];
-        n2 = t->con[2];
-        n3 = t->con[3];
-    }
-    scatter_lds_wave_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
-}
-
 // Deterministic records: particle pid writes its 4 (node, weight) pairs at 4*pid + v, so the
 // record array is in pid order; slots of absent pids hold the sentinel node 0xFFFFFFFF
 // (pre-filled).  A stable sort on the node bits then leaves each node's weights in
@@ -303,18 +190,14 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
 }
 
 // The atomic scatter a kernel's template arguments select (nothing when DET: the records are
-// written where each walk ends): AGG 0 one atomic per weight, 1 grouped per wave by tet,
-// 2 LDS hash per workgroup (every thread of the block calls it), 4 LDS hash per wave.
+// written where each walk ends): AGG 0 one atomic per weight, 2 LDS hash per workgroup (every
+// thread of the block calls it).
 template <int DET, int AGG>
 __device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, double* __restrict__ den, bool ok,
                                             int32_t fc, double l0, double l1, double l2, double l3) {
     if (DET) return;
-    if (AGG == 4) {
-        scatter_lds_wave(tets, den, ok, fc, l0, l1, l2, l3);
-    } else if (AGG == 2) {
+    if (AGG == 2) {
         scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
-    } else if (AGG == 1) {
-        scatter_grouped(tets, den, ok, fc, l0, l1, l2, l3);
     } else if (ok) {
         const DevTet* t = tets + fc;
         unsafeAtomicAdd(den + t->con[0], l0);

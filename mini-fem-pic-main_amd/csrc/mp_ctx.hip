@@ -101,15 +101,11 @@ mp_options default_options() {
     o.sort_interval = 10;
     o.wall_mode = MP_WALL_ABSORB;
     o.block_size = 256;
-    o.threads_per_particle = 1;
     o.particles_per_thread = 1;
     o.walk_cap = 4096;
     o.scatter_agg = 2;
     o.sort_shift = 0;
-    o.stage_tets = 0;
     o.keep_pid_order = 0;
-    o.compact_crossers = 0;
-    o.defer_walk = 0;
     o.gather_sort = 0;
     o.sort_order = 0;
     return o;
@@ -160,24 +156,6 @@ int reserve_tally(mp_ctx* c, int64_t n, int block) {
     const int64_t nc = need + need / 2;
     if (dalloc(&c->tally, nc) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "tally buffer allocation failed");
     c->tally_n = nc;
-    return MP_OK;
-}
-
-int reserve_defer_queue(mp_ctx* c, int64_t n, int block) {
-    int64_t entries = 0, waves = 0;
-    mpd::defer_queue_size(std::max<int64_t>(n, 1), block, &entries, &waves);
-    if (entries <= c->wq_n && waves <= c->wq_waves) return MP_OK;
-    dfree(c->wq);
-    dfree(c->wq_cnt);
-    c->wq = nullptr;
-    c->wq_cnt = nullptr;
-    c->wq_n = c->wq_waves = 0;
-    entries += entries / 8;
-    waves += waves / 8;
-    if (dalloc(&c->wq, entries) != hipSuccess || dalloc(&c->wq_cnt, waves) != hipSuccess)
-        return set_err(c, MP_ERR_ALLOC, "walk queue allocation failed");
-    c->wq_n = entries;
-    c->wq_waves = waves;
     return MP_OK;
 }
 
@@ -347,13 +325,11 @@ void gather_done(mp_ctx* c, int64_t n_live) {
 bool good_options(const mp_options& o) {
     return o.sort_interval >= 0 && (o.wall_mode == MP_WALL_ABSORB || o.wall_mode == MP_WALL_REFLECT) &&
            (o.block_size == 64 || o.block_size == 128 || o.block_size == 256) &&
-           (o.threads_per_particle == 1 || (o.threads_per_particle == 4 && o.particles_per_thread == 1)) &&
            (o.particles_per_thread == 1 || o.particles_per_thread == 2 || o.particles_per_thread == 4 ||
             o.particles_per_thread == 8) &&
-           o.walk_cap >= 1 && o.scatter_agg >= 0 && o.scatter_agg <= 4 && o.sort_shift >= 0 &&
-           o.sort_shift <= 16 && (o.stage_tets == 0 || o.stage_tets == 1) &&
+           o.walk_cap >= 1 && (o.scatter_agg == 0 || o.scatter_agg == 2) && o.sort_shift >= 0 &&
+           o.sort_shift <= 16 &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
-           (o.compact_crossers == 0 || o.compact_crossers == 1) && (o.defer_walk == 0 || o.defer_walk == 1) &&
            (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1);
 }
 
@@ -512,7 +488,7 @@ void mp_destroy(mp_ctx* c) {
     void* bufs[] = {c->tets, c->node_vol, c->den_raw, c->den_out, c->ctr, c->soa_mem[0], c->soa_mem[1],
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
-                    c->inj_tmp, c->cmp_buf, c->wq, c->wq_cnt, c->den_steps, c->perm, c->ef_stage,
+                    c->inj_tmp, c->cmp_buf, c->den_steps, c->perm, c->ef_stage,
                     c->live_cnt, c->tet_rank};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
@@ -713,8 +689,6 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     {
         int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
         if (rc) return rc;
-        if (c->opt.defer_walk) rc = reserve_defer_queue(c, c->n_slots, c->opt.block_size);
-        if (rc) return rc;
     }
     mpd::PushArgs a;
     a.tets = c->tets;
@@ -733,13 +707,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
     a.ppt = c->opt.particles_per_thread;
-    a.tpp = c->opt.threads_per_particle;
-    a.stage = c->opt.stage_tets != 0;
     a.rec_by_slot = c->pid_ordered;
-    a.compact = c->opt.compact_crossers != 0;
-    a.defer = c->opt.defer_walk != 0;
-    a.wq = c->wq;
-    a.wq_cnt = c->wq_cnt;
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));
@@ -821,10 +789,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
         a.agg = c->opt.scatter_agg;
         a.block = 256;
         a.ppt = 1;
-        a.tpp = 1;
-        a.stage = false;
         a.rec_by_slot = false;
-        a.compact = false;
         if (c->timing) HIP_TRY(c, record_event(c));
         HIP_TRY(c, mpd::launch_push_steps(a, c->n_nodes, chunk, c->stream));
         if (c->timing) HIP_TRY(c, record_event(c));

@@ -65,10 +65,6 @@ struct mp_ctx {
     double* den_steps = nullptr;
     int64_t den_steps_n = 0;
     int32_t den_steps_k = 0;
-    // deferred walk: per-wave queues of crossing particles and their counts
-    uint32_t* wq = nullptr;
-    int32_t* wq_cnt = nullptr;
-    int64_t wq_n = 0, wq_waves = 0;
     // per-block tallies of the push kernels
     int32_t* tally = nullptr;
     int64_t tally_n = 0;

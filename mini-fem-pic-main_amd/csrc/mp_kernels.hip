@@ -39,34 +39,26 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
     }
 }
 
-// AGG: 0 = one atomic per weight, 1 = grouped per wave by tet, 2 = LDS hash per block,
-// 4 = LDS hash per wave (no workgroup barrier anywhere: a wave whose particles walk far does
-// not hold up the other waves of its block).  Each thread takes ppt particles, one per round,
-// a round being blockDim.x consecutive particles; MULTI == false fixes ppt = 1 at compile time
-// (the round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).
 #ifndef MP_PUSH_WAVES
 #define MP_PUSH_WAVES 5
 #endif
-//
-// DEFER (ppt 1 only): a particle whose push takes it into a neighbour tet is not walked here.
-// It is stored pushed, with that neighbour as its cell, and listed for k_walk_deferred in its
-// wave's queue region: wq[wave slot * 64 + rank] = slot | (moves so far << 31), the count in
-// wq_cnt[wave slot] (wave slot = blockIdx.x * waves per block + wave; plain stores, no
-// atomics).  The waves here then stream without waiting on walk round trips.
-template <int WALL, int DET, int AGG, bool MULTI, bool DEFER = false>
+
+// Push + XtoLtet walk + scatter of the resident particles, one step.  AGG: 0 = one atomic per
+// weight, 2 = LDS hash per workgroup.  Each thread takes ppt particles, one per round, a round
+// being blockDim.x consecutive particles; MULTI == false fixes ppt = 1 at compile time (the
+// round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).
+template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot,
-           uint32_t* __restrict__ wq = nullptr, int32_t* __restrict__ wq_cnt = nullptr) {
-    static_assert(!(DEFER && MULTI), "the deferred walk takes one particle per thread");
+           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot) {
     const int ppt = MULTI ? ppt_arg : 1;
     const int64_t b0 = xcd_block() * ppt;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
     for (int k = 0; k < ppt; k++) {
         const int64_t i = (b0 + k) * blockDim.x + threadIdx.x;
-        bool ok = false, defer = false;
-        int32_t fc = -1, dmoves = 0;
+        bool ok = false;
+        int32_t fc = -1;
         double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
         if (i < n) {
             // The particle's state is loaded with its cell, not after it: only the start tet
@@ -92,21 +84,8 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             double y = y0 + vy * dt;
             double z = z0 + vz * dt;
             WalkStats st{0, 0};
-            int r;
-            if (DEFER) {
-                r = first_step<WALL>(g0, c, x, y, z, l0, l1, l2, l3, live ? cap : 0, st);
-                defer = live && r == kContinue;
-                dmoves = st.moves;
-            } else {
-                r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
-            }
-            if (DEFER && defer) {
-                // the walk's counts are taken where it finishes
-                n_act++;
-                s.cell[i] = c;
-                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-            } else if (live) {
+            const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+            if (live) {
                 n_act++;
                 n_cross += (st.moves - st.refl) > 0;
                 n_moves += st.moves;
@@ -125,21 +104,9 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
         }
-        if (DEFER) {
-            const unsigned long long m = __ballot(defer);
-            const int64_t ws = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-            if (defer) {
-                const int rank = __popcll(m & ((1ull << lane_id()) - 1ull));
-                wq[ws * kWave + rank] = (uint32_t)i | ((uint32_t)dmoves << 31);
-            }
-            if (lane_id() == 0) wq_cnt[ws] = __popcll(m);
-        }
         scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
-    if (AGG == 4 && !DET)
-        tally_wave(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
-    else
-        tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
 // ---- several steps per launch under a frozen field ------------------------------------------
@@ -228,482 +195,6 @@ __global__ void k_normalise_steps(double* __restrict__ den, const double* __rest
     den[i] = den[i] * (spwt / nvol[i % nn]);
 }
 
-// ---- deferred walk ------------------------------------------------------------------------
-// Walks the particles k_push_soa<..., DEFER> listed, in full waves.  Workgroup b takes 64
-// wave slots of the push grid: those of push workgroups h = (k * G + t) * 8 + xcd, t < G,
-// G = 64 / waves per push workgroup, k = b / 8, xcd = b % 8.  Workgroups are dispatched to the
-// XCDs round-robin, so the walk runs on the XCD that pushed the particles and whose L2 holds
-// their start region's tets.  Lanes of the first wave prefix-sum the 64 counts into LDS; each
-// thread then finds its entry's slot by binary search over the sums.  Cells, positions and
-// velocities (reflecting walls) are written back, the scatter is the push kernel's.
-template <int WALL, int DET, int AGG>
-__global__ void __launch_bounds__(256)
-k_walk_deferred(const DevTet* __restrict__ tets, Soa s, const uint32_t* __restrict__ wq,
-                const int32_t* __restrict__ wq_cnt, int64_t n_waves, int wpb, double* __restrict__ den,
-                uint32_t* __restrict__ keys, double* __restrict__ vals, int32_t* __restrict__ part, int cap,
-                bool rec_by_slot) {
-    __shared__ int32_t pre[kWave + 1];
-    const int64_t xcd = blockIdx.x % kXcds, grp = blockIdx.x / kXcds;
-    const int G = kWave / wpb;
-    if (threadIdx.x < kWave) {
-        const int l = threadIdx.x;
-        const int64_t ws = ((grp * G + l / wpb) * kXcds + xcd) * wpb + l % wpb;
-        int v = ws < n_waves ? wq_cnt[ws] : 0;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int y = __shfl_up(v, o, kWave);
-            if (l >= o) v += y;
-        }
-        pre[l + 1] = v;
-        if (l == 0) pre[0] = 0;
-    }
-    __syncthreads();
-    const int total = pre[kWave];
-    int n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
-    for (int b = 0; b < total; b += blockDim.x) {  // uniform over the workgroup
-        const int j = b + (int)threadIdx.x;
-        bool ok = false;
-        int32_t fc = -1;
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-        if (j < total) {
-            int lo = 0, hi = kWave;  // pre[lo] <= j < pre[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (pre[mid] <= j) lo = mid;
-                else hi = mid;
-            }
-            const int64_t ws = ((grp * G + lo / wpb) * kXcds + xcd) * wpb + lo % wpb;
-            const uint32_t e = wq[ws * kWave + (j - pre[lo])];
-            const int64_t i = (int64_t)(e & 0x7FFFFFFFu);
-            WalkStats st{(int)(e >> 31), 0};
-            int32_t c = s.cell[i];
-            double x = ld(s.x + i), y = ld(s.y + i), z = ld(s.z + i);
-            double vx = 0.0, vy = 0.0, vz = 0.0;
-            if (WALL) {
-                vx = ld(s.vx + i);
-                vy = ld(s.vy + i);
-                vz = ld(s.vz + i);
-            }
-            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
-            n_cross += (st.moves - st.refl) > 0;
-            n_moves += st.moves;
-            n_refl += st.refl;
-            if (r == kInside) {
-                s.cell[i] = c;
-                ok = true;
-                fc = c;
-                if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tets + c, l0, l1, l2, l3);
-            } else {
-                s.cell[i] = -1;
-                n_lost++;
-                n_cap += (r == kCapped);
-            }
-            if (WALL) {
-                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-            }
-        }
-        scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
-    }
-    tally_counts(part, 0, n_lost, n_cross, n_moves, n_cap, n_refl);
-}
-
-// ---- start tets staged in LDS -----------------------------------------------------------
-// The push is bound by the vector-memory address/data path, not by HBM bytes: every lane
-// gathers its start tet's 192 B record with 12 16-byte loads, and after the cell sort ~15
-// lanes of a workgroup ask for the same record.  Here the workgroup first collects its
-// distinct start cells in an LDS hash (kStageSlots entries), loads each record once with
-// consecutive lanes reading consecutive 16-byte chunks, and the particles read their start
-// tet from LDS.  Cells that find no free entry (more than kStageSlots distinct cells in the
-// workgroup) read it from global memory as before; walk steps into neighbours always do.
-// The particle state loads are issued before the staging barriers so both round trips
-// overlap.  AGG as in k_push_soa (0, 2 or 4).
-constexpr int kStageSlots = 64;
-constexpr int kTetChunks = (int)(sizeof(DevTet) / 16);
-
-template <int WALL, int DET, int AGG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
-k_push_staged(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
-              double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-              int32_t* __restrict__ part, int cap, bool rec_by_slot) {
-    __shared__ DevTet stage[kStageSlots];
-    __shared__ int32_t skey[kStageSlots];
-    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    for (int q = threadIdx.x; q < kStageSlots; q += blockDim.x) skey[q] = -1;
-    const int32_t c0 = i < n ? s.cell[i] : -1;
-    double vx = 0.0, vy = 0.0, vz = 0.0, x = 0.0, y = 0.0, z = 0.0;
-    if (c0 >= 0) {
-        vx = ld(s.vx + i); vy = ld(s.vy + i); vz = ld(s.vz + i);
-        x = ld(s.x + i); y = ld(s.y + i); z = ld(s.z + i);
-    }
-    __syncthreads();
-    int slot = -1;
-    if (c0 >= 0) {
-        uint32_t h = ((uint32_t)c0 * 0x9E3779B1u) >> (32 - 6);
-        for (int p = 0; p < kStageSlots; p++) {
-            const int32_t prev = atomicCAS(&skey[h], -1, c0);
-            if (prev == -1 || prev == c0) {
-                slot = (int)h;
-                break;
-            }
-            h = (h + 1) & (kStageSlots - 1);
-        }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < kStageSlots * kTetChunks; q += blockDim.x) {
-        const int sl = q / kTetChunks, k = q - sl * kTetChunks;
-        const int32_t key = skey[sl];
-        if (key >= 0) reinterpret_cast<uint4*>(stage + sl)[k] = reinterpret_cast<const uint4*>(tets + key)[k];
-    }
-    __syncthreads();
-
-    bool act = false, lost = false, crossed = false, capped = false, ok = false;
-    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-    int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-    WalkStats st{0, 0};
-    if (c0 >= 0) {
-        act = true;
-        StartTet g0;
-        double ex, ey, ez;
-        if (slot >= 0) {
-            const DevTet* t = stage + slot;
-            load_start(t, g0);
-            ex = t->ef[0]; ey = t->ef[1]; ez = t->ef[2];
-        } else {
-            const DevTet* t = tets + c0;
-            load_start(t, g0);
-            ex = t->ef[0]; ey = t->ef[1]; ez = t->ef[2];
-        }
-        vx = vx + (qm * ex) * dt;
-        vy = vy + (qm * ey) * dt;
-        vz = vz + (qm * ez) * dt;
-        x = x + vx * dt;
-        y = y + vy * dt;
-        z = z + vz * dt;
-        int32_t c = c0;
-        const int r = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
-        crossed = (st.moves - st.refl) > 0;
-        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-        if (r == kInside) {
-            s.cell[i] = c;
-            ok = true;
-            if (c == c0 && slot >= 0) {
-                const DevTet* t = stage + slot;
-                n0 = t->con[0]; n1 = t->con[1]; n2 = t->con[2]; n3 = t->con[3];
-            } else {
-                const DevTet* t = tets + c;
-                n0 = t->con[0]; n1 = t->con[1]; n2 = t->con[2]; n3 = t->con[3];
-            }
-            if (DET)
-                put_records_nodes(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], (uint32_t)n0, (uint32_t)n1,
-                                  (uint32_t)n2, (uint32_t)n3, l0, l1, l2, l3);
-        } else {
-            s.cell[i] = -1;
-            lost = true;
-            capped = (r == kCapped);
-        }
-    }
-    if (!DET) {
-        if (AGG == 2) {
-            scatter_lds_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
-        } else if (ok) {
-            unsafeAtomicAdd(den + n0, l0);
-            unsafeAtomicAdd(den + n1, l1);
-            unsafeAtomicAdd(den + n2, l2);
-            unsafeAtomicAdd(den + n3, l3);
-        }
-    }
-    tally(part, act, lost, crossed, st.moves, capped, st.refl);
-}
-
-// ---- crossing particles compacted before the walk ---------------------------------------
-// After the first walk iteration most lanes are done (~70% of particles stay in their tet);
-// the rest would walk in every wave of the workgroup with most lanes idle.  Here the waves
-// append their crossing particles (state, next tet, moves so far) to an LDS queue with one
-// ballot and one LDS atomic per wave, and after a barrier the queue is walked by the first
-// lanes of the workgroup, densely packed: ~2 waves walk instead of 4.  A crossing particle
-// that finds the queue full walks in place.  Scatter and counters then cover both the
-// particles finished in place and the queued ones.  AGG 2 (LDS hash) or the deterministic
-// records.
-constexpr int kQueue = 128;
-
-template <int WALL, int DET>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
-k_push_compact(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
-               double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-               int32_t* __restrict__ part, int cap, bool rec_by_slot) {
-    __shared__ double qx[kQueue], qy[kQueue], qz[kQueue], qvx[kQueue], qvy[kQueue], qvz[kQueue];
-    __shared__ int32_t qc[kQueue], qi[kQueue], qmv[kQueue];
-    __shared__ int32_t qn;
-    if (threadIdx.x == 0) qn = 0;
-    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
-    bool ok1 = false, ok2 = false;
-    int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
-    bool live = false, need = false;
-    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-    int32_t c = 0;
-    WalkStats st{0, 0};
-    int r = kInside;
-    if (i < n) {
-        const int32_t c_in = s.cell[i];
-        vx = ld(s.vx + i); vy = ld(s.vy + i); vz = ld(s.vz + i);
-        const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
-        __builtin_amdgcn_sched_barrier(0);
-        live = c_in >= 0;
-        c = live ? c_in : 0;
-        const DevTet* t = tets + c;
-        StartTet g0;
-        load_start(t, g0);
-        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-        vx = vx + (qm * ex) * dt;
-        vy = vy + (qm * ey) * dt;
-        vz = vz + (qm * ez) * dt;
-        x = x0 + vx * dt;
-        y = y0 + vy * dt;
-        z = z0 + vz * dt;
-        r = first_step<WALL>(g0, c, x, y, z, l0, l1, l2, l3, live ? cap : 0, st);
-        need = live && r == kContinue;
-    }
-    __syncthreads();  // qn is cleared
-    // append the crossing particles: one LDS atomic per wave
-    const unsigned long long mask = __ballot(need);
-    int slot = -1;
-    if (mask) {
-        const int lane = lane_id();
-        const int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&qn, __popcll(mask));
-        base = __shfl(base, leader, kWave);
-        if (need) slot = base + __popcll(mask & ((1ull << lane) - 1));
-    }
-    if (need && slot < kQueue) {
-        qx[slot] = x; qy[slot] = y; qz[slot] = z;
-        qvx[slot] = vx; qvy[slot] = vy; qvz[slot] = vz;
-        qc[slot] = c;
-        qi[slot] = (int32_t)threadIdx.x;
-        qmv[slot] = st.moves;
-    } else if (need) {
-        r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);  // queue full: walk here
-    }
-    const bool queued = need && slot < kQueue;
-    if (live && !queued) {
-        n_act++;
-        n_moves += st.moves;
-        n_refl += st.refl;
-        n_cross += (st.moves - st.refl) > 0;
-        if (r == kInside) {
-            s.cell[i] = c;
-            ok1 = true;
-            const DevTet* tf = tets + c;
-            a0 = tf->con[0]; a1 = tf->con[1]; a2 = tf->con[2]; a3 = tf->con[3];
-            if (DET) put_records_nodes(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], (uint32_t)a0, (uint32_t)a1,
-                                       (uint32_t)a2, (uint32_t)a3, l0, l1, l2, l3);
-        } else {
-            s.cell[i] = -1;
-            n_lost++;
-            n_cap += r == kCapped;
-        }
-        st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-        st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-    }
-    __syncthreads();  // the queue is complete
-    const int nq = qn < kQueue ? qn : kQueue;
-    if ((int)threadIdx.x < nq) {
-        const int q = threadIdx.x;
-        const int64_t ip = xcd_block() * blockDim.x + qi[q];  // the queued particle's slot
-        double px = qx[q], py = qy[q], pz = qz[q], pvx = qvx[q], pvy = qvy[q], pvz = qvz[q];
-        int32_t pc = qc[q];
-        WalkStats pst{qmv[q], 0};
-        const int pr = walk<WALL>(tets, pc, px, py, pz, pvx, pvy, pvz, m0, m1, m2, m3, cap, pst);
-        n_act++;
-        n_moves += pst.moves;
-        n_refl += pst.refl;
-        n_cross += (pst.moves - pst.refl) > 0;
-        if (pr == kInside) {
-            s.cell[ip] = pc;
-            ok2 = true;
-            const DevTet* tf = tets + pc;
-            b0 = tf->con[0]; b1 = tf->con[1]; b2 = tf->con[2]; b3 = tf->con[3];
-            if (DET) put_records_nodes(keys, vals, rec_by_slot ? ip : (int64_t)s.pid[ip], (uint32_t)b0, (uint32_t)b1,
-                                       (uint32_t)b2, (uint32_t)b3, m0, m1, m2, m3);
-        } else {
-            s.cell[ip] = -1;
-            n_lost++;
-            n_cap += pr == kCapped;
-        }
-        st_(s.x + ip, px); st_(s.y + ip, py); st_(s.z + ip, pz);
-        st_(s.vx + ip, pvx); st_(s.vy + ip, pvy); st_(s.vz + ip, pvz);
-    }
-    if (!DET) {
-        __shared__ int32_t hk[kNodeSlots];
-        __shared__ double hv[kNodeSlots];
-        for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
-            hk[q] = -1;
-            hv[q] = 0.0;
-        }
-        __syncthreads();
-        if (ok1) {
-            lds_insert(hk, hv, den, a0, l0);
-            lds_insert(hk, hv, den, a1, l1);
-            lds_insert(hk, hv, den, a2, l2);
-            lds_insert(hk, hv, den, a3, l3);
-        }
-        if (ok2) {
-            lds_insert(hk, hv, den, b0, m0);
-            lds_insert(hk, hv, den, b1, m1);
-            lds_insert(hk, hv, den, b2, m2);
-            lds_insert(hk, hv, den, b3, m3);
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
-            const int32_t node = hk[q];
-            if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
-        }
-    }
-    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
-}
-
-// ---- four threads per particle ------------------------------------------------------------
-// Lane v (0..3) of a quad owns vertex v of the current tet: it gathers that vertex's 32 B row
-// (alpha, beta, gamma, delta) instead of all 128 B, computes lc_v, and the quad exchanges the
-// four weights through DPP quad broadcasts, so every lane takes the same inside/argmin decision
-// on bit-identical values.  The push and the particle state are replicated across the quad;
-// each lane stores a share of the state and scatters its own vertex.  Control flow is uniform
-// within a quad.  AGG: 0 = one atomic per weight, otherwise the per-workgroup LDS hash.
-template <int K>
-__device__ __forceinline__ double quad_bcast(double v) {
-    constexpr int ctrl = K * 0x55;  // quad_perm [K, K, K, K]
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ void lds_scatter_one(double* __restrict__ den, bool ok, int32_t node, double w) {
-    __shared__ int32_t hk[kNodeSlots];
-    __shared__ double hv[kNodeSlots];
-    for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
-        hk[q] = -1;
-        hv[q] = 0.0;
-    }
-    __syncthreads();
-    if (ok) lds_insert(hk, hv, den, node, w);
-    __syncthreads();
-    for (int q = threadIdx.x; q < kNodeSlots; q += blockDim.x) {
-        const int32_t nd = hk[q];
-        if (nd >= 0) unsafeAtomicAdd(den + nd, hv[q]);
-    }
-}
-
-template <int WALL, int DET, int AGG>
-__global__ void __launch_bounds__(256)
-k_push_tpp4(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
-            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-            int32_t* __restrict__ part, int cap, bool rec_by_slot) {
-    const int sub = threadIdx.x & 3;
-    const int64_t i = (xcd_block() * blockDim.x + threadIdx.x) >> 2;
-    bool act = false, lost = false, crossed = false, capped = false, ok = false;
-    int32_t node = -1;
-    double lw = 0.0;
-    int moves = 0, refl = 0;
-    if (i < n) {
-        int32_t c = s.cell[i];
-        if (c >= 0) {
-            act = true;
-            const DevTet* t = tets + c;
-            double a = t->c[sub][0], b = t->c[sub][1], g = t->c[sub][2], d = t->c[sub][3];
-            double vol = t->vol;
-            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-            double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
-            double x = ld(s.x + i), y = ld(s.y + i), z = ld(s.z + i);
-            vx = vx + (qm * ex) * dt;
-            vy = vy + (qm * ey) * dt;
-            vz = vz + (qm * ez) * dt;
-            x = x + vx * dt;
-            y = y + vy * dt;
-            z = z + vz * dt;
-            int status;
-            while (true) {
-                const double l = bary(a, b, g, d, x, y, z, vol);
-                const double l0 = quad_bcast<0>(l), l1 = quad_bcast<1>(l), l2 = quad_bcast<2>(l),
-                             l3 = quad_bcast<3>(l);
-                const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
-                                 (l3 < 0 || l3 > 1.0);
-                if (!out) {
-                    lw = l;
-                    status = kInside;
-                    break;
-                }
-                int mi = 0;
-                double ml = l0;
-                if (l1 < ml) { ml = l1; mi = 1; }
-                if (l2 < ml) { ml = l2; mi = 2; }
-                if (l3 < ml) { ml = l3; mi = 3; }
-                const int32_t nb = t->nbr[mi];
-                if (moves >= cap) {
-                    status = kCapped;
-                    break;
-                }
-                moves++;
-                if (nb >= 0) {
-                    c = nb;
-                } else if (WALL == 0) {
-                    status = kLost;
-                    break;
-                } else {
-                    const double gx = -t->c[mi][1], gy = t->c[mi][2], gz = -t->c[mi][3];
-                    const double gg = (gx * gx + gy * gy) + gz * gz;
-                    const double sc = ((2.0 * ml) * (6.0 * vol)) / gg;
-                    x = x - sc * gx;
-                    y = y - sc * gy;
-                    z = z - sc * gz;
-                    const double w = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
-                    vx = vx - w * gx;
-                    vy = vy - w * gy;
-                    vz = vz - w * gz;
-                    refl++;
-                }
-                t = tets + c;
-                a = t->c[sub][0];
-                b = t->c[sub][1];
-                g = t->c[sub][2];
-                d = t->c[sub][3];
-                vol = t->vol;
-            }
-            crossed = (moves - refl) > 0;
-            // each lane stores a share of the particle's state
-            if (sub == 0) { st_(s.x + i, x); st_(s.vx + i, vx); }
-            if (sub == 1) { st_(s.y + i, y); st_(s.vy + i, vy); }
-            if (sub == 2) { st_(s.z + i, z); st_(s.vz + i, vz); }
-            if (status == kInside) {
-                if (sub == 3) s.cell[i] = c;
-                ok = true;
-                node = t->con[sub];
-                if (DET) {
-                    const int64_t r = 4 * (rec_by_slot ? i : (int64_t)s.pid[i]) + sub;
-                    keys[r] = (uint32_t)node;
-                    vals[r] = lw;
-                }
-            } else {
-                if (sub == 3) s.cell[i] = -1;
-                lost = true;
-                capped = status == kCapped;
-            }
-        }
-    }
-    if (!DET) {
-        if (AGG == 0) {
-            if (ok) unsafeAtomicAdd(den + node, lw);
-        } else {
-            lds_scatter_one(den, ok, node, lw);
-        }
-    }
-    const bool q0 = sub == 0;  // one lane per particle counts
-    tally(part, act && q0, lost && q0, crossed && q0, q0 ? moves : 0, capped && q0, q0 ? refl : 0);
-}
-
 // Drop-in kernel over the reference's AoS Particle (88 B: pos@0 vel@24 lc@48 cell@80).
 // Absorbing walls, lc written back as the reference kernel does; pid = array index.
 template <int DET>
@@ -749,133 +240,6 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
         }
     }
     tally(part, act, lost, crossed, st.moves, capped, st.refl);
-}
-
-// Persistent, software-pipelined push (atomic scatter).  Each workgroup walks one contiguous
-// XCD-local range of the particle array 256 particles at a time and loads the next chunk's
-// cell and state while the current chunk is pushed and walked.  Order matters on CDNA: the
-// vector-memory counter retires loads in issue order, so a wait for the current chunk's start
-// tet also waits for every load issued before it.  The start-tet loads are therefore issued
-// first and the next chunk's loads right after them (scheduling barriers keep that order;
-// the prefetch address is clamped in range rather than branched around, so no control flow
-// merges two different wait states).  The LDS hash (2048 slots) accumulates over
-// kPFlushEvery chunks between flushes, so most chunks pass no workgroup barrier; an insert
-// that finds no slot within kPProbe probes goes straight to a global atomic.
-constexpr int kPHash = 2048;
-constexpr int kPFlushEvery = 4;
-constexpr int kPProbe = 32;
-
-__device__ __forceinline__ void lds_insert_or_global(int32_t* hk, double* hv, double* __restrict__ den,
-                                                     int32_t node, double w) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - 11);
-    for (int p = 0; p < kPProbe; p++) {
-        const int32_t prev = atomicCAS(&hk[h], -1, node);
-        if (prev == -1 || prev == node) {
-            atomicAdd(&hv[h], w);
-            return;
-        }
-        h = (h + 1) & (kPHash - 1);
-    }
-    unsafeAtomicAdd(den + node, w);
-}
-
-template <int WALL>
-__global__ void __launch_bounds__(256)
-k_push_persist(const DevTet* __restrict__ tets, Soa s, int64_t n, int64_t chunk, double qm, double dt,
-               double* __restrict__ den, int32_t* __restrict__ part, int cap) {
-    __shared__ int32_t hk[kPHash];
-    __shared__ double hv[kPHash];
-    for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
-        hk[q] = -1;
-        hv[q] = 0.0;
-    }
-    __syncthreads();
-    const int64_t beg = xcd_block() * chunk;
-    const int64_t end = beg + chunk < n ? beg + chunk : n;
-    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
-    // the first chunk
-    int32_t c = -1;
-    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0;
-    if (beg < end) {
-        const int64_t i = beg + threadIdx.x;
-        const int64_t ii = i < end ? i : beg;
-        const int32_t ci = s.cell[ii];
-        x0 = ld(s.x + ii); y0 = ld(s.y + ii); z0 = ld(s.z + ii);
-        vx0 = ld(s.vx + ii); vy0 = ld(s.vy + ii); vz0 = ld(s.vz + ii);
-        c = i < end ? ci : -1;
-    }
-    int k = 0;
-    for (int64_t base = beg; base < end; base += blockDim.x, k++) {
-        const int64_t i = base + threadIdx.x;
-        const bool live = i < end && c >= 0;
-        const DevTet* t = tets + (live ? c : 0);
-        StartTet g0;
-        load_start(t, g0);
-        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-        // (compiler-only fences: no instruction, they keep the two load groups in this order)
-        __asm__ volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-        // the next chunk, issued behind the start tet
-        const int64_t j = i + blockDim.x;
-        const int64_t jj = j < end ? j : beg;
-        const int32_t cj = s.cell[jj];
-        const double xn = ld(s.x + jj), yn = ld(s.y + jj), zn = ld(s.z + jj);
-        const double vxn = ld(s.vx + jj), vyn = ld(s.vy + jj), vzn = ld(s.vz + jj);
-        __asm__ volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-        double vx = vx0 + (qm * ex) * dt;
-        double vy = vy0 + (qm * ey) * dt;
-        double vz = vz0 + (qm * ez) * dt;
-        double x = x0 + vx * dt;
-        double y = y0 + vy * dt;
-        double z = z0 + vz * dt;
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-        WalkStats st{0, 0};
-        int32_t cw = live ? c : 0;
-        const int r = walk_start<WALL>(tets, g0, cw, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
-        if (live) {
-            n_act++;
-            n_moves += st.moves;
-            n_refl += st.refl;
-            n_cross += (st.moves - st.refl) > 0;
-            if (r == kInside) {
-                s.cell[i] = cw;
-                const DevTet* tf = tets + cw;
-                const int32_t n0 = tf->con[0], n1 = tf->con[1], n2 = tf->con[2], n3 = tf->con[3];
-                lds_insert_or_global(hk, hv, den, n0, l0);
-                lds_insert_or_global(hk, hv, den, n1, l1);
-                lds_insert_or_global(hk, hv, den, n2, l2);
-                lds_insert_or_global(hk, hv, den, n3, l3);
-            } else {
-                s.cell[i] = -1;
-                n_lost++;
-                n_cap += r == kCapped;
-            }
-            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-        }
-        c = j < end ? cj : -1;
-        x0 = xn; y0 = yn; z0 = zn;
-        vx0 = vxn; vy0 = vyn; vz0 = vzn;
-        if ((k + 1) % kPFlushEvery == 0) {  // workgroup-uniform
-            __syncthreads();
-            for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
-                const int32_t node = hk[q];
-                if (node >= 0) {
-                    unsafeAtomicAdd(den + node, hv[q]);
-                    hk[q] = -1;
-                    hv[q] = 0.0;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < kPHash; q += blockDim.x) {
-        const int32_t node = hk[q];
-        if (node >= 0) unsafeAtomicAdd(den + node, hv[q]);
-    }
-    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
 __global__ void k_normalise(double* __restrict__ raw, double* __restrict__ out,
@@ -1460,133 +824,6 @@ __global__ void k_pack(Soa s, int64_t n, double* __restrict__ pos, double* __res
 
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Persistent grid: as many 256-thread workgroups as fit on the device at once (multiple of 8
-// for the XCD mapping), each owning a contiguous range of ceil(n / grid) particles.
-static hipError_t launch_push_persist(const PushArgs& a, hipStream_t st) {
-    static int per_cu = 0, cus = 0;
-    if (per_cu == 0) {
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = a.wall == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_persist<0>, 256, 0)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_persist<1>, 256, 0);
-        if (e != hipSuccess) return e;
-        if (per_cu < 1) per_cu = 1;
-    }
-    int64_t g = (int64_t)per_cu * cus;
-    g = std::min<int64_t>(g, (a.n + 255) / 256);
-    g = (g + kXcds - 1) / kXcds * kXcds;
-    int64_t chunk = (a.n + g - 1) / g;
-    chunk = (chunk + 255) / 256 * 256;
-    if (a.wall == 0)
-        hipLaunchKernelGGL((k_push_persist<0>), dim3((unsigned)g), dim3(256), 0, st, a.tets, a.soa, a.n, chunk, a.qm,
-                           a.dt, a.den, a.part, a.cap);
-    else
-        hipLaunchKernelGGL((k_push_persist<1>), dim3((unsigned)g), dim3(256), 0, st, a.tets, a.soa, a.n, chunk, a.qm,
-                           a.dt, a.den, a.part, a.cap);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, g, a.ctr, st);
-}
-
-static hipError_t launch_push_compact(const PushArgs& a, hipStream_t st) {
-    const unsigned g = (grid_for(a.n, 256) + kXcds - 1) / kXcds * kXcds;
-#define MP_COMPACT(W, D)                                                                                   \
-    hipLaunchKernelGGL((k_push_compact<W, D>), dim3(g), dim3(256), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, a.den, \
-                       a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
-    if (a.wall == 0) {
-        if (a.det) MP_COMPACT(0, 1);
-        else MP_COMPACT(0, 0);
-    } else {
-        if (a.det) MP_COMPACT(1, 1);
-        else MP_COMPACT(1, 0);
-    }
-#undef MP_COMPACT
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, g, a.ctr, st);
-}
-
-static hipError_t launch_push_staged(const PushArgs& a, hipStream_t st) {
-    const int bs = a.block;
-    const unsigned g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
-#define MP_STAGED(W, D, G)                                                                              \
-    hipLaunchKernelGGL((k_push_staged<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
-                       a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
-    if (a.wall == 0) {
-        if (a.det) MP_STAGED(0, 1, 0);
-        else if (a.agg == 0) MP_STAGED(0, 0, 0);
-        else MP_STAGED(0, 0, 2);
-    } else {
-        if (a.det) MP_STAGED(1, 1, 0);
-        else if (a.agg == 0) MP_STAGED(1, 0, 0);
-        else MP_STAGED(1, 0, 2);
-    }
-#undef MP_STAGED
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, g, a.ctr, st);
-}
-
-static hipError_t launch_push_tpp4(const PushArgs& a, hipStream_t st) {
-    const int bs = a.block;
-    const unsigned g = (grid_for(4 * a.n, bs) + kXcds - 1) / kXcds * kXcds;
-#define MP_TPP4(W, D, G)                                                                            \
-    hipLaunchKernelGGL((k_push_tpp4<W, D, G>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, a.qm, a.dt, \
-                       a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot)
-    if (a.wall == 0) {
-        if (a.det) MP_TPP4(0, 1, 0);
-        else if (a.agg == 0) MP_TPP4(0, 0, 0);
-        else MP_TPP4(0, 0, 2);
-    } else {
-        if (a.det) MP_TPP4(1, 1, 0);
-        else if (a.agg == 0) MP_TPP4(1, 0, 0);
-        else MP_TPP4(1, 0, 2);
-    }
-#undef MP_TPP4
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, g, a.ctr, st);
-}
-
-// Push without the walk beyond the start tet, then k_walk_deferred over the listed crossers.
-static hipError_t launch_push_defer(const PushArgs& a, hipStream_t st) {
-    const int bs = a.block;
-    const int wpb = bs / kWave;
-    const int64_t g = (grid_for(a.n, bs) + kXcds - 1) / kXcds * kXcds;
-    const int64_t n_waves = g * wpb;
-    const int G = kWave / wpb;
-    const int64_t gw = kXcds * ((g / kXcds + G - 1) / G);
-#define MP_DEFER(W, D, A)                                                                                  \
-    do {                                                                                                   \
-        hipLaunchKernelGGL((k_push_soa<W, D, A, false, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, \
-                           a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1, a.rec_by_slot, a.wq, a.wq_cnt); \
-        e = hipGetLastError();                                                                             \
-        if (e != hipSuccess) return e;                                                                     \
-        e = launch_tally_finish(a.part, (!D && A == 4) ? n_waves : g, a.ctr, st);                         \
-        if (e != hipSuccess) return e;                                                                     \
-        hipLaunchKernelGGL((k_walk_deferred<W, D, A>), dim3(gw), dim3(256), 0, st, a.tets, a.soa, a.wq,    \
-                           a.wq_cnt, n_waves, wpb, a.den, a.keys, a.vals, a.part, a.cap, a.rec_by_slot);    \
-    } while (0)
-#define MP_DEFER_W(W)                           \
-    do {                                        \
-        if (a.det) MP_DEFER(W, 1, 0);           \
-        else if (a.agg == 4) MP_DEFER(W, 0, 4); \
-        else if (a.agg == 2) MP_DEFER(W, 0, 2); \
-        else if (a.agg == 1) MP_DEFER(W, 0, 1); \
-        else MP_DEFER(W, 0, 0);                 \
-    } while (0)
-    hipError_t e = hipSuccess;
-    if (a.wall == 0) MP_DEFER_W(0);
-    else MP_DEFER_W(1);
-#undef MP_DEFER_W
-#undef MP_DEFER
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_tally_finish(a.part, gw, a.ctr, st);
-}
-
 int steps_block() { return MP_STEPS_THREADS; }
 
 hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStream_t st) {
@@ -1626,25 +863,10 @@ hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, i
     return hipGetLastError();
 }
 
-// Grid and queue sizes of the deferred walk for n particles in workgroups of `block`: queue
-// entries (64 per wave slot) and wave-slot counts.
-void defer_queue_size(int64_t n, int block, int64_t* entries, int64_t* waves) {
-    const int64_t g = (grid_for(n, block) + kXcds - 1) / kXcds * kXcds;
-    *waves = g * (block / kWave);
-    *entries = *waves * kWave;
-}
-
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-    if (a.tpp == 4) return launch_push_tpp4(a, st);
-    if (a.defer && a.wq && a.wq_cnt && a.ppt <= 1 && !a.stage && !a.compact && (a.det || a.agg != 3) &&
-        a.n < ((int64_t)1 << 31))
-        return launch_push_defer(a, st);
-    if (!a.det && a.agg == 3) return launch_push_persist(a, st);
     const int bs = a.block;
     const int ppt = a.ppt > 0 ? a.ppt : 1;
-    if (a.compact && ppt == 1 && (a.det || a.agg == 2)) return launch_push_compact(a, st);
-    if (a.stage && ppt == 1 && (a.det || a.agg == 0 || a.agg == 2)) return launch_push_staged(a, st);
     const unsigned g = (grid_for(a.n, (int64_t)bs * ppt) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                                  \
     do {                                                                                                  \
@@ -1658,9 +880,7 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
 #define MP_PUSH_W(W)                          \
     do {                                      \
         if (a.det) MP_PUSH(W, 1, 0);          \
-        else if (a.agg == 4) MP_PUSH(W, 0, 4); \
         else if (a.agg == 2) MP_PUSH(W, 0, 2); \
-        else if (a.agg == 1) MP_PUSH(W, 0, 1); \
         else MP_PUSH(W, 0, 0);                \
     } while (0)
     if (a.wall == 0) MP_PUSH_W(0);
@@ -1669,9 +889,7 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
 #undef MP_PUSH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // AGG 4 leaves one tally row per wave, the others one per block
-    const int64_t rows = (!a.det && a.agg == 4) ? (int64_t)g * (bs / kWave) : (int64_t)g;
-    return launch_tally_finish(a.part, rows, a.ctr, st);
+    return launch_tally_finish(a.part, g, a.ctr, st);
 }
 
 hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st) {

@@ -35,13 +35,7 @@ struct PushArgs {
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
-    int tpp;  // threads per particle: 1, or 4 (one lane per tet vertex)
-    bool stage;  // stage each workgroup's distinct start tets in LDS
     bool rec_by_slot;  // deterministic records at 4*slot+v (slots in pid order), else 4*pid+v
-    bool compact;      // compact the crossing particles of each workgroup before the walk
-    bool defer = false;  // walk the crossers in a second kernel (queue wq / wq_cnt, defer_queue_size)
-    uint32_t* wq = nullptr;
-    int32_t* wq_cnt = nullptr;
     // launch_push_steps only: with perm set, particle i is read from slot perm[i] of src and
     // written to slot i of soa (the gather half of a cell sort)
     Soa src{};
@@ -62,7 +56,6 @@ hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, do
 // Step counters: the push kernels leave per-block partials; this folds them into ctr.
 hipError_t launch_tally_finish(const int32_t* part, int64_t nb, unsigned long long* ctr, hipStream_t st);
 int64_t tally_ints(int64_t n, int block);
-void defer_queue_size(int64_t n, int block, int64_t* entries, int64_t* waves);
 hipError_t launch_normalise(double* raw, double* out, const double* nvol, int32_t nn, double spwt,
                             hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);

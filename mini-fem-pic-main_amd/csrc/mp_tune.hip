@@ -125,10 +125,7 @@ struct Trial {
     }
 };
 
-bool valid(const mp_options& o) {
-    if (o.threads_per_particle == 4 && o.particles_per_thread != 1) return false;
-    return true;
-}
+bool valid(const mp_options&) { return true; }
 
 }  // namespace
 
@@ -188,14 +185,10 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
     } else {
         sweep([](mp_options& o, int v) { o.sort_shift = v; }, {0, 1, 2, 3, 4, 5, 6});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
-        if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 1, 2, 3, 4});
+        if (mode == MP_SCATTER_ATOMIC) sweep([](mp_options& o, int v) { o.scatter_agg = v; }, {0, 2});
         sweep([](mp_options& o, int v) { o.sort_order = v; }, {0, 1});
-        sweep([](mp_options& o, int v) { o.defer_walk = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
         sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4, 8});
-        sweep([](mp_options& o, int v) { o.threads_per_particle = v; }, {1, 4});
-        sweep([](mp_options& o, int v) { o.stage_tets = v; }, {0, 1});
-        sweep([](mp_options& o, int v) { o.compact_crossers = v; }, {0, 1});
         if (mode == MP_SCATTER_DETERMINISTIC) sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
     }

@@ -217,10 +217,9 @@ def test_cpp_dropin_program(mp):
     assert r.stdout.count(" ok") == 2
 
 
-@pytest.mark.parametrize("agg", [0, 1, 2, 3, 4], ids=["plain", "wave-by-tet", "lds-by-node", "persistent",
-                                                     "lds-by-node-per-wave"])
+@pytest.mark.parametrize("agg", [0, 2], ids=["plain", "lds-by-node"])
 def test_atomic_aggregation_modes(mp, orc, agg):
-    """Every atomic-scatter aggregation gives the oracle's density to 1e-12 relative."""
+    """Both atomic-scatter aggregations give the oracle's density to 1e-12 relative."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=13, n=80000)
     dt = 4e-7
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
@@ -263,9 +262,9 @@ def test_particles_per_thread_walls_and_cap(mp, orc, wall, ppt, block):
         assert c["cap_hits"] > 0 and c["lost"] > c["cap_hits"] * (1 - wall) and (wall == 0 or c["reflections"] > 0)
 
 
-@pytest.mark.parametrize("ppt,agg,block", [(2, 2, 256), (4, 4, 128), (2, 4, 64), (4, 0, 256)])
+@pytest.mark.parametrize("ppt,agg,block", [(2, 2, 256), (4, 2, 128), (2, 0, 64), (4, 0, 256)])
 def test_particles_per_thread_matches_oracle(mp, orc, ppt, agg, block):
-    """Several particles per thread (and per-wave aggregation, smaller blocks) change only the
+    """Several particles per thread (with smaller blocks) change only the
     schedule: cells, positions, velocities bit-exact, both density modes to their bar."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=15, n=70001)
     dt = 4e-7
@@ -287,18 +286,17 @@ def test_particles_per_thread_matches_oracle(mp, orc, ppt, agg, block):
             assert c[key] == run.stats[key], key
 
 
-@pytest.mark.parametrize("stage", [0, 1], ids=["per-lane-gather", "lds-staged"])
 @pytest.mark.parametrize("sort_interval", [0, 3], ids=["unsorted", "sorted"])
 @pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-def test_start_tet_staging_matches_oracle(mp, orc, stage, sort_interval, wall):
-    """Start tets staged in LDS per workgroup or gathered per lane, on cell-sorted particles
-    and on unsorted ones (more distinct cells per workgroup than the staging table holds, so
-    some lanes take the global fallback): same bits, same counters, both density modes."""
+def test_sorted_and_unsorted_alternating_modes(mp, orc, sort_interval, wall):
+    """Cell-sorted particles (a few start tets per wave) and unsorted ones (nearly one per lane)
+    with a ragged last wave, atomic and deterministic steps alternating: same bits, same
+    counters, both density modes to their bar."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=27 + wall, n=90001)
     dt = 4e-7
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall)
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(stage_tets=stage, sort_interval=sort_interval, wall_mode=wall)
+        mv.set_options(sort_interval=sort_interval, wall_mode=wall)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         for k in range(4):
@@ -311,35 +309,6 @@ def test_start_tet_staging_matches_oracle(mp, orc, stage, sort_interval, wall):
         c = mv.counters()
         for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
             assert c[key] == run.stats[key], key
-
-
-@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-@pytest.mark.parametrize("agg,block", [(0, 256), (2, 64), (2, 256)])
-def test_four_threads_per_particle_matches_oracle(mp, orc, wall, agg, block):
-    """One lane per tet vertex (quad DPP exchange of the barycentrics): same decisions, same
-    bits as the serial walk; atomic and deterministic steps alternate; a walk cap of 3 is hit."""
-    mesh, ef, pos, vel, cell = box_case(mp, seed=19 + wall, n=50003)
-    dt = 9e-7
-    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=3)
-    with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(threads_per_particle=4, scatter_agg=agg, block_size=block, sort_interval=2,
-                       wall_mode=wall, walk_cap=3)
-        mv.upload_field(ef)
-        mv.set_particles(pos, vel, cell)
-        for k in range(4):
-            mode = k % 2
-            mv.step(QM, dt, SPWT, mode)
-            ref = run.step(QM, dt, SPWT)
-            check_den(mv.density(), ref, mode, mp)
-        gp, gv, gc, gid = mv.get_particles()
-        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
-        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
-        c = mv.counters()
-        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
-            assert c[key] == run.stats[key], key
-        assert c["cap_hits"] > 0
-        if wall == 1:
-            assert c["reflections"] > 0
 
 
 @pytest.mark.parametrize("shift", [3, 7])
@@ -503,8 +472,8 @@ def test_autotune_leaves_state_and_keeps_parity(mp, orc):
             assert a.tobytes() == b.tobytes()
         assert before[1].tobytes() == after[1].tobytes() and before[2] == after[2]
         assert mv.options == best
-        assert best["threads_per_particle"] in (1, 4) and best["particles_per_thread"] in (1, 2, 4, 8)
-        assert best["block_size"] in (64, 128, 256) and 0 <= best["scatter_agg"] <= 4
+        assert best["particles_per_thread"] in (1, 2, 4, 8)
+        assert best["block_size"] in (64, 128, 256) and best["scatter_agg"] in (0, 2)
         # the reported time is one of the chosen options' timings, within 1% of the fastest trial
         # or from the final re-check against the starting options
         assert len(trials) > 20 and ms > 0
@@ -568,7 +537,7 @@ def test_error_paths(mp):
         with pytest.raises(mp.MpError):
             mv.set_options(block_size=100)
         with pytest.raises(mp.MpError):
-            mv.set_options(threads_per_particle=4, particles_per_thread=2)
+            mv.set_options(scatter_agg=1)
         with pytest.raises(mp.MpError):
             mv.step(QM, 1e-7, SPWT, 7)
         assert L.mp_set_sort_interval(mv._h, -1) == 1 and b"interval" in L.mp_last_error(mv._h)
@@ -596,60 +565,7 @@ def test_error_paths(mp):
 
 
 @pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-@pytest.mark.parametrize("dt,sort_interval", [(4e-7, 3), (1.5e-6, 0)], ids=["sorted", "fast-unsorted"])
-def test_compact_crossers_matches_oracle(mp, orc, wall, dt, sort_interval):
-    """Crossing particles queued in LDS after the first walk iteration and walked densely
-    packed; the fast unsorted case overfills the queue (the overflow walks in place) and hits
-    the walk cap.  Same bits, same counters, both density modes."""
-    mesh, ef, pos, vel, cell = box_case(mp, seed=37 + wall, n=70001)
-    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=6)
-    with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(compact_crossers=1, sort_interval=sort_interval, wall_mode=wall, walk_cap=6)
-        mv.upload_field(ef)
-        mv.set_particles(pos, vel, cell)
-        for k in range(4):
-            mode = k % 2
-            mv.step(QM, dt, SPWT, mode)
-            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
-        gp, gv, gc, gid = mv.get_particles()
-        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
-        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
-        c = mv.counters()
-        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
-            assert c[key] == run.stats[key], key
-
-
-@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-@pytest.mark.parametrize("agg,block,pid_order", [(2, 256, 0), (0, 64, 1), (4, 128, 0), (1, 256, 1)],
-                         ids=["lds-256", "plain-64-pidorder", "wave-128", "grouped-256-pidorder"])
-@pytest.mark.parametrize("dt,sort_interval", [(4e-7, 3), (1.5e-6, 0)], ids=["sorted", "fast-unsorted"])
-def test_defer_walk_matches_oracle(mp, orc, wall, agg, block, pid_order, dt, sort_interval):
-    """The walk beyond the start tet deferred to a second kernel over per-wave queues, for
-    every aggregation and workgroup size (64, 128, 256 threads: 64, 32, 16 push workgroups per
-    walk workgroup) and both record layouts; the fast unsorted case hits the walk cap.  Same
-    bits, same counters, both density modes."""
-    mesh, ef, pos, vel, cell = box_case(mp, seed=41 + wall, n=70001)
-    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell, wall_mode=wall, walk_cap=6)
-    with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(defer_walk=1, sort_interval=sort_interval, wall_mode=wall, walk_cap=6, scatter_agg=agg,
-                       block_size=block, keep_pid_order=pid_order)
-        mv.upload_field(ef)
-        mv.set_particles(pos, vel, cell)
-        for k in range(4):
-            mode = k % 2
-            mv.step(QM, dt, SPWT, mode)
-            check_den(mv.density(), run.step(QM, dt, SPWT), mode, mp)
-        gp, gv, gc, gid = mv.get_particles()
-        assert np.array_equal(gid, run.pid) and np.array_equal(gc, run.cell)
-        assert gp.tobytes() == run.pos.tobytes() and gv.tobytes() == run.vel.tobytes()
-        c = mv.counters()
-        for key in ("particle_steps", "lost", "crossed", "moves", "cap_hits", "reflections"):
-            assert c[key] == run.stats[key], key
-        assert c["crossed"] > 0 and (wall == 0 or c["reflections"] > 0)
-
-
-@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
-@pytest.mark.parametrize("agg", [2, 0, 4, 1], ids=["lds", "plain", "wave", "grouped-as-lds"])
+@pytest.mark.parametrize("agg", [2, 0], ids=["lds", "plain"])
 @pytest.mark.parametrize("dt,sort_interval,n_steps", [(4e-7, 3, 7), (1.5e-6, 0, 5)],
                          ids=["sorted-3", "fast-unsorted"])
 def test_step_many_matches_oracle(mp, orc, wall, agg, dt, sort_interval, n_steps):
@@ -705,14 +621,12 @@ def test_step_many_deterministic_and_timing(mp, orc):
         assert np.array_equal(gid, run.pid) and gp.tobytes() == run.pos.tobytes()
 
 
-@pytest.mark.parametrize("opts", [dict(scatter_agg=2), dict(compact_crossers=1), dict(stage_tets=1),
-                                  dict(threads_per_particle=4)],
-                         ids=["lds-hash", "compact", "staged", "tpp4"])
+@pytest.mark.parametrize("opts", [dict(scatter_agg=2), dict(scatter_agg=2, particles_per_thread=2)],
+                         ids=["lds-hash", "lds-hash-2ppt"])
 def test_node_hash_overflow_unsorted_large_mesh(mp, orc, opts):
     """Unsorted particles on a 20^3 box (9261 nodes): a workgroup's 256 particles bring up to
-    1024 distinct nodes (1536 with the compaction queue's second set), more than the node hash
-    holds within its probe limit, so contributions spill to direct HBM atomics.  Same bits and
-    densities as the oracle."""
+    1024 distinct nodes, more than the node hash holds within its probe limit, so contributions
+    spill to direct HBM atomics.  Same bits and densities as the oracle."""
     mesh, ef, pos, vel, cell = box_case(mp, n_hex=20, side=0.1, n=40001, seed=61)
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
     with mp.Mover.from_mesh(mesh) as mv:
