@@ -5,8 +5,9 @@ it is the checker, never the thing measured or shipped.
 
   liboracle.so      plain-C serial restatement of the mover (mp_oracle.c) and of the
                     reference's dense FE solver (fe_dense.c), -ffp-contract=off
-  _ref/libref.so    the reference's own src/meshes.cpp, maths.cpp, trace.cpp compiled where
-                    they lie, plus ref_harness.cpp (built only where the reference exists)
+  _ref/libref.so    the reference's own src/meshes.cpp, maths.cpp, trace.cpp, FESolver.cpp and
+                    particles.cpp compiled where they lie, plus ref_harness.cpp (built only
+                    where the reference exists)
 """
 from __future__ import annotations
 
@@ -53,6 +54,12 @@ _REF_SIG = {
     "ref_rnd": (None, [_I64, _P]),
     "ref_det3": (_D, [_P]),
     "ref_det4": (_D, [_P]),
+    "ref_fe_create": (C.c_int, [_D, _D, _D, _D]),
+    "ref_fe_compute_phi": (C.c_int, [_P, C.c_int]),
+    "ref_fe_lapack_system": (C.c_int, [_P, _P, _P]),
+    "ref_fe_get": (None, [_P, _P]),
+    "ref_output_mesh": (None, [C.c_int, _P, _P, _P]),
+    "ref_output_particles": (None, [_P, _I64]),
 }
 
 _lib = None
@@ -179,6 +186,38 @@ class DenseFE:
         uh = np.empty(self.nn)
         ef = np.empty((self.ne, 3))
         lib().or_fe_get(self._h, _ptr(uh), _ptr(ef))
+        return uh, ef
+
+
+class RefFE:
+    """The reference's own FESolver (oracle/_ref) on the volume last loaded with
+    ref().ref_load_volume / ref_load_surface, set up as the reference's main does.  Methods 0
+    NonLinear and 1 GaussSeidel; the reference exits the process when a Newton solve does not
+    converge."""
+
+    def __init__(self, n_nodes, n_tets, n0, phi0, kTe, wall_potential=0.0):
+        self.nn, self.ne = n_nodes, n_tets
+        self.neq = int(ref().ref_fe_create(n0, phi0, kTe, wall_potential))
+        if self.neq < 0:
+            raise RuntimeError("ref_fe_create: no volume loaded")
+
+    def compute_phi(self, ion_den, method=0):
+        den = np.ascontiguousarray(ion_den, np.float64)
+        if ref().ref_fe_compute_phi(_ptr(den), method) != 0:
+            raise ValueError("method must be 0 (NonLinear) or 1 (GaussSeidel)")
+
+    def lapack_system(self, ion_den):
+        """(J, G) of the Lapack method's linearised step: J row-major with only j >= i set."""
+        den = np.ascontiguousarray(ion_den, np.float64)
+        J = np.zeros((self.neq, self.neq))
+        G = np.zeros(self.neq)
+        ref().ref_fe_lapack_system(_ptr(den), _ptr(J), _ptr(G))
+        return J, G
+
+    def phi_ef(self):
+        uh = np.empty(self.nn)
+        ef = np.empty((self.ne, 3))
+        ref().ref_fe_get(_ptr(uh), _ptr(ef))
         return uh, ef
 
 

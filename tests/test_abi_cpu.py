@@ -98,3 +98,19 @@ def test_field_linear_potential_is_uniform(mp):
     phi = m.node_pos @ a + 1.0
     ef = m.field_from_potential(phi)
     np.testing.assert_allclose(ef, np.broadcast_to(-a, ef.shape), rtol=1e-9, atol=1e-9)
+
+
+REF_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference sources absent (build container only)")
+def test_dropin_header_binds_reference_types():
+    """include/minipic_amd/move_particles.hpp instantiated with the reference's own Species,
+    Volume, FESolver and Parameters (src/particles.h, meshes.h, FESolver.h, parameters.h),
+    compiled with g++ -fsyntax-only where those headers lie (tests/cpp/dropin_reftypes.cpp)."""
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_reftypes.cpp")
+    r = subprocess.run(["g++", "-fsyntax-only", "-std=c++20", f"-I{REF_SRC}", f"-I{INCLUDE}", src],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
