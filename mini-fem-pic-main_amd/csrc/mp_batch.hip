@@ -19,6 +19,10 @@ namespace mpd {
 
 // The push of k_push_soa (same prologue: state, run and cell loaded together, no branch on
 // the cell before the start tet is evaluated) with the run's q/m and dt from the run table.
+// Persistent: the grid is what fits on the device at once (a multiple of 8), and each
+// workgroup loops over 256-particle chunks of its XCD's contiguous eighth of the array, so one
+// launch (and one captured graph) covers any live count up to the capacity; the count is read
+// from device memory (n_dev) or given (i0, n_host: one run's range).
 template <int WALL, int DET>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
@@ -26,54 +30,62 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
              double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
              int32_t* __restrict__ part, int cap) {
     const int64_t n = n_dev ? *n_dev : i0 + n_host;
-    const int64_t i = i0 + xcd_block() * blockDim.x + threadIdx.x;
-    bool act = false, lost = false, crossed = false, capped = false, ok = false;
-    int32_t fc = -1, base = 0;
-    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-    WalkStats st{0, 0};
-    if (i < n) {
-        const int32_t c_in = s.cell[i];
-        const int32_t r = s.run[i];
-        double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
-        const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
-        __builtin_amdgcn_sched_barrier(0);
-        const bool live = c_in >= 0;
-        int32_t c = live ? c_in : 0;
-        const double qm = par[3 * r], dt = par[3 * r + 1];
-        const DevTet* t = tets + c;
-        StartTet g0;
-        load_start(t, g0);
-        const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
-        vx = vx + (qm * ex) * dt;
-        vy = vy + (qm * ey) * dt;
-        vz = vz + (qm * ez) * dt;
-        double x = x0 + vx * dt;
-        double y = y0 + vy * dt;
-        double z = z0 + vz * dt;
-        const int rr = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
-        if (live) {
-            act = true;
-            base = r * nn;
-            crossed = (st.moves - st.refl) > 0;
-            if (rr == kInside) {
-                s.cell[i] = c;
-                ok = true;
-                fc = c;
-                if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3, (uint32_t)base);
-            } else {
-                s.cell[i] = -1;
-                lost = true;
-                capped = (rr == kCapped);
+    const int64_t chunks = (n - i0 + 255) / 256;
+    const int64_t per_xcd = (chunks + kXcds - 1) / kXcds;
+    const int64_t xcd = blockIdx.x % kXcds, stride = gridDim.x / kXcds;
+    int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
+    // block-uniform loop: every thread of the workgroup takes part in each chunk's scatter
+    for (int64_t k = blockIdx.x / kXcds; k < per_xcd; k += stride) {
+        const int64_t ch = xcd * per_xcd + k;
+        if (ch >= chunks) break;
+        const int64_t i = i0 + ch * 256 + threadIdx.x;
+        bool ok = false;
+        int32_t fc = -1, base = 0;
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        if (i < n) {
+            const int32_t c_in = s.cell[i];
+            const int32_t r = s.run[i];
+            double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
+            const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
+            __builtin_amdgcn_sched_barrier(0);
+            const bool live = c_in >= 0;
+            int32_t c = live ? c_in : 0;
+            const double qm = par[3 * r], dt = par[3 * r + 1];
+            const DevTet* t = tets + c;
+            StartTet g0;
+            load_start(t, g0);
+            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+            vx = vx + (qm * ex) * dt;
+            vy = vy + (qm * ey) * dt;
+            vz = vz + (qm * ez) * dt;
+            double x = x0 + vx * dt;
+            double y = y0 + vy * dt;
+            double z = z0 + vz * dt;
+            WalkStats st{0, 0};
+            const int rr = walk_start<WALL>(tets, g0, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, live ? cap : 0, st);
+            if (live) {
+                n_act++;
+                base = r * nn;
+                n_cross += (st.moves - st.refl) > 0;
+                n_moves += st.moves;
+                n_refl += st.refl;
+                if (rr == kInside) {
+                    s.cell[i] = c;
+                    ok = true;
+                    fc = c;
+                    if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3, (uint32_t)base);
+                } else {
+                    s.cell[i] = -1;
+                    n_lost++;
+                    n_cap += (rr == kCapped);
+                }
+                st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
+                st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
-            st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
-            st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
-        } else {
-            st.moves = 0;
-            st.refl = 0;
         }
+        if (!DET) scatter_lds(tets, den, ok, fc, l0, l1, l2, l3, base);
     }
-    if (!DET) scatter_lds(tets, den, ok, fc, l0, l1, l2, l3, base);
-    tally(part, act, lost, crossed, st.moves, capped, st.refl);
+    tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
 
 // K <= kStepMaxLaunch steps of k_push_batch in one launch, as k_push_steps does for one run:
@@ -90,8 +102,12 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
 #ifndef MP_BATCH_STEPS_WAVES
 #define MP_BATCH_STEPS_WAVES 3
 #endif
+// threads per workgroup of k_push_batch_steps (a multiple of 64, at most 1024)
+#ifndef MP_BATCH_STEPS_THREADS
+#define MP_BATCH_STEPS_THREADS 256
+#endif
 template <int WALL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_BATCH_STEPS_WAVES, 8)))
+__global__ void __launch_bounds__(MP_BATCH_STEPS_THREADS) __attribute__((amdgpu_waves_per_eu(MP_BATCH_STEPS_WAVES, 8)))
 k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ par, int32_t nn,
                    int32_t R, double* __restrict__ den, int K, int32_t* __restrict__ part, int cap) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -116,7 +132,7 @@ k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const doub
     load_tet_regs(tets + c, t);
     __shared__ uint32_t hk[1 << MP_BATCH_STEP_BITS];
     __shared__ double hv[1 << MP_BATCH_STEP_BITS];
-    __shared__ int32_t shc[256 / kWave * 6];
+    __shared__ int32_t shc[MP_BATCH_STEPS_THREADS / kWave * 6];
     __shared__ int32_t done;
     if (threadIdx.x == 0) done = 0;
     for (int q = threadIdx.x; q < (1 << MP_BATCH_STEP_BITS); q += blockDim.x) {
@@ -341,11 +357,33 @@ int batch_sort(mp_batch* b) {
     return MP_OK;
 }
 
+// Grid of the persistent k_push_batch<W, D> for `span` particles: one workgroup per 256
+// particles, at most as many as are resident on the device at once, a multiple of 8 (the
+// kernel's XCD mapping).
+template <int W, int D>
+unsigned persistent_grid(int64_t span) {
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mpd::k_push_batch<W, D>, 256, 0) != hipSuccess ||
+            cus < 1 || per_cu < 1) {
+            (void)hipGetLastError();
+            cus = 256;
+            per_cu = 4;
+        }
+        resident = std::max(mpd::kXcds, cus * per_cu / mpd::kXcds * mpd::kXcds);
+    }
+    const int64_t need = ((span + 255) / 256 + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds;
+    return (unsigned)std::min<int64_t>(need, resident);
+}
+
 int launch_atomic_step(mp_batch* b, const int64_t* n_dev, int64_t i0, int64_t n_host, int64_t k0, int64_t kcount) {
     mp_ctx* c = b->ctx;
     const int64_t span = n_dev ? b->n_slots : n_host;
     if (span > 0) {
-        const unsigned g = (unsigned)(((span + 255) / 256 + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds);
+        const unsigned g = c->opt.wall_mode == MP_WALL_ABSORB ? persistent_grid<0, 0>(span) : persistent_grid<1, 0>(span);
 #define MP_B(W) hipLaunchKernelGGL((mpd::k_push_batch<W, 0>), dim3(g), dim3(256), 0, c->stream, c->tets, \
                                    b->soa[b->cur], i0, n_host, n_dev, b->par_d, b->nn, b->den_raw, \
                                    (uint32_t*)nullptr, (double*)nullptr, b->tally, c->opt.walk_cap)
@@ -393,7 +431,8 @@ int det_step(mp_batch* b) {
         return set_err(c, MP_ERR_ALLOC, "mp_batch: segment table allocation failed");
     HIP_TRY(c, mpd::launch_fill_u32(b->keys[0], m, 0xFFFFFFFFu, c->stream));
     if (b->n_slots > 0) {
-        const unsigned g = (unsigned)(((b->n_slots + 255) / 256 + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds);
+        const unsigned g = c->opt.wall_mode == MP_WALL_ABSORB ? persistent_grid<0, 1>(b->n_slots)
+                                                              : persistent_grid<1, 1>(b->n_slots);
 #define MP_B(W) hipLaunchKernelGGL((mpd::k_push_batch<W, 1>), dim3(g), dim3(256), 0, c->stream, c->tets, \
                                    b->soa[b->cur], (int64_t)0, b->n_slots, (const int64_t*)nullptr, b->par_d, \
                                    b->nn, (double*)nullptr, b->keys[0], b->vals[0], b->tally, c->opt.walk_cap)
@@ -611,8 +650,9 @@ int mp_batch_step_many(mp_batch* b, int32_t mode, int32_t n_steps) {
         rc = reserve_tally_b(b, b->n_slots);
         if (rc) return rc;
         if (b->n_slots > 0) {
-            const unsigned g = (unsigned)(((b->n_slots + 255) / 256 + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds);
-#define MP_BS(W) hipLaunchKernelGGL((mpd::k_push_batch_steps<W>), dim3(g), dim3(256), 0, c->stream, c->tets, \
+            constexpr int kT = MP_BATCH_STEPS_THREADS;
+            const unsigned g = (unsigned)(((b->n_slots + kT - 1) / kT + mpd::kXcds - 1) / mpd::kXcds * mpd::kXcds);
+#define MP_BS(W) hipLaunchKernelGGL((mpd::k_push_batch_steps<W>), dim3(g), dim3(kT), 0, c->stream, c->tets, \
                                     b->soa[b->cur], b->n_slots, b->par_d, b->nn, b->R, \
                                     b->den_steps + (int64_t)done * per_step, chunk, b->tally, c->opt.walk_cap)
             if (c->opt.wall_mode == MP_WALL_ABSORB) MP_BS(0);
