@@ -374,7 +374,7 @@ def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
 
         forward(1.05 * DRIFT)
         tgt = mv.density()
-        fwd, adj = [], []
+        fwd, adj, adj_det = [], [], []
         for _ in range(reps):
             t0 = time.perf_counter()
             forward(DRIFT)
@@ -382,9 +382,14 @@ def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
             t0 = time.perf_counter()
             loss, g, gd = mv.adjoint(pos, u + DRIFT * d, cell, QM, dt, SPWT, steps, tgt, d, 10)
             adj.append(time.perf_counter() - t0)
-    f, a = min(fwd), min(adj)
-    log(f"adjoint: forward {1e3 * f:.1f} ms, adjoint {1e3 * a:.1f} ms, ratio {a / f:.2f}")
-    return {"forward_ms": 1e3 * f, "adjoint_ms": 1e3 * a, "ratio": a / f, "steps": steps, "particles": n,
+            t0 = time.perf_counter()
+            mv.adjoint(pos, u + DRIFT * d, cell, QM, dt, SPWT, steps, tgt, d, 10, mode=mp.SCATTER_DETERMINISTIC)
+            adj_det.append(time.perf_counter() - t0)
+    f, a, ad = min(fwd), min(adj), min(adj_det)
+    log(f"adjoint: forward {1e3 * f:.1f} ms, adjoint {1e3 * a:.1f} ms, ratio {a / f:.2f}; deterministic "
+        f"{1e3 * ad:.1f} ms, ratio {ad / f:.2f}")
+    return {"forward_ms": 1e3 * f, "adjoint_ms": 1e3 * a, "ratio": a / f, "deterministic_adjoint_ms": 1e3 * ad,
+            "deterministic_ratio": ad / f, "steps": steps, "particles": n,
             "loss": loss, "grad_norm": float(np.linalg.norm(g)), "dL_ddrift": gd,
             "workload": "2M particles on a 30^3 six-tet box (162,000 tets), 50 steps, checkpoint every 10; both "
                         "timings include the particle upload (best of 3)"}

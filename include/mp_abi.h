@@ -257,6 +257,15 @@ int  mp_adjoint(mp_ctx* ctx, const double* pos0, const double* vel0, const int32
                 double q_over_m, double dt, double spwt, int32_t steps, const double* den_target,
                 const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,
                 double* dL_ddrift);
+/* mp_adjoint with a scatter mode.  MP_SCATTER_ATOMIC is mp_adjoint (FP64 atomics: the sums
+ * of dL/dE vary in the last bits from run to run).  MP_SCATTER_DETERMINISTIC gives the same
+ * bits on every run: the last step's density is the ascending-pid serial fold, and dL/dE is
+ * summed exactly in 128-bit fixed point (scaled by a power of two chosen from a bound on every
+ * contribution) and rounded to FP64 once; loss and dL/d(drift) are summed in a fixed order. */
+int  mp_adjoint_ex(mp_ctx* ctx, const double* pos0, const double* vel0, const int32_t* cell0, int64_t n,
+                   double q_over_m, double dt, double spwt, int32_t steps, const double* den_target,
+                   const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,
+                   double* dL_ddrift, int32_t mode);
 
 /* ---- drop-in step with no allocation (no reference counterpart) ------------------------ */
 /* For callers that cannot allocate device memory: code under HIP graph capture, tight serving

@@ -159,6 +159,10 @@ int reserve_tally(mp_ctx* c, int64_t n, int block) {
     return MP_OK;
 }
 
+}  // namespace
+
+namespace mpctx {
+
 int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
     if (m > c->rec_cap) {
         for (int k = 0; k < 2; k++) {
@@ -200,6 +204,11 @@ int det_finish(mp_ctx* c, int64_t m, double spwt) {
                                     c->den_out, c->stream));
     return MP_OK;
 }
+
+}  // namespace mpctx
+
+namespace {
+using namespace mpctx;
 
 // Counting sort of the resident particles by bucket = cell >> sort_shift, dropping the lost
 // ones; order inside a bucket is whatever the atomics hand out (pid is carried along).

@@ -82,6 +82,10 @@ int set_err(mp_ctx* c, int code, const std::string& msg);
 int bit_width(uint64_t v);
 void dfree(void* p);
 int reserve_scan(mp_ctx* c, int64_t n);
+// Deterministic-scatter records for m record slots (keys[0..1], vals[0..1], sort scratch and
+// segment table), and the sort + ordered fold of m records into den_out (normalised by spwt).
+int reserve_records(mp_ctx* c, int64_t m, int end_bit);
+int det_finish(mp_ctx* c, int64_t m, double spwt);
 hipError_t alloc_soa(void** mem, Soa* s, int64_t n);
 
 template <typename T>

@@ -102,6 +102,8 @@ SIGNATURES = {
     "mp_get_timing": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64)]),
     "mp_adjoint": (C.c_int, [_P, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32, C.POINTER(_D), _P,
                              C.POINTER(_D)]),
+    "mp_adjoint_ex": (C.c_int, [_P, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32, C.POINTER(_D), _P,
+                                C.POINTER(_D), _I32]),
     "mp_autotune": (C.c_int, [_P, _D, _D, _D, _I32, _I32, C.POINTER(MpOptions), C.POINTER(_D),
                               C.POINTER(MpTuneTrial), _I32, C.POINTER(_I32)]),
     "mp_batch_create": (C.c_int, [_P, _I32, C.POINTER(_P)]),
@@ -609,16 +611,18 @@ class Mover:
     def stream(self):
         return lib().mp_stream(self._h)
 
-    def adjoint(self, pos0, vel0, cell0, q_over_m, dt, spwt, steps, den_target, drift_dir, checkpoint_every=10):
-        """(loss, dL/dE [n_tets][3], dL/d drift) of K frozen-field steps (mp_adjoint)."""
+    def adjoint(self, pos0, vel0, cell0, q_over_m, dt, spwt, steps, den_target, drift_dir, checkpoint_every=10,
+                mode=SCATTER_ATOMIC):
+        """(loss, dL/dE [n_tets][3], dL/d drift) of K frozen-field steps (mp_adjoint_ex; mode
+        SCATTER_DETERMINISTIC gives the same bits on every run)."""
         pos0, vel0 = _f64(pos0, (-1, 3)), _f64(vel0, (-1, 3))
         cell0 = np.ascontiguousarray(cell0, dtype=np.int32)
         tgt, dd = _f64(den_target), _f64(drift_dir)
         loss, dd_out = C.c_double(), C.c_double()
         g = np.empty((self.n_tets, 3), np.float64)
-        self._check(lib().mp_adjoint(self._h, _ptr(pos0), _ptr(vel0), _ptr(cell0), len(pos0), q_over_m, dt, spwt,
-                                     steps, _ptr(tgt), _ptr(dd), checkpoint_every, C.byref(loss), _ptr(g),
-                                     C.byref(dd_out)))
+        self._check(lib().mp_adjoint_ex(self._h, _ptr(pos0), _ptr(vel0), _ptr(cell0), len(pos0), q_over_m, dt,
+                                        spwt, steps, _ptr(tgt), _ptr(dd), checkpoint_every, C.byref(loss), _ptr(g),
+                                        C.byref(dd_out), mode))
         return float(loss.value), g, float(dd_out.value)
 
     def autotune(self, q_over_m, dt, spwt, mode=SCATTER_ATOMIC, min_steps=10, max_trials=64, multi_step=False):

@@ -44,6 +44,31 @@ def test_adjoint_matches_oracle(mp, orc):
     assert abs(gd - gdr) <= 1e-10 * abs(gdr)
 
 
+def test_adjoint_deterministic_same_bits_and_matches_oracle(mp, orc):
+    """mp_adjoint_ex in deterministic mode: the same loss, dL/dE and dL/d(drift) bits on every
+    run, also with a different cell-sort key (which reorders the particles and so the atomics);
+    against the oracle gradient to the atomic mode's bar."""
+    mesh, ef, pos, u, cell, d, dt, steps, drift = problem(mp, n=20000)
+    tgt = target(mp, mesh, ef, pos, u, cell, d, dt, steps, drift)
+    outs = []
+    for shift, order in ((0, 0), (3, 1), (0, 0)):
+        with mp.Mover.from_mesh(mesh) as mv:
+            mv.set_options(sort_shift=shift, sort_order=order)
+            mv.upload_field(ef)
+            outs.append(mv.adjoint(pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d, checkpoint_every=5,
+                                   mode=mp.SCATTER_DETERMINISTIC))
+    for o in outs[1:]:
+        assert o[0] == outs[0][0] and o[2] == outs[0][2]
+        assert o[1].tobytes() == outs[0][1].tobytes()
+    L, g, gd = outs[0]
+    Lr, gr, gdr = orc.adjoint(mesh.tets, ef, mesh.node_volume, pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d)
+    assert abs(L - Lr) <= 1e-12 * abs(Lr)
+    scale = np.abs(gr).max()
+    assert scale > 0 and np.count_nonzero(g) > 0
+    np.testing.assert_allclose(g, gr, rtol=0, atol=1e-10 * scale)
+    assert abs(gd - gdr) <= 1e-10 * abs(gdr)
+
+
 @pytest.mark.parametrize("ckpt", [1, 4, 12])
 def test_adjoint_checkpointing_invariant(mp, ckpt):
     """The gradient does not depend on the checkpoint interval (replay is exact)."""

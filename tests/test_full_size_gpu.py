@@ -212,7 +212,8 @@ def test_clustered_100cubed_reflecting_matches_oracle(mp, orc, box100, mode, ord
 def test_adjoint_2M_30cubed_50_steps_matches_oracle(mp, orc):
     """The adjoint workload: 2M particles on the 30^3 box (162,000 tets), 50 steps, checkpoint
     every 10, target density from a forward run at 1.05 x drift; loss, dL/dE (3 x 162,000) and
-    dL/d(drift) against the oracle's FP64 gradient."""
+    dL/d(drift) against the oracle's FP64 gradient, in atomic and deterministic mode; the
+    deterministic mode gives the same bits twice."""
     mesh = mp.Mesh.box_mesh(30, 1.0)
     ef = mesh.field_sinusoid(10.0)
     pos, u = mp.synth_uniform(20240605, 2_000_000, [0, 0, 0], [1, 1, 1], 2000.0, [0, 0, 0])
@@ -225,10 +226,14 @@ def test_adjoint_2M_30cubed_50_steps_matches_oracle(mp, orc):
         for _ in range(steps):
             mv.step(QM, dt, SPWT, 1)
         tgt = mv.density()
-        L, g, gd = mv.adjoint(pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d, checkpoint_every=10)
+        runs = [mv.adjoint(pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d, checkpoint_every=10, mode=mode)
+                for mode in (0, 1, 1)]
     Lr, gr, gdr = orc.adjoint(mesh.tets, ef, mesh.node_volume, pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d)
-    assert Lr > 0 and abs(L - Lr) <= 1e-12 * abs(Lr)
     scale = np.abs(gr).max()
-    assert scale > 0 and np.count_nonzero(gr) > len(gr)
-    np.testing.assert_allclose(g, gr, rtol=0, atol=1e-10 * scale)
-    assert abs(gd - gdr) <= 1e-10 * abs(gdr)
+    assert Lr > 0 and scale > 0 and np.count_nonzero(gr) > len(gr)
+    for L, g, gd in runs:  # atomic, deterministic, deterministic again
+        assert abs(L - Lr) <= 1e-12 * abs(Lr)
+        np.testing.assert_allclose(g, gr, rtol=0, atol=1e-10 * scale)
+        assert abs(gd - gdr) <= 1e-10 * abs(gdr)
+    (L1, g1, d1), (L2, g2, d2) = runs[1], runs[2]
+    assert L1 == L2 and d1 == d2 and g1.tobytes() == g2.tobytes(), "deterministic adjoint differs between runs"
