@@ -78,6 +78,15 @@ int     mp_fe_get_phi(const mp_fe* fe, double* uh);
 /* The assembled stiffness matrix in compressed rows over the unknowns:
  * row_ptr[neq+1], col[nnz], val[nnz]; any pointer may be NULL (then only *nnz is set). */
 int     mp_fe_get_stiffness(const mp_fe* fe, int64_t* nnz, int64_t* row_ptr, int32_t* col, double* val);
+/* The solver's own method timings, the rows FESolver's TRACE_ME calls add to trace.csv
+ * (src/FESolver.cpp:48-622, src/trace.cpp:36-76): entry k in [0, MP_FE_PROFILE_ENTRIES) gives
+ * the method's name, its call count, its inclusive wall seconds and the seconds spent in its
+ * traced callees, accumulated since mp_fe_create.  "solveLinearDirect" is MP_FE_DIRECT's solve
+ * (the reference's solveLinearLapack); ~FESolver, startAssembly (part of preAssembly here),
+ * addKe/addFe and inverse have no separate rows.  Any output pointer may be NULL. */
+#define MP_FE_PROFILE_ENTRIES 11
+int     mp_fe_get_profile(const mp_fe* fe, int32_t k, const char** name, int64_t* calls, double* total_s,
+                          double* child_s);
 const char* mp_fe_last_error(void);
 
 #ifdef __cplusplus

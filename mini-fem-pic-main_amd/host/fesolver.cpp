@@ -9,6 +9,7 @@
 #include "mp_fem.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -27,6 +28,23 @@ int fail(int code, const std::string& m) {
 // src/FESolver.h:18-19
 constexpr double kEPS0 = 8.8541878e-12;
 constexpr double kQE = 1.602e-19;
+
+// The solver's methods as the reference's trace rows name them (src/FESolver.cpp TRACE_ME):
+// the direct method stands in for solveLinearLapack.
+enum FeFn {
+    kFnFESolver, kFnComputeNX, kFnPreAssembly, kFnComputePhi, kFnSolve, kFnSolveNonLinear, kFnBuildF1Vector, kFnBuildJmatrix,
+    kFnSolveLinear, kFnSolveLinearDirect, kFnUpdateEf, kFnCount
+};
+const char* const kFnName[kFnCount] = {"FESolver",       "computeNX",     "preAssembly",  "computePhi",   "solve",
+                                       "solveNonLinear", "buildF1Vector", "buildJmatrix", "solveLinear",
+                                       "solveLinearDirect", "updateEf"};
+
+static_assert(kFnCount == MP_FE_PROFILE_ENTRIES, "profile rows");
+
+struct FnProfile {
+    int64_t calls = 0;
+    double total = 0.0, child = 0.0;  // inclusive seconds, seconds of traced callees
+};
 
 }  // namespace
 
@@ -48,7 +66,29 @@ struct mp_fe {
     int32_t flags = 0;
     double l[2] = {0, 0}, W[2] = {1, 1};
     int n_int = 2;
+    // wall time per method and the stack of open ones (the reference's TRACE_ME bookkeeping)
+    mutable FnProfile prof[kFnCount];
+    mutable std::vector<int> open;
 };
+
+namespace {
+
+// Times one method call: inclusive time to the method, and to its caller's callee time.
+struct Traced {
+    const mp_fe& fe;
+    int fn;
+    std::chrono::steady_clock::time_point t0;
+    Traced(const mp_fe& f, int k) : fe(f), fn(k), t0(std::chrono::steady_clock::now()) { fe.open.push_back(k); }
+    ~Traced() {
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        fe.prof[fn].calls++;
+        fe.prof[fn].total += dt;
+        fe.open.pop_back();
+        if (!fe.open.empty()) fe.prof[fe.open.back()].child += dt;
+    }
+};
+
+}  // namespace
 
 namespace {
 
@@ -82,6 +122,7 @@ double quad_na(const mp_fe& fe, int e, int a, double v) {
 
 // preAssembly (src/FESolver.cpp:434-485) after startAssembly (:155-163): K, F0.
 void pre_assembly(mp_fe& fe) {
+    Traced tr_(fe, kFnPreAssembly);
     std::fill(fe.K.begin(), fe.K.end(), 0.0);
     std::fill(fe.F0.begin(), fe.F0.end(), 0.0);
     std::fill(fe.F1.begin(), fe.F1.end(), 0.0);
@@ -122,6 +163,7 @@ void pre_assembly(mp_fe& fe) {
 // buildF1Vector (src/FESolver.cpp:488-526): adds this call's "ff" term into F1 (cleared
 // first under MP_FE_RESET_F1).
 void build_f1(mp_fe& fe, const double* ion_den) {
+    Traced tr_(fe, kFnBuildF1Vector);
     if (fe.flags & MP_FE_RESET_F1) std::fill(fe.F1.begin(), fe.F1.end(), 0.0);
     std::vector<double> f((size_t)fe.neq);
     for (int n = 0; n < fe.n_nodes; n++) {
@@ -147,6 +189,7 @@ void mat_vec(const mp_fe& fe, const std::vector<double>& A, const std::vector<do
 
 // buildJmatrix (src/FESolver.cpp:367-431): J = K - diag(FP).
 void build_j(mp_fe& fe) {
+    Traced tr_(fe, kFnBuildJmatrix);
     std::vector<double> fp_term((size_t)fe.neq), FP((size_t)fe.neq, 0.0);
     for (int n = 0; n < fe.neq; n++)
         fp_term[n] = -kQE / kEPS0 * fe.n0 * std::exp((fe.d[n] - fe.phi0) / fe.kTe) * (1 / fe.kTe);
@@ -169,6 +212,7 @@ void build_j(mp_fe& fe) {
 // residual checked every 25 sweeps, tol 1e-4 on sqrt(sum r^2) / neq, at most 10000 sweeps.
 void gauss_seidel(const mp_fe& fe, const std::vector<double>& A, std::vector<double>& x,
                   const std::vector<double>& b, mp_fe_stats& st) {
+    Traced tr_(fe, kFnSolveLinear);
     const double tol = 1e-4;
     for (int u = 0; u < fe.neq; u++)
         if (std::fabs(A[fe.dg[u]]) < 1e-12) st.zero_diagonal++;
@@ -210,6 +254,7 @@ void gauss_seidel(const mp_fe& fe, const std::vector<double>& A, std::vector<dou
 // the factorisation runs on -J (and -b) then.
 bool cholesky_solve(const mp_fe& fe, const std::vector<double>& A, std::vector<double>& x,
                     const std::vector<double>& b_in) {
+    Traced tr_(fe, kFnSolveLinearDirect);
     const int n = fe.neq;
     if (n == 0) return true;
     const double sg = A[fe.dg[0]] < 0 ? -1.0 : 1.0;
@@ -251,6 +296,141 @@ void residual(const mp_fe& fe, std::vector<double>& G) {
     for (int i = 0; i < fe.neq; i++) G[i] = G[i] - fe.F1[i];
 }
 
+// solve (src/FESolver.cpp:290-321): y starts at zero and carries over between Newton
+// iterations as the Gauss-Seidel initial guess; d -= y after each linear solve.
+int solve(mp_fe& fe, const double* ion_den, int32_t method, mp_fe_stats& st, std::string& err) {
+    Traced tr_(fe, kFnSolve);
+    std::vector<double> y((size_t)fe.neq, 0.0), G((size_t)fe.neq);
+    if (method == MP_FE_NONLINEAR) {
+        Traced tr_nr(fe, kFnSolveNonLinear);
+        double L2 = 0;
+        for (int it = 0; it < 10; it++) {
+            build_f1(fe, ion_den);
+            residual(fe, G);
+            build_j(fe);
+            gauss_seidel(fe, fe.J, y, G, st);
+            for (int n = 0; n < fe.neq; n++) fe.d[n] -= y[n];
+            double sum = 0;
+            for (int u = 0; u < fe.neq; u++) sum += y[u] * y[u];
+            L2 = std::sqrt(sum) / fe.neq;
+            st.nr_iterations = it + 1;
+            st.nr_l2 = L2;
+            if (L2 < 1e-2) return MP_OK;
+        }
+        err = "NR failed to converge, L2 = " + std::to_string(L2);
+        return MP_ERR_STATE;
+    }
+    build_f1(fe, ion_den);
+    residual(fe, G);
+    build_j(fe);
+    if (method == MP_FE_GAUSS_SEIDEL) {
+        gauss_seidel(fe, fe.J, y, G, st);
+    } else if (!cholesky_solve(fe, fe.J, y, G)) {
+        err = "mp_fe_compute_phi: J is not definite";
+        return MP_ERR_STATE;
+    }
+    for (int n = 0; n < fe.neq; n++) fe.d[n] -= y[n];
+    st.nr_iterations = 1;
+    return MP_OK;
+}
+
+// The constructor, startAssembly and preAssembly of src/FESolver.cpp:48-128, 155-164, 434-486
+// over the mesh's tables; MP_ERR_ARG for a node index outside the mesh.
+int fe_setup(mp_fe& fe, const mp_mesh* m, double wall_potential) {
+    fe.n_nodes = mp_mesh_num_nodes(m);
+    fe.n_el = mp_mesh_num_tets(m);
+    const mp_tetra* tets = mp_mesh_tets(m);
+    const double* pos = mp_mesh_node_pos(m);
+    const int32_t* type = mp_mesh_node_type(m);
+    // unknowns: NORMAL and OPEN nodes, in node order (src/FESolver.cpp:51-55, 101-107)
+    fe.ID.assign((size_t)fe.n_nodes, -1);
+    for (int n = 0; n < fe.n_nodes; n++)
+        if (type[n] == MP_NODE_NORMAL || type[n] == MP_NODE_OPEN) fe.ID[n] = fe.neq++;
+    fe.con.resize((size_t)fe.n_el * 4);
+    fe.LM.resize((size_t)fe.n_el * 4);
+    fe.NX.resize((size_t)fe.n_el * 12);
+    fe.detJ.resize((size_t)fe.n_el);
+    for (int e = 0; e < fe.n_el; e++)
+        for (int a = 0; a < 4; a++) {
+            const int32_t nd = tets[e].con[a];
+            if (nd < 0 || nd >= fe.n_nodes) return MP_ERR_ARG;
+            fe.con[(size_t)e * 4 + a] = nd;
+            fe.LM[(size_t)e * 4 + a] = fe.ID[nd];
+        }
+    {
+        Traced tr_nx(fe, kFnComputeNX);
+        for (int e = 0; e < fe.n_el; e++) {
+            double x[4][3], NX[4][3];
+            for (int a = 0; a < 4; a++) {
+                const int32_t nd = fe.con[(size_t)e * 4 + a];
+                for (int q = 0; q < 3; q++) x[a][q] = pos[3 * (size_t)nd + q];
+            }
+            mphost::element_nx(x, NX, &fe.detJ[e]);
+            for (int a = 0; a < 4; a++)
+                for (int q = 0; q < 3; q++) fe.NX[(size_t)e * 12 + 3 * a + q] = NX[a][q];
+        }
+    }
+    // quadrature (src/FESolver.cpp:120-123)
+    fe.l[0] = -std::sqrt(1.0 / 3.0);
+    fe.l[1] = std::sqrt(1.0 / 3.0);
+    fe.W[0] = 1;
+    fe.W[1] = 1;
+    fe.n_int = 2;
+    // row pattern: unknown pairs that share an element
+    std::vector<std::vector<int32_t>> cols((size_t)fe.neq);
+    for (int e = 0; e < fe.n_el; e++)
+        for (int a = 0; a < 4; a++) {
+            const int P = fe.LM[(size_t)e * 4 + a];
+            if (P < 0) continue;
+            for (int b = 0; b < 4; b++) {
+                const int Q = fe.LM[(size_t)e * 4 + b];
+                if (Q >= 0) cols[P].push_back(Q);
+            }
+        }
+    fe.rp.assign((size_t)fe.neq + 1, 0);
+    for (int P = 0; P < fe.neq; P++) {
+        auto& c = cols[P];
+        c.push_back(P);  // the diagonal exists even for a node in no element
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+        fe.rp[P + 1] = fe.rp[P] + (int64_t)c.size();
+    }
+    fe.ci.resize((size_t)fe.rp[fe.neq]);
+    fe.dg.resize((size_t)fe.neq);
+    for (int P = 0; P < fe.neq; P++) {
+        std::copy(cols[P].begin(), cols[P].end(), fe.ci.begin() + fe.rp[P]);
+        fe.dg[P] = fe.rp[P] + (std::lower_bound(cols[P].begin(), cols[P].end(), P) - cols[P].begin());
+    }
+    fe.slot.assign((size_t)fe.n_el * 16, -1);
+    for (int e = 0; e < fe.n_el; e++)
+        for (int a = 0; a < 4; a++) {
+            const int P = fe.LM[(size_t)e * 4 + a];
+            if (P < 0) continue;
+            for (int b = 0; b < 4; b++) {
+                const int Q = fe.LM[(size_t)e * 4 + b];
+                if (Q < 0) continue;
+                const auto b0 = fe.ci.begin() + fe.rp[P], b1 = fe.ci.begin() + fe.rp[P + 1];
+                fe.slot[(size_t)e * 16 + 4 * a + b] = std::lower_bound(b0, b1, Q) - fe.ci.begin();
+            }
+        }
+    const size_t nnz = fe.ci.size();
+    fe.K.assign(nnz, 0.0);
+    fe.J.assign(nnz, 0.0);
+    fe.F0.assign((size_t)fe.neq, 0.0);
+    fe.F1.assign((size_t)fe.neq, 0.0);
+    fe.d.assign((size_t)fe.neq, 0.0);
+    fe.uh.assign((size_t)fe.n_nodes, 0.0);
+    // g: 0 on INLET, -wall_potential on FIXED, 0 elsewhere (src/mini-pic.cpp:97-102)
+    fe.g.assign((size_t)fe.n_nodes, 0.0);
+    for (int n = 0; n < fe.n_nodes; n++) {
+        if (type[n] == MP_NODE_INLET) fe.g[n] = 0;
+        else if (type[n] == MP_NODE_FIXED) fe.g[n] = -wall_potential;
+        else fe.g[n] = 0;
+    }
+    pre_assembly(fe);
+    return MP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -265,96 +445,18 @@ int mp_fe_create(const mp_mesh* m, double n0, double phi0, double kTe, double wa
     if (flags & ~MP_FE_RESET_F1) return fail(MP_ERR_ARG, "mp_fe_create: unknown flags");
     mp_fe* fe = new mp_fe;
     fe->flags = flags;
-    fe->n_nodes = mp_mesh_num_nodes(m);
-    fe->n_el = mp_mesh_num_tets(m);
-    const mp_tetra* tets = mp_mesh_tets(m);
-    const double* pos = mp_mesh_node_pos(m);
-    const int32_t* type = mp_mesh_node_type(m);
     fe->n0 = n0;
     fe->phi0 = phi0;
     fe->kTe = kTe;
-    // unknowns: NORMAL and OPEN nodes, in node order (src/FESolver.cpp:51-55, 101-107)
-    fe->ID.assign((size_t)fe->n_nodes, -1);
-    for (int n = 0; n < fe->n_nodes; n++)
-        if (type[n] == MP_NODE_NORMAL || type[n] == MP_NODE_OPEN) fe->ID[n] = fe->neq++;
-    fe->con.resize((size_t)fe->n_el * 4);
-    fe->LM.resize((size_t)fe->n_el * 4);
-    fe->NX.resize((size_t)fe->n_el * 12);
-    fe->detJ.resize((size_t)fe->n_el);
-    for (int e = 0; e < fe->n_el; e++) {
-        double x[4][3], NX[4][3];
-        for (int a = 0; a < 4; a++) {
-            const int32_t nd = tets[e].con[a];
-            if (nd < 0 || nd >= fe->n_nodes) {
-                delete fe;
-                return fail(MP_ERR_ARG, "mp_fe_create: node index out of range");
-            }
-            fe->con[(size_t)e * 4 + a] = nd;
-            fe->LM[(size_t)e * 4 + a] = fe->ID[nd];
-            for (int q = 0; q < 3; q++) x[a][q] = pos[3 * (size_t)nd + q];
-        }
-        mphost::element_nx(x, NX, &fe->detJ[e]);
-        for (int a = 0; a < 4; a++)
-            for (int q = 0; q < 3; q++) fe->NX[(size_t)e * 12 + 3 * a + q] = NX[a][q];
+    int rc;
+    {
+        Traced tr_(*fe, kFnFESolver);
+        rc = fe_setup(*fe, m, wall_potential);
     }
-    // quadrature (src/FESolver.cpp:120-123)
-    fe->l[0] = -std::sqrt(1.0 / 3.0);
-    fe->l[1] = std::sqrt(1.0 / 3.0);
-    fe->W[0] = 1;
-    fe->W[1] = 1;
-    fe->n_int = 2;
-    // row pattern: unknown pairs that share an element
-    std::vector<std::vector<int32_t>> cols((size_t)fe->neq);
-    for (int e = 0; e < fe->n_el; e++)
-        for (int a = 0; a < 4; a++) {
-            const int P = fe->LM[(size_t)e * 4 + a];
-            if (P < 0) continue;
-            for (int b = 0; b < 4; b++) {
-                const int Q = fe->LM[(size_t)e * 4 + b];
-                if (Q >= 0) cols[P].push_back(Q);
-            }
-        }
-    fe->rp.assign((size_t)fe->neq + 1, 0);
-    for (int P = 0; P < fe->neq; P++) {
-        auto& c = cols[P];
-        c.push_back(P);  // the diagonal exists even for a node in no element
-        std::sort(c.begin(), c.end());
-        c.erase(std::unique(c.begin(), c.end()), c.end());
-        fe->rp[P + 1] = fe->rp[P] + (int64_t)c.size();
+    if (rc) {
+        delete fe;
+        return fail(MP_ERR_ARG, "mp_fe_create: node index out of range");
     }
-    fe->ci.resize((size_t)fe->rp[fe->neq]);
-    fe->dg.resize((size_t)fe->neq);
-    for (int P = 0; P < fe->neq; P++) {
-        std::copy(cols[P].begin(), cols[P].end(), fe->ci.begin() + fe->rp[P]);
-        fe->dg[P] = fe->rp[P] + (std::lower_bound(cols[P].begin(), cols[P].end(), P) - cols[P].begin());
-    }
-    fe->slot.assign((size_t)fe->n_el * 16, -1);
-    for (int e = 0; e < fe->n_el; e++)
-        for (int a = 0; a < 4; a++) {
-            const int P = fe->LM[(size_t)e * 4 + a];
-            if (P < 0) continue;
-            for (int b = 0; b < 4; b++) {
-                const int Q = fe->LM[(size_t)e * 4 + b];
-                if (Q < 0) continue;
-                const auto b0 = fe->ci.begin() + fe->rp[P], b1 = fe->ci.begin() + fe->rp[P + 1];
-                fe->slot[(size_t)e * 16 + 4 * a + b] = std::lower_bound(b0, b1, Q) - fe->ci.begin();
-            }
-        }
-    const size_t nnz = fe->ci.size();
-    fe->K.assign(nnz, 0.0);
-    fe->J.assign(nnz, 0.0);
-    fe->F0.assign((size_t)fe->neq, 0.0);
-    fe->F1.assign((size_t)fe->neq, 0.0);
-    fe->d.assign((size_t)fe->neq, 0.0);
-    fe->uh.assign((size_t)fe->n_nodes, 0.0);
-    // g: 0 on INLET, -wall_potential on FIXED, 0 elsewhere (src/mini-pic.cpp:97-102)
-    fe->g.assign((size_t)fe->n_nodes, 0.0);
-    for (int n = 0; n < fe->n_nodes; n++) {
-        if (type[n] == MP_NODE_INLET) fe->g[n] = 0;
-        else if (type[n] == MP_NODE_FIXED) fe->g[n] = -wall_potential;
-        else fe->g[n] = 0;
-    }
-    pre_assembly(*fe);
     *out = fe;
     return MP_OK;
 }
@@ -367,44 +469,13 @@ int mp_fe_compute_phi(mp_fe* fe, const double* ion_den, int32_t method, mp_fe_st
     if (!fe || !ion_den) return fail(MP_ERR_ARG, "mp_fe_compute_phi: null argument");
     if (method != MP_FE_NONLINEAR && method != MP_FE_GAUSS_SEIDEL && method != MP_FE_DIRECT)
         return fail(MP_ERR_ARG, "mp_fe_compute_phi: unknown method");
+    Traced tr_phi(*fe, kFnComputePhi);
     mp_fe_stats st{0, 0, 0, 0, 0.0};
-    // solve (src/FESolver.cpp:290-321): y starts at zero and carries over between Newton
-    // iterations as the Gauss-Seidel initial guess
-    std::vector<double> y((size_t)fe->neq, 0.0), G((size_t)fe->neq);
-    if (method == MP_FE_NONLINEAR) {
-        bool converged = false;
-        double L2 = 0;
-        for (int it = 0; it < 10; it++) {
-            build_f1(*fe, ion_den);
-            residual(*fe, G);
-            build_j(*fe);
-            gauss_seidel(*fe, fe->J, y, G, st);
-            for (int n = 0; n < fe->neq; n++) fe->d[n] -= y[n];
-            double sum = 0;
-            for (int u = 0; u < fe->neq; u++) sum += y[u] * y[u];
-            L2 = std::sqrt(sum) / fe->neq;
-            st.nr_iterations = it + 1;
-            st.nr_l2 = L2;
-            if (L2 < 1e-2) {
-                converged = true;
-                break;
-            }
-        }
-        if (!converged) {
-            if (st_out) *st_out = st;
-            return fail(MP_ERR_STATE, "NR failed to converge, L2 = " + std::to_string(L2));
-        }
-    } else {
-        build_f1(*fe, ion_den);
-        residual(*fe, G);
-        build_j(*fe);
-        if (method == MP_FE_GAUSS_SEIDEL) {
-            gauss_seidel(*fe, fe->J, y, G, st);
-        } else if (!cholesky_solve(*fe, fe->J, y, G)) {
-            return fail(MP_ERR_STATE, "mp_fe_compute_phi: J is not definite");
-        }
-        for (int n = 0; n < fe->neq; n++) fe->d[n] -= y[n];
-        st.nr_iterations = 1;
+    std::string err;
+    const int rc = solve(*fe, ion_den, method, st, err);
+    if (rc) {
+        if (st_out) *st_out = st;
+        return fail(rc, err);
     }
     // computePhi (src/FESolver.cpp:604-619): uh = g + d on the unknowns
     for (int n = 0; n < fe->n_nodes; n++) {
@@ -418,6 +489,7 @@ int mp_fe_compute_phi(mp_fe* fe, const double* ion_den, int32_t method, mp_fe_st
 
 int mp_fe_update_ef(const mp_fe* fe, double* ef) {
     if (!fe || !ef) return fail(MP_ERR_ARG, "mp_fe_update_ef: null argument");
+    Traced tr_(*fe, kFnUpdateEf);
     for (int e = 0; e < fe->n_el; e++) {
         double NX[4][3], u[4];
         for (int a = 0; a < 4; a++) {
@@ -441,6 +513,17 @@ int mp_fe_get_stiffness(const mp_fe* fe, int64_t* nnz, int64_t* row_ptr, int32_t
     if (row_ptr) std::copy(fe->rp.begin(), fe->rp.end(), row_ptr);
     if (col) std::copy(fe->ci.begin(), fe->ci.end(), col);
     if (val) std::copy(fe->K.begin(), fe->K.end(), val);
+    return MP_OK;
+}
+
+int mp_fe_get_profile(const mp_fe* fe, int32_t k, const char** name, int64_t* calls, double* total_s,
+                      double* child_s) {
+    if (!fe) return fail(MP_ERR_ARG, "mp_fe_get_profile: null solver");
+    if (k < 0 || k >= kFnCount) return fail(MP_ERR_ARG, "mp_fe_get_profile: no such entry");
+    if (name) *name = kFnName[k];
+    if (calls) *calls = fe->prof[k].calls;
+    if (total_s) *total_s = fe->prof[k].total;
+    if (child_s) *child_s = fe->prof[k].child;
     return MP_OK;
 }
 

@@ -139,6 +139,8 @@ SIGNATURES = {
     "mp_fe_update_ef": (C.c_int, [_P, _P]),
     "mp_fe_get_phi": (C.c_int, [_P, _P]),
     "mp_fe_get_stiffness": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P]),
+    "mp_fe_get_profile": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(_I64),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mp_fe_last_error": (C.c_char_p, []),
     # mp_output.h
     "mp_write_mesh_vtu": (C.c_int, [_P, C.c_char_p, _P, _P, _P]),
@@ -319,6 +321,7 @@ def synth_mixture(seed, n, lo, hi, weights, centres, sigma_pos, sigma_vel, drift
 # ---------------------------------------------------------------------------------------
 FE_NONLINEAR, FE_GAUSS_SEIDEL, FE_DIRECT = 0, 1, 2
 FE_RESET_F1 = 1
+FE_PROFILE_ENTRIES = 11
 KB = 8.617333262e-5   # src/FESolver.h:21, eV/K
 EPS0 = 8.8541878e-12  # src/FESolver.h:18
 
@@ -375,6 +378,19 @@ class FESolver:
         if rc != MP_OK:
             raise MpError(rc, lib().mp_fe_last_error().decode())
         return rp, ci, val
+
+    def profile(self):
+        """The solver's method timings since creation, as trace rows: {name: (calls, inclusive
+        seconds, seconds in traced callees)} (FESolver's TRACE_ME rows, src/FESolver.cpp)."""
+        rows = {}
+        for k in range(FE_PROFILE_ENTRIES):
+            name, calls = C.c_char_p(), C.c_int64()
+            total, child = C.c_double(), C.c_double()
+            rc = lib().mp_fe_get_profile(self._h, k, C.byref(name), C.byref(calls), C.byref(total), C.byref(child))
+            if rc != MP_OK:
+                raise MpError(rc, lib().mp_fe_last_error().decode())
+            rows[name.value.decode()] = (int(calls.value), float(total.value), float(child.value))
+        return rows
 
 
 class Rng:

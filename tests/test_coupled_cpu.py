@@ -249,7 +249,13 @@ def test_vtk_output_layout(mp, orc, cp, tmp_path):
     calls = {r[0]: int(r[4]) for r in rows[1:]}
     assert calls["InjectIons"] == calls["MoveParticles"] == calls["computePhi"] == calls["updateEf"] == 10
     assert calls["OutputMesh"] == 2 and calls["OutputParticles"] == 1
+    # the solver's own methods (src/FESolver.cpp TRACE_ME): one Newton solve per step
+    assert calls["FESolver"] == calls["computeNX"] == calls["preAssembly"] == 1
+    assert calls["solve"] == calls["solveNonLinear"] == 10
+    assert calls["buildF1Vector"] == calls["buildJmatrix"] == calls["solveLinear"] >= 10
+    assert "solveLinearDirect" not in calls  # never called, so no row
     assert [r[0] for r in rows[-3:]] == ["traceEnter", "traceExit", "traceTotalTime"]
+    assert len({r[0] for r in rows}) == len(rows)
     text = (out / "mesh_0011.vtu").read_text()
     root = ET.fromstring(text)
     piece = root.find("UnstructuredGrid/Piece")
@@ -269,3 +275,35 @@ def test_vtk_output_layout(mp, orc, cp, tmp_path):
     proot = ET.fromstring((out / "particles.vtp").read_text())
     assert int(proot.find("PolyData/Piece").get("NumberOfPoints")) == len(run.parts)
     assert proot.find("PolyData/Piece/Points/DataArray").text.split() == ["%g" % v for v in run.parts["pos"].ravel()]
+
+
+def test_fe_profile_nests_like_the_reference_trace(mp, cp, tmp_path):
+    """FESolver's trace rows: computePhi encloses solve, which encloses the Newton loop and its
+    buildF1Vector / buildJmatrix / solveLinear calls, so each parent's callee time is the sum of
+    its children's inclusive times and no row's exclusive time is negative."""
+    mesh = cp.box_case(3, 0.1, str(tmp_path))
+    fe = mp.FESolver(mesh, n0=1e10, phi0=0.0, kTe=mp.KB * 23209.036, wall_potential=0.0, flags=mp.FE_RESET_F1)
+    den = np.full(mesh.n_nodes, 1e10)
+    for method in (mp.FE_NONLINEAR, mp.FE_GAUSS_SEIDEL, mp.FE_DIRECT):
+        fe.compute_phi(den, method)
+    fe.update_ef()
+    prof = fe.profile()
+    assert len(prof) == mp.FE_PROFILE_ENTRIES
+    n = {k: v[0] for k, v in prof.items()}
+    tot = {k: v[1] for k, v in prof.items()}
+    child = {k: v[2] for k, v in prof.items()}
+    assert n["FESolver"] == n["computeNX"] == n["preAssembly"] == 1
+    assert n["computePhi"] == n["solve"] == 3 and n["solveNonLinear"] == 1 and n["solveLinearDirect"] == 1
+    assert n["buildF1Vector"] == n["buildJmatrix"] == n["solveLinear"] + 1 and n["updateEf"] == 1
+    rel = 1e-9
+    assert child["FESolver"] == pytest.approx(tot["computeNX"] + tot["preAssembly"], rel=rel)
+    assert child["computePhi"] == pytest.approx(tot["solve"], rel=rel)
+    assert child["solve"] == pytest.approx(tot["solveNonLinear"] + tot["solveLinearDirect"]
+                                           + (tot["buildF1Vector"] + tot["buildJmatrix"] + tot["solveLinear"]
+                                              - child["solveNonLinear"]), rel=rel)
+    for k in ("computeNX", "preAssembly", "buildF1Vector", "buildJmatrix", "solveLinear", "solveLinearDirect",
+              "updateEf"):
+        assert child[k] == 0.0, k
+    assert all(tot[k] >= child[k] >= 0 for k in prof)
+    assert mp.lib().mp_fe_get_profile(fe._h, mp.FE_PROFILE_ENTRIES, None, None, None, None) == 1
+    assert b"no such entry" in mp.lib().mp_fe_last_error()
