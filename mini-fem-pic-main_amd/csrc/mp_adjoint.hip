@@ -40,7 +40,7 @@ namespace mpd {
 // SCATTER = 2 writes the final step's deterministic records (node, weight) at 4 * pid + v.
 template <int SCATTER>
 __global__ void __launch_bounds__(256)
-k_adj_step(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt, int cap,
+k_adj_step(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt, int cap,
            int32_t* __restrict__ path, double* __restrict__ den, uint32_t* __restrict__ keys = nullptr,
            double* __restrict__ vals = nullptr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -51,10 +51,10 @@ k_adj_step(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
         int32_t c = s.cell[i];
         if (path) path[i] = c;
         if (c >= 0) {
-            const DevTet* t = tets + c;
-            double vx = s.vx[i] + (qm * t->ef[0]) * dt;
-            double vy = s.vy[i] + (qm * t->ef[1]) * dt;
-            double vz = s.vz[i] + (qm * t->ef[2]) * dt;
+            const TetView t = tet_at(tets, c);
+            double vx = s.vx[i] + (qm * t.ex()) * dt;
+            double vy = s.vy[i] + (qm * t.ey()) * dt;
+            double vz = s.vz[i] + (qm * t.ez()) * dt;
             double x = s.x[i] + vx * dt;
             double y = s.y[i] + vy * dt;
             double z = s.z[i] + vz * dt;
@@ -66,7 +66,7 @@ k_adj_step(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                 s.cell[i] = c;
                 ok = true;
                 fc = c;
-                if (SCATTER == 2) put_records(keys, vals, (int64_t)s.pid[i], tets + c, l0, l1, l2, l3);
+                if (SCATTER == 2) put_records(keys, vals, (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
             } else {
                 s.cell[i] = -1;
             }
@@ -102,19 +102,19 @@ k_adj_loss(double* __restrict__ raw, const double* __restrict__ tgt, const doubl
 }
 
 // lambda_x of each particle in the domain after K steps; lambda_v cleared.
-__global__ void k_adj_lambda(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ g,
+__global__ void k_adj_lambda(const TetBlock* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ g,
                              double* __restrict__ lx, double* __restrict__ lv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double a = 0.0, b = 0.0, c3 = 0.0;
     const int32_t c = s.cell[i];
     if (c >= 0) {
-        const DevTet* t = tets + c;
+        const TetView t = tet_at(tets, c);
         for (int q = 0; q < 4; q++) {
-            const double f = g[t->con[q]] / (6.0 * t->vol);
-            a += f * -t->c[q][1];
-            b += f * t->c[q][2];
-            c3 += f * -t->c[q][3];
+            const double f = g[t.con(q)] / (6.0 * t.vol());
+            a += f * -t.c(q, 1);
+            b += f * t.c(q, 2);
+            c3 += f * -t.c(q, 3);
         }
     }
     lx[3 * i + 0] = a; lx[3 * i + 1] = b; lx[3 * i + 2] = c3;

@@ -25,7 +25,7 @@ namespace mpd {
 // from device memory (n_dev) or given (i0, n_host: one run's range).
 template <int WALL, int DET>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
+k_push_batch(const TetBlock* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
              const int64_t* __restrict__ n_dev, const double* __restrict__ par, int32_t nn,
              double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
              int32_t* __restrict__ part, int cap) {
@@ -51,10 +51,10 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
             const bool live = c_in >= 0;
             int32_t c = live ? c_in : 0;
             const double qm = par[3 * r], dt = par[3 * r + 1];
-            const DevTet* t = tets + c;
+            const TetView t = tet_at(tets, c);
             StartTet g0;
             load_start(t, g0);
-            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+            const double ex = t.ex(), ey = t.ey(), ez = t.ez();
             vx = vx + (qm * ex) * dt;
             vy = vy + (qm * ey) * dt;
             vz = vz + (qm * ez) * dt;
@@ -73,7 +73,7 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
                     s.cell[i] = c;
                     ok = true;
                     fc = c;
-                    if (DET) put_records(keys, vals, s.pid[i], tets + c, l0, l1, l2, l3, (uint32_t)base);
+                    if (DET) put_records(keys, vals, s.pid[i], tet_at(tets, c), l0, l1, l2, l3, (uint32_t)base);
                 } else {
                     s.cell[i] = -1;
                     n_lost++;
@@ -108,7 +108,7 @@ k_push_batch(const DevTet* __restrict__ tets, Soa s, int64_t i0, int64_t n_host,
 #endif
 template <int WALL>
 __global__ void __launch_bounds__(MP_BATCH_STEPS_THREADS) __attribute__((amdgpu_waves_per_eu(MP_BATCH_STEPS_WAVES, 8)))
-k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ par, int32_t nn,
+k_push_batch_steps(const TetBlock* __restrict__ tets, Soa s, int64_t n, const double* __restrict__ par, int32_t nn,
                    int32_t R, double* __restrict__ den, int K, int32_t* __restrict__ part, int cap) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
@@ -129,7 +129,7 @@ k_push_batch_steps(const DevTet* __restrict__ tets, Soa s, int64_t n, const doub
     int32_t c = live ? c_in : 0;
     const double qm = par[3 * r], dt = par[3 * r + 1];
     TetRegs t;
-    load_tet_regs(tets + c, t);
+    load_tet_regs(tet_at(tets, c), t);
     __shared__ uint32_t hk[1 << MP_BATCH_STEP_BITS];
     __shared__ double hv[1 << MP_BATCH_STEP_BITS];
     __shared__ int32_t shc[MP_BATCH_STEPS_THREADS / kWave * 6];

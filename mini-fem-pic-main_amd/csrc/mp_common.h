@@ -150,16 +150,16 @@ __device__ __forceinline__ void scatter_lds_nodes(double* __restrict__ den, bool
     }
 }
 
-__device__ __forceinline__ void scatter_lds(const DevTet* __restrict__ tets, double* __restrict__ den,
+__device__ __forceinline__ void scatter_lds(const TetBlock* __restrict__ tets, double* __restrict__ den,
                                             bool ok, int32_t fc, double l0, double l1, double l2,
                                             double l3, int32_t base = 0) {
     int32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
     if (ok) {
-        const DevTet* t = tets + fc;
-        n0 = base + t->con[0];
-        n1 = base + t->con[1];
-        n2 = base + t->con[2];
-        n3 = base + t->con[3];
+        const TetView t = tet_at(tets, fc);
+        n0 = base + t.con(0);
+        n1 = base + t.con(1);
+        n2 = base + t.con(2);
+        n3 = base + t.con(3);
     }
     scatter_lds_nodes(den, ok, n0, n1, n2, n3, l0, l1, l2, l3);
 }
@@ -183,27 +183,27 @@ __device__ __forceinline__ void put_records_nodes(uint32_t* __restrict__ keys, d
 }
 
 __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double* __restrict__ vals,
-                                            int64_t pid, const DevTet* __restrict__ tf, double l0,
+                                            int64_t pid, const TetView tf, double l0,
                                             double l1, double l2, double l3, uint32_t base = 0) {
-    put_records_nodes(keys, vals, pid, base + (uint32_t)tf->con[0], base + (uint32_t)tf->con[1],
-                      base + (uint32_t)tf->con[2], base + (uint32_t)tf->con[3], l0, l1, l2, l3);
+    put_records_nodes(keys, vals, pid, base + (uint32_t)tf.con(0), base + (uint32_t)tf.con(1),
+                      base + (uint32_t)tf.con(2), base + (uint32_t)tf.con(3), l0, l1, l2, l3);
 }
 
 // The atomic scatter a kernel's template arguments select (nothing when DET: the records are
 // written where each walk ends): AGG 0 one atomic per weight, 2 LDS hash per workgroup (every
 // thread of the block calls it).
 template <int DET, int AGG>
-__device__ __forceinline__ void scatter_any(const DevTet* __restrict__ tets, double* __restrict__ den, bool ok,
+__device__ __forceinline__ void scatter_any(const TetBlock* __restrict__ tets, double* __restrict__ den, bool ok,
                                             int32_t fc, double l0, double l1, double l2, double l3) {
     if (DET) return;
     if (AGG == 2) {
         scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
     } else if (ok) {
-        const DevTet* t = tets + fc;
-        unsafeAtomicAdd(den + t->con[0], l0);
-        unsafeAtomicAdd(den + t->con[1], l1);
-        unsafeAtomicAdd(den + t->con[2], l2);
-        unsafeAtomicAdd(den + t->con[3], l3);
+        const TetView t = tet_at(tets, fc);
+        unsafeAtomicAdd(den + t.con(0), l0);
+        unsafeAtomicAdd(den + t.con(1), l1);
+        unsafeAtomicAdd(den + t.con(2), l2);
+        unsafeAtomicAdd(den + t.con(3), l3);
     }
 }
 
@@ -278,7 +278,7 @@ __device__ __forceinline__ void steps_finish(uint32_t* hk, double* hv, double* _
 // push, in the same order.  Step k's weights go to key0 + k * per_step + node: the LDS hash
 // (HASH) or direct atomics on den.  Counters as in k_push_soa.
 template <int WALL, bool HASH, int BITS = kStepBits>
-__device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetRegs& t, int32_t& c, bool& live,
+__device__ __forceinline__ void lane_steps(const TetBlock* __restrict__ tets, TetRegs& t, int32_t& c, bool& live,
                                            double& x, double& y, double& z, double& vx, double& vy, double& vz,
                                            double qm, double dt, int K, int cap, double* __restrict__ den,
                                            uint32_t key0, uint32_t per_step, uint32_t* hk, double* hv, int& n_act,
@@ -338,7 +338,7 @@ __device__ __forceinline__ void lane_steps(const DevTet* __restrict__ tets, TetR
         if (!capped) st.moves++;
         if (!capped && nb >= 0) {
             c = nb;
-            load_tet_regs(tets + nb, t);
+            load_tet_regs(tet_at(tets, nb), t);
             continue;
         }
         if (capped || WALL == 0) {

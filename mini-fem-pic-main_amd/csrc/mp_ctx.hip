@@ -439,26 +439,28 @@ int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, i
     c->n_tets = n_tets;
     c->n_nodes = n_nodes;
     c->opt = default_options();
-    std::vector<DevTet> h((size_t)n_tets);
+    // the device table (layout in mp_device.h); E starts at zero, padding tets are all zero
+    const int64_t n_blk = mpd::tet_blocks(n_tets);
+    std::vector<TetBlock> h((size_t)n_blk);
+    std::memset((void*)h.data(), 0, sizeof(TetBlock) * h.size());
     for (int32_t e = 0; e < n_tets; e++) {
-        DevTet& d = h[e];
-        std::memset(&d, 0, sizeof(d));
+        mpd::TetChunk* q = mpd::tet_chunks(h.data(), e);
         for (int v = 0; v < 4; v++) {
-            d.c[v][0] = tets[e].alpha[v];
-            d.c[v][1] = tets[e].beta[v];
-            d.c[v][2] = tets[e].gamma[v];
-            d.c[v][3] = tets[e].delta[v];
-            d.nbr[v] = tets[e].cell_con[v];
-            d.con[v] = tets[e].con[v];
+            q[(2 * v) * mpd::kTetBlock].d[0] = tets[e].alpha[v];
+            q[(2 * v) * mpd::kTetBlock].d[1] = tets[e].beta[v];
+            q[(2 * v + 1) * mpd::kTetBlock].d[0] = tets[e].gamma[v];
+            q[(2 * v + 1) * mpd::kTetBlock].d[1] = tets[e].delta[v];
+            q[mpd::kChNbr * mpd::kTetBlock].i[v] = tets[e].cell_con[v];
+            q[mpd::kChCon * mpd::kTetBlock].i[v] = tets[e].con[v];
         }
-        d.vol = tets[e].volume;
+        q[mpd::kChVolEx * mpd::kTetBlock].d[0] = tets[e].volume;
     }
     auto bail = [&](const std::string& m) {
         mp_destroy(c);
         return set_err(nullptr, MP_ERR_ALLOC, m);
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail("mp_create: stream creation failed");
-    if (dalloc(&c->tets, n_tets) != hipSuccess || dalloc(&c->node_vol, n_nodes) != hipSuccess ||
+    if (dalloc(&c->tets, n_blk) != hipSuccess || dalloc(&c->node_vol, n_nodes) != hipSuccess ||
         dalloc(&c->den_raw, n_nodes) != hipSuccess || dalloc(&c->den_out, n_nodes) != hipSuccess ||
         dalloc(&c->ctr, mpd::kNumCounters) != hipSuccess || dalloc(&c->cell_cnt, (int64_t)n_tets + 1) != hipSuccess ||
         dalloc(&c->cell_cur, (int64_t)n_tets + 1) != hipSuccess)
@@ -474,7 +476,7 @@ int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, i
                                                     hipGetErrorString(e));
         }
     }
-    if (hipMemcpy(c->tets, h.data(), sizeof(DevTet) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(c->tets, h.data(), sizeof(TetBlock) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->node_vol, node_volume, sizeof(double) * (size_t)n_nodes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->den_raw, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||
         hipMemset(c->den_out, 0, sizeof(double) * (size_t)n_nodes) != hipSuccess ||

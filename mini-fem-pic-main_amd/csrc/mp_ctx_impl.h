@@ -8,14 +8,14 @@
 #include "mp_launch.h"
 #include "mp_staging.h"
 
-using mpd::DevTet;
+using mpd::TetBlock;
 using mpd::Soa;
 
 struct mp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int32_t n_tets = 0, n_nodes = 0;
-    DevTet* tets = nullptr;
+    TetBlock* tets = nullptr;  // tet_blocks(n_tets) blocks
     double* node_vol = nullptr;
     double* den_raw = nullptr;
     double* den_out = nullptr;

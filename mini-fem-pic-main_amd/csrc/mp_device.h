@@ -1,10 +1,19 @@
 // Device-side data layout and the per-particle arithmetic shared by every kernel.
 //
-// HBM layout of a tet (192 B, 64 B aligned, one record per tet, gathered by cell index):
-//   c[i][0..3] = alpha_i, beta_i, gamma_i, delta_i of vertex i   (128 B)
-//   vol, nbr[4] (= Tetra::cell_con), con[4], ef[3]              ( 64 B)
-// Everything the walk, the push (start-cell E) and the scatter (node ids) need for one
-// tet is in one record, i.e. one or two 128 B lines per gather.
+// HBM layout of the tet table.  A tet's record is 12 chunks of 16 B:
+//   chunks 0-7  alpha_i, beta_i | gamma_i, delta_i of vertex i (two chunks per vertex)
+//   chunk  8    vol, ef[0]
+//   chunk  9    ef[1], ef[2]
+//   chunk 10    nbr[4] (= Tetra::cell_con)
+//   chunk 11    con[4]
+// Records are grouped in blocks of kTetBlock consecutive tets, chunk-major inside a block:
+// chunk k of tet j of the block at k * 16 * kTetBlock + j * 16.  The default, one tet per
+// block, is one 192 B record per tet: what the walk, the push (start-cell E) and the scatter
+// (node ids) need for one tet in one or two 128 B lines.  Blocks of 8 tets (a cell-sorted
+// wave's gather instruction then touches one line instead of one per lane) measured no faster
+// for the single-step push at 20M particles on 60^3 and 7% slower for the multi-step kernel,
+// which reloads whole records at random as particles move: the push waits on the latency of
+// its dependent gathers, not on the lookups.
 //
 // Arithmetic follows the reference kernel with no contraction (built -ffp-contract=off):
 //   push   src/mini-pic.cpp:291-295   v = v + (qm*E)*dt ; x = x + v*dt
@@ -17,14 +26,51 @@
 
 namespace mpd {
 
-struct alignas(64) DevTet {
-    double c[4][4];  // per vertex: alpha, beta, gamma, delta
-    double vol;
-    int32_t nbr[4];
-    int32_t con[4];
-    double ef[3];
+#ifndef MP_TET_BLOCK
+#define MP_TET_BLOCK 1
+#endif
+constexpr int kTetBlock = MP_TET_BLOCK;
+static_assert(kTetBlock >= 1 && (kTetBlock & (kTetBlock - 1)) == 0, "tets per block: a power of two");
+
+struct alignas(16) TetChunk {
+    union {
+        double d[2];
+        int32_t i[4];
+    };
 };
-static_assert(sizeof(DevTet) == 192, "DevTet must stay 192 B");
+
+struct TetBlock {
+    TetChunk ch[12][kTetBlock];  // ch[k][j]: chunk k of tet j of the block
+};
+static_assert(sizeof(TetBlock) == 192 * kTetBlock, "192 B per tet");
+
+constexpr int kChVolEx = 8, kChEyEz = 9, kChNbr = 10, kChCon = 11;
+
+__host__ __device__ inline int64_t tet_blocks(int64_t n_tets) { return (n_tets + kTetBlock - 1) / kTetBlock; }
+
+// One tet's record in the table (read side): c(v, f) is coefficient f (alpha, beta, gamma,
+// delta) of vertex v.
+struct TetView {
+    const TetChunk* q;  // chunk 0; chunk k at q[k * kTetBlock]
+    __device__ __forceinline__ double c(int v, int f) const { return q[(2 * v + (f >> 1)) * kTetBlock].d[f & 1]; }
+    __device__ __forceinline__ double vol() const { return q[kChVolEx * kTetBlock].d[0]; }
+    __device__ __forceinline__ double ex() const { return q[kChVolEx * kTetBlock].d[1]; }
+    __device__ __forceinline__ double ey() const { return q[kChEyEz * kTetBlock].d[0]; }
+    __device__ __forceinline__ double ez() const { return q[kChEyEz * kTetBlock].d[1]; }
+    __device__ __forceinline__ int32_t nbr(int v) const { return q[kChNbr * kTetBlock].i[v]; }
+    __device__ __forceinline__ int32_t con(int v) const { return q[kChCon * kTetBlock].i[v]; }
+};
+
+// Tet e's chunk 0, writable (the table's builder and the field upload); chunk k at [k * kTetBlock].
+__host__ __device__ inline TetChunk* tet_chunks(TetBlock* tets, int64_t e) {
+    return &tets[e / kTetBlock].ch[0][e % kTetBlock];
+}
+
+// The record of tet c >= 0.
+__device__ __forceinline__ TetView tet_at(const TetBlock* __restrict__ tets, int32_t c) {
+    const uint32_t u = (uint32_t)c;
+    return TetView{&tets[u / kTetBlock].ch[0][u % kTetBlock]};
+}
 
 // Structure-of-arrays particle state in HBM.
 struct Soa {
@@ -61,17 +107,17 @@ struct WalkStats {
 // fields are read where they are used rather than staged in registers, which keeps the
 // loop's live registers few.
 template <int WALL>
-__device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c, double& x,
+__device__ __forceinline__ int walk(const TetBlock* __restrict__ tets, int32_t& c, double& x,
                                     double& y, double& z, double& vx, double& vy, double& vz,
                                     double& l0, double& l1, double& l2, double& l3, int cap,
                                     WalkStats& st) {
     while (true) {
-        const DevTet* t = tets + c;
-        const double vol = t->vol;
-        l0 = bary(t->c[0][0], t->c[0][1], t->c[0][2], t->c[0][3], x, y, z, vol);
-        l1 = bary(t->c[1][0], t->c[1][1], t->c[1][2], t->c[1][3], x, y, z, vol);
-        l2 = bary(t->c[2][0], t->c[2][1], t->c[2][2], t->c[2][3], x, y, z, vol);
-        l3 = bary(t->c[3][0], t->c[3][1], t->c[3][2], t->c[3][3], x, y, z, vol);
+        const TetView t = tet_at(tets, c);
+        const double vol = t.vol();
+        l0 = bary(t.c(0, 0), t.c(0, 1), t.c(0, 2), t.c(0, 3), x, y, z, vol);
+        l1 = bary(t.c(1, 0), t.c(1, 1), t.c(1, 2), t.c(1, 3), x, y, z, vol);
+        l2 = bary(t.c(2, 0), t.c(2, 1), t.c(2, 2), t.c(2, 3), x, y, z, vol);
+        l3 = bary(t.c(3, 0), t.c(3, 1), t.c(3, 2), t.c(3, 3), x, y, z, vol);
         const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
                          (l3 < 0 || l3 > 1.0);
         if (!out) return kInside;
@@ -80,7 +126,7 @@ __device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c,
         if (l1 < ml) { ml = l1; mi = 1; }
         if (l2 < ml) { ml = l2; mi = 2; }
         if (l3 < ml) { ml = l3; mi = 3; }
-        const int32_t nb = t->nbr[mi];
+        const int32_t nb = t.nbr(mi);
         if (st.moves >= cap) return kCapped;
         st.moves++;
         if (nb >= 0) {
@@ -90,7 +136,7 @@ __device__ __forceinline__ int walk(const DevTet* __restrict__ tets, int32_t& c,
         if (WALL == 0) return kLost;
         // Specular reflection about the plane lc_mi = 0.  grad lc_mi is proportional to
         // g = (-beta, gamma, -delta) / (6 V); x' = x - 2 lc g (6V)/|g|^2, v' = v - 2 (v.g) g/|g|^2.
-        const double gx = -t->c[mi][1], gy = t->c[mi][2], gz = -t->c[mi][3];
+        const double gx = -t.c(mi, 1), gy = t.c(mi, 2), gz = -t.c(mi, 3);
         const double gg = (gx * gx + gy * gy) + gz * gz;
         const double s = ((2.0 * ml) * (6.0 * vol)) / gg;
         x = x - s * gx;
@@ -112,13 +158,13 @@ struct StartTet {
     int32_t n0, n1, n2, n3;
 };
 
-__device__ __forceinline__ void load_start(const DevTet* __restrict__ t, StartTet& s) {
-    s.a0 = t->c[0][0]; s.b0 = t->c[0][1]; s.g0 = t->c[0][2]; s.d0 = t->c[0][3];
-    s.a1 = t->c[1][0]; s.b1 = t->c[1][1]; s.g1 = t->c[1][2]; s.d1 = t->c[1][3];
-    s.a2 = t->c[2][0]; s.b2 = t->c[2][1]; s.g2 = t->c[2][2]; s.d2 = t->c[2][3];
-    s.a3 = t->c[3][0]; s.b3 = t->c[3][1]; s.g3 = t->c[3][2]; s.d3 = t->c[3][3];
-    s.vol = t->vol;
-    s.n0 = t->nbr[0]; s.n1 = t->nbr[1]; s.n2 = t->nbr[2]; s.n3 = t->nbr[3];
+__device__ __forceinline__ void load_start(const TetView t, StartTet& s) {
+    s.a0 = t.c(0, 0); s.b0 = t.c(0, 1); s.g0 = t.c(0, 2); s.d0 = t.c(0, 3);
+    s.a1 = t.c(1, 0); s.b1 = t.c(1, 1); s.g1 = t.c(1, 2); s.d1 = t.c(1, 3);
+    s.a2 = t.c(2, 0); s.b2 = t.c(2, 1); s.g2 = t.c(2, 2); s.d2 = t.c(2, 3);
+    s.a3 = t.c(3, 0); s.b3 = t.c(3, 1); s.g3 = t.c(3, 2); s.d3 = t.c(3, 3);
+    s.vol = t.vol();
+    s.n0 = t.nbr(0); s.n1 = t.nbr(1); s.n2 = t.nbr(2); s.n3 = t.nbr(3);
 }
 
 // The first iteration of XtoLtet on the start tet's fields s (already in registers).  Returns
@@ -160,7 +206,7 @@ __device__ __forceinline__ int first_step(const StartTet& s, int32_t& c, double 
 
 // XtoLtet from cell c whose start-tet fields s are already in registers.
 template <int WALL>
-__device__ __forceinline__ int walk_start(const DevTet* __restrict__ tets, const StartTet& s, int32_t& c,
+__device__ __forceinline__ int walk_start(const TetBlock* __restrict__ tets, const StartTet& s, int32_t& c,
                                           double& x, double& y, double& z, double& vx, double& vy,
                                           double& vz, double& l0, double& l1, double& l2, double& l3,
                                           int cap, WalkStats& st) {
@@ -177,22 +223,22 @@ struct TetRegs {
     int32_t m0, m1, m2, m3;
 };
 
-__device__ __forceinline__ void load_tet_regs(const DevTet* __restrict__ t, TetRegs& r) {
+__device__ __forceinline__ void load_tet_regs(const TetView t, TetRegs& r) {
     load_start(t, r.g);
-    r.ex = t->ef[0];
-    r.ey = t->ef[1];
-    r.ez = t->ef[2];
-    r.m0 = t->con[0];
-    r.m1 = t->con[1];
-    r.m2 = t->con[2];
-    r.m3 = t->con[3];
+    r.ex = t.ex();
+    r.ey = t.ey();
+    r.ez = t.ez();
+    r.m0 = t.con(0);
+    r.m1 = t.con(1);
+    r.m2 = t.con(2);
+    r.m3 = t.con(3);
 }
 
 // XtoLtet from cell c whose record t is in registers; each move loads the next tet's whole
 // record into t, so on return t is the record of the final tet c.  Same decisions, moves and
 // reflections as first_step + walk (cap >= 1).
 template <int WALL>
-__device__ __forceinline__ int walk_regs(const DevTet* __restrict__ tets, TetRegs& t, int32_t& c, double& x,
+__device__ __forceinline__ int walk_regs(const TetBlock* __restrict__ tets, TetRegs& t, int32_t& c, double& x,
                                          double& y, double& z, double& vx, double& vy, double& vz, double& l0,
                                          double& l1, double& l2, double& l3, int cap, WalkStats& st) {
     while (true) {
@@ -214,7 +260,7 @@ __device__ __forceinline__ int walk_regs(const DevTet* __restrict__ tets, TetReg
         st.moves++;
         if (nb >= 0) {
             c = nb;
-            load_tet_regs(tets + nb, t);
+            load_tet_regs(tet_at(tets, nb), t);
             continue;
         }
         if (WALL == 0) return kLost;

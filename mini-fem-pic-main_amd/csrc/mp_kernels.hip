@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
 // round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).
 template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
-k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
+k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
            int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot) {
     const int ppt = MULTI ? ppt_arg : 1;
@@ -73,10 +73,10 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
             __builtin_amdgcn_sched_barrier(0);
             const bool live = c_in >= 0;
             int32_t c = live ? c_in : 0;
-            const DevTet* t = tets + c;
+            const TetView t = tet_at(tets, c);
             StartTet g0;
             load_start(t, g0);
-            const double ex = t->ef[0], ey = t->ef[1], ez = t->ef[2];
+            const double ex = t.ex(), ey = t.ey(), ez = t.ez();
             vx = vx + (qm * ex) * dt;
             vy = vy + (qm * ey) * dt;
             vz = vz + (qm * ez) * dt;
@@ -94,7 +94,7 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
                     s.cell[i] = c;
                     ok = true;
                     fc = c;
-                    if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tets + c, l0, l1, l2, l3);
+                    if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
                 } else {
                     s.cell[i] = -1;
                     n_lost++;
@@ -135,7 +135,7 @@ k_push_soa(const DevTet* __restrict__ tets, Soa s, int64_t n, double qm, double 
 // i of s, so the sorted order costs a 4 B index instead of a 56 B record copy.
 template <int WALL, int AGG, bool GATHER>
 __global__ void __launch_bounds__(MP_STEPS_THREADS) __attribute__((amdgpu_waves_per_eu(MP_STEPS_WAVES, 8)))
-k_push_steps(const DevTet* __restrict__ tets, Soa s, Soa src, const int32_t* __restrict__ perm, int64_t n,
+k_push_steps(const TetBlock* __restrict__ tets, Soa s, Soa src, const int32_t* __restrict__ perm, int64_t n,
              double qm, double dt, double* __restrict__ den, int32_t nn, int K, int32_t* __restrict__ part,
              int cap) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -158,7 +158,7 @@ k_push_steps(const DevTet* __restrict__ tets, Soa s, Soa src, const int32_t* __r
     bool live = c_in >= 0;
     int32_t c = live ? c_in : 0;
     TetRegs t;
-    load_tet_regs(tets + c, t);
+    load_tet_regs(tet_at(tets, c), t);
     if (GATHER && i < n) s.pid[i] = pid;
     constexpr bool kHash = AGG != 0;
     __shared__ uint32_t hk[kHash ? (1 << MP_STEPS_BITS) : 1];
@@ -199,7 +199,7 @@ __global__ void k_normalise_steps(double* __restrict__ den, const double* __rest
 // Absorbing walls, lc written back as the reference kernel does; pid = array index.
 template <int DET>
 __global__ void __launch_bounds__(256)
-k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int64_t n,
+k_push_aos(const TetBlock* __restrict__ tets, mp_particle* __restrict__ parts, int64_t n,
            double qm, double dt, double* __restrict__ den, uint32_t* __restrict__ keys,
            double* __restrict__ vals, int32_t* __restrict__ part, int cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,10 +209,10 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
         mp_particle* p = parts + i;
         int32_t c = p->cell_index;
         act = true;
-        const DevTet* t = tets + c;
-        double vx = p->vel[0] + (qm * t->ef[0]) * dt;
-        double vy = p->vel[1] + (qm * t->ef[1]) * dt;
-        double vz = p->vel[2] + (qm * t->ef[2]) * dt;
+        const TetView t = tet_at(tets, c);
+        double vx = p->vel[0] + (qm * t.ex()) * dt;
+        double vy = p->vel[1] + (qm * t.ey()) * dt;
+        double vz = p->vel[2] + (qm * t.ez()) * dt;
         double x = p->pos[0] + vx * dt;
         double y = p->pos[1] + vy * dt;
         double z = p->pos[2] + vz * dt;
@@ -224,14 +224,14 @@ k_push_aos(const DevTet* __restrict__ tets, mp_particle* __restrict__ parts, int
         p->lc[0] = l0; p->lc[1] = l1; p->lc[2] = l2; p->lc[3] = l3;
         if (r == kInside) {
             p->cell_index = c;
-            const DevTet* tf = tets + c;
+            const TetView tf = tet_at(tets, c);
             if (DET) {
                 put_records(keys, vals, i, tf, l0, l1, l2, l3);
             } else {
-                unsafeAtomicAdd(den + tf->con[0], l0);
-                unsafeAtomicAdd(den + tf->con[1], l1);
-                unsafeAtomicAdd(den + tf->con[2], l2);
-                unsafeAtomicAdd(den + tf->con[3], l3);
+                unsafeAtomicAdd(den + tf.con(0), l0);
+                unsafeAtomicAdd(den + tf.con(1), l1);
+                unsafeAtomicAdd(den + tf.con(2), l2);
+                unsafeAtomicAdd(den + tf.con(3), l3);
             }
         } else {
             p->cell_index = -1;
@@ -794,12 +794,13 @@ __global__ void k_mark_pid(const int32_t* __restrict__ cell, const int32_t* __re
 
 // ---- field / particle staging --------------------------------------------------------
 
-__global__ void k_set_field(DevTet* __restrict__ tets, const double* __restrict__ ef, int32_t nt) {
+__global__ void k_set_field(TetBlock* __restrict__ tets, const double* __restrict__ ef, int32_t nt) {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nt) return;
-    tets[e].ef[0] = ef[3 * (int64_t)e + 0];
-    tets[e].ef[1] = ef[3 * (int64_t)e + 1];
-    tets[e].ef[2] = ef[3 * (int64_t)e + 2];
+    TetChunk* q = tet_chunks(tets, e);
+    q[kChVolEx * kTetBlock].d[1] = ef[3 * (int64_t)e + 0];
+    q[kChEyEz * kTetBlock].d[0] = ef[3 * (int64_t)e + 1];
+    q[kChEyEz * kTetBlock].d[1] = ef[3 * (int64_t)e + 2];
 }
 
 __global__ void k_unpack(const double* __restrict__ pos, const double* __restrict__ vel,
@@ -906,7 +907,7 @@ int64_t tally_ints(int64_t n, int block) {
     return ((4 * n + kWave - 1) / kWave + kXcds * (256 / kWave)) * kTally;
 }
 
-hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
+hipError_t launch_push_aos(const TetBlock* tets, mp_particle* parts, int64_t n, double qm, double dt,
                            double* den, uint32_t* keys, double* vals, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st) {
     if (n <= 0) return hipSuccess;
@@ -1079,7 +1080,7 @@ hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* o
     return hipGetLastError();
 }
 
-hipError_t launch_set_field(DevTet* tets, const double* ef, int32_t nt, hipStream_t st) {
+hipError_t launch_set_field(TetBlock* tets, const double* ef, int32_t nt, hipStream_t st) {
     hipLaunchKernelGGL(k_set_field, dim3(grid_for(nt, 256)), dim3(256), 0, st, tets, ef, nt);
     return hipGetLastError();
 }

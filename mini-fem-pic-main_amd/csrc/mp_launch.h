@@ -19,7 +19,7 @@ enum CounterIndex : int {
 };
 
 struct PushArgs {
-    const DevTet* tets;
+    const TetBlock* tets;
     Soa soa;
     int64_t n;
     double qm;
@@ -50,7 +50,7 @@ hipError_t launch_push_steps(const PushArgs& a, int32_t nn, int k_steps, hipStre
 int steps_block();
 hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, int k_steps, double spwt,
                                   hipStream_t st);
-hipError_t launch_push_aos(const DevTet* tets, mp_particle* parts, int64_t n, double qm, double dt,
+hipError_t launch_push_aos(const TetBlock* tets, mp_particle* parts, int64_t n, double qm, double dt,
                            double* den, uint32_t* keys, double* vals, int32_t* part,
                            unsigned long long* ctr, int cap, bool det, hipStream_t st);
 // Step counters: the push kernels leave per-block partials; this folds them into ctr.
@@ -91,7 +91,7 @@ hipError_t launch_check_cells_aos(const mp_particle* p, int64_t n, int32_t n_tet
                                   hipStream_t st);
 hipError_t launch_compact_aos(const mp_particle* in, int64_t n, const int32_t* off, mp_particle* out,
                               hipStream_t st);
-hipError_t launch_set_field(DevTet* tets, const double* ef, int32_t nt, hipStream_t st);
+hipError_t launch_set_field(TetBlock* tets, const double* ef, int32_t nt, hipStream_t st);
 hipError_t launch_unpack(const double* pos, const double* vel, const int32_t* cell, int64_t n, int64_t pid0,
                          const Soa& s, int64_t at, hipStream_t st);
 hipError_t launch_pack(const Soa& s, int64_t n, double* pos, double* vel, hipStream_t st);
