@@ -281,25 +281,17 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
                         f"no drift, reflecting walls, sort every 8 steps (shift {shift}), dt {dt:g} s"}
 
 
-def run_coupled(steps=200):
-    """mini-pic's coupled loop on a 12^3 six-tet box of side 0.1 m (10,368 tets, 2,197 nodes):
-    O+ ions injected at 7 km/s through the x = 0 face (n = 1e10 m^-3, spwt 200: 1750 a step),
-    Boltzmann electrons at Te = 2 eV, dt = 5e-7 s; each step InjectIons (host) -> MoveParticles
-    (GPU drop-in, deterministic scatter) -> FE Poisson-Boltzmann (host, Newton + Gauss-Seidel,
-    F1 cleared per solve) -> updateEf.  Also runs the reference's own solver setting (F1
-    accumulating) until its Newton iteration stops converging, as the reference's would."""
-    import tempfile
+ION_TEMPERATURE_1EV = 11604.518121745585  # kelvin: 1 eV / k_B
 
+
+def coupled_loop(mesh, p, steps, resident):
+    """One coupled run of `steps` steps with the ions moved by the drop-in GPU mover (host
+    particle array) or kept resident on the device; times per phase."""
     import coupled as cp
 
-    with tempfile.TemporaryDirectory() as d:
-        mesh = cp.box_case(12, 0.1, d)
-    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
-                      dt=5e-7, max_iter=steps)
-    out = {}
-    gpu = cp.GpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
+    mover = (cp.ResidentGpuMover if resident else cp.GpuMover)(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
     try:
-        run = cp.CoupledRun(mesh, p, move=gpu, fe_flags=mp.FE_RESET_F1)
+        run = cp.CoupledRun(mesh, p, move=mover, fe_flags=mp.FE_RESET_F1)
         t_inj = t_move = t_fe = 0.0
         psteps = 0
         t0 = time.perf_counter()
@@ -313,25 +305,48 @@ def run_coupled(steps=200):
             run.solve()
             t_inj, t_move, t_fe = t_inj + b - a, t_move + c - b, t_fe + time.perf_counter() - c
         wall = time.perf_counter() - t0
-        out = {"steps": steps, "particles_final": run.n_parts, "ms_per_step": 1e3 * wall / steps,
-               "inject_ms_per_step": 1e3 * t_inj / steps, "move_ms_per_step": 1e3 * t_move / steps,
-               "fe_ms_per_step": 1e3 * t_fe / steps, "move_particle_steps_per_s": psteps / t_move}
-        # the same loop with the ions resident on the device
-        res = cp.ResidentGpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
-        try:
-            rrun = cp.CoupledRun(mesh, p, move=res, fe_flags=mp.FE_RESET_F1)
-            t_move_r = 0.0
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                rrun.inject()
-                c = time.perf_counter()
-                rrun.move_step()
-                t_move_r += time.perf_counter() - c
-                rrun.solve()
-            out["resident_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
-            out["resident_move_ms_per_step"] = 1e3 * t_move_r / steps
-        finally:
-            res.close()
+        return {"ms_per_step": 1e3 * wall / steps, "inject_ms_per_step": 1e3 * t_inj / steps,
+                "move_ms_per_step": 1e3 * t_move / steps, "fe_ms_per_step": 1e3 * t_fe / steps,
+                "move_particle_steps_per_s": psteps / t_move, "particles_final": run.n_parts}
+    finally:
+        mover.close()
+
+
+def run_coupled(steps=200):
+    """mini-pic's coupled loop on a 12^3 six-tet box of side 0.1 m (10,368 tets, 2,197 nodes):
+    O+ ions injected at 7 km/s through the x = 0 face (n = 1e10 m^-3, spwt 200: 1750 a step),
+    Boltzmann electrons at Te = 2 eV, dt = 5e-7 s; each step InjectIons (host) -> MoveParticles
+    (GPU, deterministic scatter) -> FE Poisson-Boltzmann (host, Newton + Gauss-Seidel, F1
+    cleared per solve) -> updateEf.  Run with the reference's cold beam (the top-level fields)
+    and with the workload's 1 eV Maxwellian spread on the drift ("thermal_1eV"), each with the
+    drop-in mover and the device-resident one.  Also runs the reference's own solver setting
+    (F1 accumulating) until its Newton iteration stops converging, as the reference's would."""
+    import tempfile
+
+    import coupled as cp
+
+    with tempfile.TemporaryDirectory() as d:
+        mesh = cp.box_case(12, 0.1, d)
+    out = {"steps": steps}
+    for name, temp in (("cold_beam", 0.0), ("thermal_1eV", ION_TEMPERATURE_1EV)):
+        p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                          dt=5e-7, max_iter=steps, ion_temperature=temp)
+        r = coupled_loop(mesh, p, steps, resident=False)
+        rr = coupled_loop(mesh, p, steps, resident=True)
+        r["resident_ms_per_step"] = rr["ms_per_step"]
+        r["resident_move_ms_per_step"] = rr["move_ms_per_step"]
+        r["ion_temperature_K"] = temp
+        if name == "cold_beam":
+            out.update(r)
+        else:
+            out[name] = r
+        log(f"coupled ({name}): {r['ms_per_step']:.2f} ms/step (move {r['move_ms_per_step']:.2f}, FE "
+            f"{r['fe_ms_per_step']:.2f}), resident {r['resident_ms_per_step']:.2f} ms/step (move "
+            f"{r['resident_move_ms_per_step']:.2f}), {r['particles_final']} ions at the end")
+    p = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036, wall_potential=0.0,
+                      dt=5e-7, max_iter=steps)
+    gpu = cp.GpuMover(mesh, QM, p.dt, 2e2, mp.SCATTER_DETERMINISTIC)
+    try:
         run = cp.CoupledRun(mesh, p, move=gpu)
         fail = None
         for ts in range(steps):
@@ -343,11 +358,10 @@ def run_coupled(steps=200):
         out["reference_solver_newton_fails_at_step"] = fail
     finally:
         gpu.close()
-    log(f"coupled: {out['ms_per_step']:.2f} ms/step (move {out['move_ms_per_step']:.2f}, FE {out['fe_ms_per_step']:.2f}),"
-        f" resident {out['resident_ms_per_step']:.2f} ms/step (move {out['resident_move_ms_per_step']:.2f}),"
-        f" {out['particles_final']} ions at the end; reference-F1 Newton fails at step {out['reference_solver_newton_fails_at_step']}")
-    out["workload"] = ("12^3 six-tet box, side 0.1 m, O+ beam 7 km/s from x = 0, n 1e10 m^-3, Te 2 eV, dt 5e-7 s, "
-                       "GPU drop-in MoveParticles (deterministic) + host FE Newton/Gauss-Seidel, F1 cleared per solve")
+    log(f"coupled: reference-F1 Newton fails at step {out['reference_solver_newton_fails_at_step']}")
+    out["workload"] = ("12^3 six-tet box, side 0.1 m, O+ beam 7 km/s from x = 0 (cold, and with a 1 eV Maxwellian "
+                       "spread), n 1e10 m^-3, Te 2 eV, dt 5e-7 s, GPU MoveParticles (deterministic) + host FE "
+                       "Newton/Gauss-Seidel, F1 cleared per solve")
     return out
 
 
@@ -445,18 +459,19 @@ def run_cpu_baseline(mesh, ef, pos, vel, cell, budget_s, max_steps=30, sample=2_
 
         with tempfile.TemporaryDirectory() as d:
             cmesh = cp.box_case(12, 0.1, d)
-        prm = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036,
-                            wall_potential=0.0, dt=5e-7, max_iter=coupled_steps)
-        run = cp.CoupledRun(cmesh, prm, move=oracle.AosMover(cmesh.tets, cmesh.node_volume, QM, prm.dt, 2e2),
-                            fe_flags=mp.FE_RESET_F1)
-        t_move = 0.0
-        t0 = time.perf_counter()
-        for _ in range(coupled_steps):
-            run.inject()
-            t_move += run.move_step()[1]
-            run.solve()
-        out["coupled_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / coupled_steps
-        out["coupled_move_ms_per_step"] = 1e3 * t_move / coupled_steps
+        for key, temp in (("coupled", 0.0), ("coupled_thermal_1eV", ION_TEMPERATURE_1EV)):
+            prm = cp.Parameters(plasma_den=1e10, ion_velocity=7000.0, electron_temperature=23209.036,
+                                wall_potential=0.0, dt=5e-7, max_iter=coupled_steps, ion_temperature=temp)
+            run = cp.CoupledRun(cmesh, prm, move=oracle.AosMover(cmesh.tets, cmesh.node_volume, QM, prm.dt, 2e2),
+                                fe_flags=mp.FE_RESET_F1)
+            t_move = 0.0
+            t0 = time.perf_counter()
+            for _ in range(coupled_steps):
+                run.inject()
+                t_move += run.move_step()[1]
+                run.solve()
+            out[f"{key}_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / coupled_steps
+            out[f"{key}_move_ms_per_step"] = 1e3 * t_move / coupled_steps
         out["coupled_steps"] = coupled_steps
     return out
 
@@ -585,10 +600,12 @@ def main():
                "single_thread_sample": f"first {cb['sample']} particles, 2 steps; the full MoveParticles line adds "
                                        "the 2 x 88 B record copies, the stable erase and the normalisation"}
         if "coupled_ms_per_step" in cb:
-            cpu["coupled_ms_per_step"] = cb["coupled_ms_per_step"]
-            cpu["coupled_move_ms_per_step"] = cb["coupled_move_ms_per_step"]
+            for k in ("coupled_ms_per_step", "coupled_move_ms_per_step", "coupled_thermal_1eV_ms_per_step",
+                      "coupled_thermal_1eV_move_ms_per_step"):
+                cpu[k] = cb[k]
             cpu["coupled_sample"] = (f"the 12^3 coupled loop, {cb['coupled_steps']} steps, with the oracle's serial "
-                                     "MoveParticles (one thread) and the same host injection and FE solver")
+                                     "MoveParticles (one thread) and the same host injection and FE solver, cold beam "
+                                     "and 1 eV thermal injection")
             log(f"cpu coupled loop: {cb['coupled_ms_per_step']:.2f} ms/step (move {cb['coupled_move_ms_per_step']:.2f})")
         log(f"cpu baseline: {cb['value'] / 1e6:.2f} M particle-steps/s on {cb['threads']} threads")
     c = head["counters"]
