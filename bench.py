@@ -399,11 +399,25 @@ def run_adjoint(steps=50, n=2_000_000, seed=SEED + 4, reps=3):
             t0 = time.perf_counter()
             mv.adjoint(pos, u + DRIFT * d, cell, QM, dt, SPWT, steps, tgt, d, 10, mode=mp.SCATTER_DETERMINISTIC)
             adj_det.append(time.perf_counter() - t0)
-    f, a, ad = min(fwd), min(adj), min(adj_det)
+        # the same with reflecting walls: the forward and the adjoint (which re-walks every step
+        # from a stored start state to find the reflections)
+        mv.set_options(wall_mode=1)
+        fwd_r, adj_r = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            forward(DRIFT)
+            fwd_r.append(time.perf_counter() - t0)
+            refl = mv.counters()["reflections"]  # of the forward's 50 steps
+            t0 = time.perf_counter()
+            mv.adjoint(pos, u + DRIFT * d, cell, QM, dt, SPWT, steps, tgt, d, 10)
+            adj_r.append(time.perf_counter() - t0)
+    f, a, ad, fr, ar = min(fwd), min(adj), min(adj_det), min(fwd_r), min(adj_r)
     log(f"adjoint: forward {1e3 * f:.1f} ms, adjoint {1e3 * a:.1f} ms, ratio {a / f:.2f}; deterministic "
-        f"{1e3 * ad:.1f} ms, ratio {ad / f:.2f}")
+        f"{1e3 * ad:.1f} ms, ratio {ad / f:.2f}; reflecting walls: forward {1e3 * fr:.1f} ms, adjoint "
+        f"{1e3 * ar:.1f} ms, ratio {ar / fr:.2f}")
     return {"forward_ms": 1e3 * f, "adjoint_ms": 1e3 * a, "ratio": a / f, "deterministic_adjoint_ms": 1e3 * ad,
-            "deterministic_ratio": ad / f, "steps": steps, "particles": n,
+            "deterministic_ratio": ad / f, "reflecting_forward_ms": 1e3 * fr, "reflecting_adjoint_ms": 1e3 * ar,
+            "reflecting_ratio": ar / fr, "reflections_counted": int(refl), "steps": steps, "particles": n,
             "loss": loss, "grad_norm": float(np.linalg.norm(g)), "dL_ddrift": gd,
             "workload": "2M particles on a 30^3 six-tet box (162,000 tets), 50 steps, checkpoint every 10; both "
                         "timings include the particle upload (best of 3)"}

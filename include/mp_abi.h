@@ -245,14 +245,16 @@ int  mp_batch_get_counters(mp_batch* b, mp_counters* out);
 int  mp_batch_reset_counters(mp_batch* b);
 
 /* ---- adjoint of K frozen-field steps (no reference counterpart) ------------------------ */
-/* Runs K MoveParticles steps (absorbing walls) from the given initial particles under the
- * context's field, with L = sum_n (den_K[n] - den_target[n])^2 (den_K normalised by
- * spwt / node volume), and returns L, dL/dE [n_tets][3] and dL/d(drift) where the initial
- * velocities are u + drift * drift_dir.  Derivatives are those of the piece of the
- * piecewise-smooth loss the inputs sit in (every particle's tet path held fixed).  The
- * forward pass stores the particle state every checkpoint_every steps; the backward pass
- * replays each segment, recording the tets crossed, and sweeps it in reverse.  Replaces the
- * context's resident particle set (it holds the state after K steps on return). */
+/* Runs K MoveParticles steps from the given initial particles under the context's field and
+ * wall mode (absorbing or reflecting walls), with L = sum_n (den_K[n] - den_target[n])^2 (den_K
+ * normalised by spwt / node volume), and returns L, dL/dE [n_tets][3] and dL/d(drift) where
+ * the initial velocities are u + drift * drift_dir.  Derivatives are those of the piece of the
+ * piecewise-smooth loss the inputs sit in (every particle's tet path and reflections held
+ * fixed).  The forward pass stores the particle state every checkpoint_every steps; the
+ * backward pass replays each segment, recording the tets crossed (and, with reflecting walls,
+ * each step's start state, 48 B per particle and step), and sweeps it in reverse.  A step that
+ * reflects one particle more than 4 times is reported as MP_ERR_STATE.  Replaces the context's
+ * resident particle set (it holds the state after K steps on return). */
 int  mp_adjoint(mp_ctx* ctx, const double* pos0, const double* vel0, const int32_t* cell0, int64_t n,
                 double q_over_m, double dt, double spwt, int32_t steps, const double* den_target,
                 const double drift_dir[3], int32_t checkpoint_every, double* loss, double* dL_dE,

@@ -1,6 +1,7 @@
 // Adjoint of K push + walk + scatter steps under a frozen per-element field (mp_adjoint in
 // include/mp_abi.h).  No reference counterpart: the forward pass is MoveParticles
-// (src/mini-pic.cpp:287-308) with absorbing walls, repeated K times on the same particles.
+// (src/mini-pic.cpp:287-308), repeated K times on the same particles, with absorbing or
+// reflecting walls.
 //
 // With every particle's tet path c_0..c_{K-1} held fixed, one step is
 //     v_{k+1} = v_k + (qm E[c_k]) dt,   x_{k+1} = x_k + v_{k+1} dt
@@ -8,7 +9,11 @@
 // grad lc_v = (-beta_v, gamma_v, -delta_v) / (6 V).  The reverse sweep keeps
 //     lambda_x(k) = lambda_x(K) = sum_v dL/dlc_v grad lc_v,
 //     lambda_v(k+1) = lambda_v(k+2) + lambda_x dt,   dL/dE[c_k] += qm dt lambda_v(k+1),
-// and dL/d(drift) = lambda_v(0) . drift_dir.
+// and dL/d(drift) = lambda_v(0) . drift_dir.  A reflection maps x -> R x + b, v -> R v with
+// R = I - 2 g g^T / |g|^2 (g the wall plane's normal, b fixed by the tet), so with reflecting
+// walls the sweep first applies step k's reflections R_m .. R_1 (last first; R is symmetric)
+// to lambda_x and lambda_v; it finds them by walking step k again from its start state, which
+// the segment replay stores (48 B per particle and step).
 //
 // Memory: the state (x, v, cell) is copied every checkpoint_every steps during the forward
 // pass; the backward pass restores each checkpoint into the second particle buffer, replays
@@ -28,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "mp_common.h"
@@ -38,11 +44,12 @@ namespace mpd {
 // One forward step without density scatter (SCATTER = 0) or with the LDS-hash scatter of
 // the final step (SCATTER = 1).  path (optional) receives the cell at the start of the step.
 // SCATTER = 2 writes the final step's deterministic records (node, weight) at 4 * pid + v.
-template <int SCATTER>
+// With states, the step's start state (x, y, z, vx, vy, vz) goes to states[d * n + i].
+template <int SCATTER, int WALL>
 __global__ void __launch_bounds__(256)
 k_adj_step(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt, int cap,
-           int32_t* __restrict__ path, double* __restrict__ den, uint32_t* __restrict__ keys = nullptr,
-           double* __restrict__ vals = nullptr) {
+           int32_t* __restrict__ path, double* __restrict__ states, double* __restrict__ den,
+           uint32_t* __restrict__ keys = nullptr, double* __restrict__ vals = nullptr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
     int32_t fc = -1;
@@ -51,6 +58,14 @@ k_adj_step(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
         int32_t c = s.cell[i];
         if (path) path[i] = c;
         if (c >= 0) {
+            if (states) {
+                states[i] = s.x[i];
+                states[n + i] = s.y[i];
+                states[2 * n + i] = s.z[i];
+                states[3 * n + i] = s.vx[i];
+                states[4 * n + i] = s.vy[i];
+                states[5 * n + i] = s.vz[i];
+            }
             const TetView t = tet_at(tets, c);
             double vx = s.vx[i] + (qm * t.ex()) * dt;
             double vy = s.vy[i] + (qm * t.ey()) * dt;
@@ -59,7 +74,7 @@ k_adj_step(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
             double y = s.y[i] + vy * dt;
             double z = s.z[i] + vz * dt;
             WalkStats st{0, 0};
-            const int r = walk<0>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
+            const int r = walk<WALL>(tets, c, x, y, z, vx, vy, vz, l0, l1, l2, l3, cap, st);
             s.x[i] = x; s.y[i] = y; s.z[i] = z;
             s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
             if (r == kInside) {
@@ -211,9 +226,65 @@ __device__ __forceinline__ void lds_insert3_fixed(int32_t* hk, unsigned long lon
 
 // The reverse sweep of k_adj_sweep with dL/dE summed exactly in 128-bit fixed point:
 // acc[2 * (3 * tet + d)] = hi word, acc[... + 1] = lo word, scale 2^S.
+// The plane normals of one walk's reflections, in order (at most kMaxRefl per step).
+constexpr int kMaxRefl = 4;
+struct ReflRec {
+    double g[kMaxRefl][3];
+    int n = 0;
+    bool over = false;
+    __device__ __forceinline__ void operator()(double gx, double gy, double gz) {
+        if (n == kMaxRefl) {
+            over = true;
+            return;
+        }
+        g[n][0] = gx;
+        g[n][1] = gy;
+        g[n][2] = gz;
+        n++;
+    }
+};
+
+// Reflecting walls: step k of particle i walked again from its start state st (k_adj_step's
+// push and walk, the same bits), and its reflections applied to lambda_x (a) and lambda_v
+// (v), the last one first.  *over is set if a step reflected the particle more than
+// kMaxRefl times.
+__device__ __forceinline__ void unreflect(const TetBlock* __restrict__ tets, const double* __restrict__ st,
+                                          int64_t n, int64_t i, int32_t c, double qm, double dt, int cap,
+                                          double& ax, double& ay, double& az, double& vx, double& vy,
+                                          double& vz, int32_t* __restrict__ over) {
+    const TetView t = tet_at(tets, c);
+    double ux = st[3 * n + i] + (qm * t.ex()) * dt;
+    double uy = st[4 * n + i] + (qm * t.ey()) * dt;
+    double uz = st[5 * n + i] + (qm * t.ez()) * dt;
+    double x = st[i] + ux * dt;
+    double y = st[n + i] + uy * dt;
+    double z = st[2 * n + i] + uz * dt;
+    ReflRec rec;
+    WalkStats ws{0, 0};
+    double l0, l1, l2, l3;
+    int32_t cc = c;
+    (void)walk<1>(tets, cc, x, y, z, ux, uy, uz, l0, l1, l2, l3, cap, ws, rec);
+    if (rec.over) *over = 1;
+    for (int j = rec.n - 1; j >= 0; j--) {
+        const double gx = rec.g[j][0], gy = rec.g[j][1], gz = rec.g[j][2];
+        const double gg = (gx * gx + gy * gy) + gz * gz;
+        const double pa = (2.0 * ((ax * gx + ay * gy) + az * gz)) / gg;
+        ax = ax - pa * gx;
+        ay = ay - pa * gy;
+        az = az - pa * gz;
+        const double pv = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
+        vx = vx - pv * gx;
+        vy = vy - pv * gy;
+        vz = vz - pv * gz;
+    }
+}
+
+template <int WALL>
 __global__ void __launch_bounds__(256)
-k_adj_sweep_fixed(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n, const double* __restrict__ lx,
-                  double* __restrict__ lv, double qm, double dt, unsigned long long* __restrict__ acc, double scale) {
+k_adj_sweep_fixed(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n, double* __restrict__ lx,
+                  double* __restrict__ lv, double qm, double dt, unsigned long long* __restrict__ acc, double scale,
+                  const TetBlock* __restrict__ tets, const double* __restrict__ states, int cap,
+                  int32_t* __restrict__ over) {
     __shared__ int32_t hk[kHashSlots];
     __shared__ unsigned long long hw[6 * kHashSlots];  // per slot and axis: hi, lo
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,11 +301,13 @@ k_adj_sweep_fixed(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int6
     }
     __syncthreads();
     for (int32_t k = k1 - 1; k >= k0; k--) {
+        const int32_t c = any ? path[(int64_t)(k - k0) * n + i] : -1;
+        if (WALL && c >= 0)
+            unreflect(tets, states + (int64_t)(k - k0) * 6 * n, n, i, c, qm, dt, cap, ax, ay, az, vx, vy, vz, over);
         vx += ax * dt;
         vy += ay * dt;
         vz += az * dt;
         if (any) {
-            const int32_t c = path[(int64_t)(k - k0) * n + i];
             if (c >= 0)
                 lds_insert3_fixed(hk, hw, c, to_fixed((qm * dt) * vx, scale), to_fixed((qm * dt) * vy, scale),
                                   to_fixed((qm * dt) * vz, scale));
@@ -255,13 +328,18 @@ k_adj_sweep_fixed(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int6
     }
     if (live) {
         lv[3 * i] = vx; lv[3 * i + 1] = vy; lv[3 * i + 2] = vz;
+        if (WALL) {
+            lx[3 * i] = ax; lx[3 * i + 1] = ay; lx[3 * i + 2] = az;
+        }
     }
 }
 
 // Reverse sweep of steps [k0, k1) of one segment, path[(k - k0) * n + i] = c_k of particle i.
+template <int WALL>
 __global__ void __launch_bounds__(256)
-k_adj_sweep(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n, const double* __restrict__ lx,
-            double* __restrict__ lv, double qm, double dt, double* __restrict__ dLdE) {
+k_adj_sweep(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n, double* __restrict__ lx,
+            double* __restrict__ lv, double qm, double dt, double* __restrict__ dLdE, const TetBlock* __restrict__ tets,
+            const double* __restrict__ states, int cap, int32_t* __restrict__ over) {
     __shared__ int32_t hk[kHashSlots];
     __shared__ double hv[3 * kHashSlots];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,13 +356,13 @@ k_adj_sweep(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n,
     }
     __syncthreads();
     for (int32_t k = k1 - 1; k >= k0; k--) {
+        const int32_t c = any ? path[(int64_t)(k - k0) * n + i] : -1;
+        if (WALL && c >= 0)
+            unreflect(tets, states + (int64_t)(k - k0) * 6 * n, n, i, c, qm, dt, cap, ax, ay, az, vx, vy, vz, over);
         vx += ax * dt;
         vy += ay * dt;
         vz += az * dt;
-        if (any) {
-            const int32_t c = path[(int64_t)(k - k0) * n + i];
-            if (c >= 0) lds_insert3(hk, hv, c, (qm * dt) * vx, (qm * dt) * vy, (qm * dt) * vz);
-        }
+        if (any && c >= 0) lds_insert3(hk, hv, c, (qm * dt) * vx, (qm * dt) * vy, (qm * dt) * vz);
         __syncthreads();
         for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
             const int32_t key = hk[s];
@@ -300,6 +378,9 @@ k_adj_sweep(const int32_t* __restrict__ path, int32_t k0, int32_t k1, int64_t n,
     }
     if (live) {
         lv[3 * i] = vx; lv[3 * i + 1] = vy; lv[3 * i + 2] = vz;
+        if (WALL) {
+            lx[3 * i] = ax; lx[3 * i + 1] = ay; lx[3 * i + 2] = az;
+        }
     }
 }
 
@@ -385,7 +466,7 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
     if (mode != MP_SCATTER_ATOMIC && mode != MP_SCATTER_DETERMINISTIC)
         return set_err(c, MP_ERR_ARG, "mp_adjoint: unknown scatter mode");
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    if (c->opt.wall_mode != MP_WALL_ABSORB) return set_err(c, MP_ERR_ARG, "mp_adjoint: absorbing walls only");
+    const bool wall = c->opt.wall_mode == MP_WALL_REFLECT;
     int rc = mp_set_particles(c, pos0, vel0, cell0, n);
     if (rc) return rc;
     // one cell sort for locality (pids are irrelevant to the summed gradients)
@@ -425,7 +506,11 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
     double* part = buf.get<double>(std::max(nb_loss, nb_part));
     // deterministic mode: the 128-bit fixed-point dL/dE sums (hi, lo per component)
     unsigned long long* acc = det ? buf.get<unsigned long long>(6 * (int64_t)T) : nullptr;
-    if (!path || !lx || !lv || !den || !tgt || !g || !dE || !part || (det && !acc))
+    // reflecting walls: each replayed step's start state (6 x m doubles per step of a segment)
+    // and the flag for a step that reflected a particle more often than the sweep records
+    double* states = wall ? buf.get<double>((int64_t)C * 6 * mm) : nullptr;
+    int32_t* over = buf.get<int32_t>(1);
+    if (!path || !lx || !lv || !den || !tgt || !g || !dE || !part || (det && !acc) || (wall && !states) || !over)
         return set_err(c, MP_ERR_ALLOC, "mp_adjoint: workspace allocation failed");
     // deterministic mode: records of the last forward step at 4 * pid + v
     const int64_t n_rec = 4 * c->next_pid;
@@ -436,6 +521,7 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
     HIP_TRY(c, hipMemcpyAsync(tgt, den_target, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice, st));
     HIP_TRY(c, mpd::launch_fill_f64(den, nn, 0.0, st));
     HIP_TRY(c, mpd::launch_fill_f64(dE, 3 * (int64_t)T, 0.0, st));
+    HIP_TRY(c, mpd::launch_fill_i32(over, 1, 0, st));
     const Soa& S = c->soa[c->cur];
     const Soa& W = c->soa[1 - c->cur];
     const int cap = c->opt.walk_cap;
@@ -444,16 +530,22 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
     for (int32_t k = 0; k < steps; k++) {
         if (k % C == 0) HIP_TRY(c, copy_state(S, cks[k / C], m, st));
         if (m > 0) {
-            if (k == steps - 1 && det) {
-                HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, st));
-                hipLaunchKernelGGL((mpd::k_adj_step<2>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap,
-                                   (int32_t*)nullptr, (double*)nullptr, c->keys[0], c->vals[0]);
-            } else if (k == steps - 1)
-                hipLaunchKernelGGL((mpd::k_adj_step<1>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap,
-                                   (int32_t*)nullptr, den);
-            else
-                hipLaunchKernelGGL((mpd::k_adj_step<0>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap,
-                                   (int32_t*)nullptr, (double*)nullptr);
+            const int sc = k < steps - 1 ? 0 : det ? 2 : 1;
+            if (sc == 2) HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, st));
+#define MP_ADJ_FWD(SC, W)                                                                                     \
+    hipLaunchKernelGGL((mpd::k_adj_step<SC, W>), dim3(gm), dim3(256), 0, st, c->tets, S, m, q_over_m, dt, cap, \
+                       (int32_t*)nullptr, (double*)nullptr, sc == 1 ? den : (double*)nullptr,                  \
+                       sc == 2 ? c->keys[0] : nullptr, sc == 2 ? c->vals[0] : nullptr)
+            if (wall) {
+                if (sc == 0) MP_ADJ_FWD(0, 1);
+                else if (sc == 1) MP_ADJ_FWD(1, 1);
+                else MP_ADJ_FWD(2, 1);
+            } else {
+                if (sc == 0) MP_ADJ_FWD(0, 0);
+                else if (sc == 1) MP_ADJ_FWD(1, 0);
+                else MP_ADJ_FWD(2, 0);
+            }
+#undef MP_ADJ_FWD
             HIP_TRY(c, hipGetLastError());
         }
     }
@@ -502,16 +594,27 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
         const int32_t k0 = sgi * C, k1 = std::min(steps, k0 + C);
         HIP_TRY(c, copy_state(cks[sgi], W, m, st));
         for (int32_t k = k0; k < k1; k++) {
-            hipLaunchKernelGGL((mpd::k_adj_step<0>), dim3(gm), dim3(256), 0, st, c->tets, W, m, q_over_m, dt, cap,
-                               path + (int64_t)(k - k0) * m, (double*)nullptr);
+            int32_t* pk = path + (int64_t)(k - k0) * m;
+            if (wall)
+                hipLaunchKernelGGL((mpd::k_adj_step<0, 1>), dim3(gm), dim3(256), 0, st, c->tets, W, m, q_over_m, dt,
+                                   cap, pk, states + (int64_t)(k - k0) * 6 * m, (double*)nullptr, nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((mpd::k_adj_step<0, 0>), dim3(gm), dim3(256), 0, st, c->tets, W, m, q_over_m, dt,
+                                   cap, pk, (double*)nullptr, (double*)nullptr, nullptr, nullptr);
             HIP_TRY(c, hipGetLastError());
         }
-        if (det)
-            hipLaunchKernelGGL(mpd::k_adj_sweep_fixed, dim3(gm), dim3(256), 0, st, path, k0, k1, m, lx, lv, q_over_m,
-                               dt, acc, scale);
-        else
-            hipLaunchKernelGGL(mpd::k_adj_sweep, dim3(gm), dim3(256), 0, st, path, k0, k1, m, lx, lv, q_over_m, dt,
-                               dE);
+#define MP_ADJ_SWEEP(W)                                                                                        \
+    do {                                                                                                       \
+        if (det)                                                                                               \
+            hipLaunchKernelGGL((mpd::k_adj_sweep_fixed<W>), dim3(gm), dim3(256), 0, st, path, k0, k1, m, lx, lv, \
+                               q_over_m, dt, acc, scale, c->tets, states, cap, over);                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((mpd::k_adj_sweep<W>), dim3(gm), dim3(256), 0, st, path, k0, k1, m, lx, lv,       \
+                               q_over_m, dt, dE, c->tets, states, cap, over);                                  \
+    } while (0)
+        if (wall) MP_ADJ_SWEEP(1);
+        else MP_ADJ_SWEEP(0);
+#undef MP_ADJ_SWEEP
         HIP_TRY(c, hipGetLastError());
     }
     double D = 0.0;
@@ -547,7 +650,12 @@ extern "C" int mp_adjoint_ex(mp_ctx* c, const double* pos0, const double* vel0, 
         HIP_TRY(c, hipGetLastError());
     }
     HIP_TRY(c, hipMemcpyAsync(dL_dE, dE, sizeof(double) * 3 * (size_t)T, hipMemcpyDeviceToHost, st));
+    int32_t h_over = 0;
+    HIP_TRY(c, hipMemcpyAsync(&h_over, over, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(c, hipStreamSynchronize(st));
+    if (h_over)
+        return set_err(c, MP_ERR_STATE, "mp_adjoint: a step reflected one particle more than " +
+                                             std::to_string(mpd::kMaxRefl) + " times");
     if (det && m > 0) {
         // exact 128-bit sum of the block partials, rounded to double once
         unsigned __int128 sum = 0;

@@ -101,16 +101,21 @@ struct WalkStats {
     int refl;
 };
 
+// No reflection recorder (walk's default).
+struct NoRefl {
+    __device__ __forceinline__ void operator()(double, double, double) const {}
+};
+
 // XtoLtet (src/meshes.h:75-100) from cell c.  On return c is the last tet visited and
 // l0..l3 its barycentric weights.  WALL == 1 reflects at boundary faces instead of
-// reporting the particle lost; a reflection counts as one move toward cap.  The tet's
-// fields are read where they are used rather than staged in registers, which keeps the
-// loop's live registers few.
-template <int WALL>
+// reporting the particle lost; a reflection counts as one move toward cap, and rec(g) is
+// called with each reflection's plane normal g, in order.  The tet's fields are read where
+// they are used rather than staged in registers, which keeps the loop's live registers few.
+template <int WALL, class Rec = NoRefl>
 __device__ __forceinline__ int walk(const TetBlock* __restrict__ tets, int32_t& c, double& x,
                                     double& y, double& z, double& vx, double& vy, double& vz,
                                     double& l0, double& l1, double& l2, double& l3, int cap,
-                                    WalkStats& st) {
+                                    WalkStats& st, Rec&& rec = Rec{}) {
     while (true) {
         const TetView t = tet_at(tets, c);
         const double vol = t.vol();
@@ -137,6 +142,7 @@ __device__ __forceinline__ int walk(const TetBlock* __restrict__ tets, int32_t& 
         // Specular reflection about the plane lc_mi = 0.  grad lc_mi is proportional to
         // g = (-beta, gamma, -delta) / (6 V); x' = x - 2 lc g (6V)/|g|^2, v' = v - 2 (v.g) g/|g|^2.
         const double gx = -t.c(mi, 1), gy = t.c(mi, 2), gz = -t.c(mi, 3);
+        rec(gx, gy, gz);
         const double gg = (gx * gx + gy * gy) + gz * gz;
         const double s = ((2.0 * ml) * (6.0 * vol)) / gg;
         x = x - s * gx;

@@ -34,7 +34,7 @@ _SIG = {
     "or_compact": (_I64, [_P, _P, _P, _P, _P, _I64]),
     "or_compact_aos": (_I64, [_P, _I64]),
     "or_normalise": (None, [_P, _P, _I32, _D]),
-    "or_adjoint": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32,
+    "or_adjoint": (C.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _P, _I64, _D, _D, _D, _I32, _P, _P, _I32, _I32,
                              _P, _P, _P]),
     "or_fe_create": (_P, [_P, _P, _I32, _P, _I32, _D, _D, _D, _D, C.c_int]),
     "or_fe_free": (None, [_P]),
@@ -293,7 +293,8 @@ class Run:
         return self.den
 
 
-def adjoint(tets, ef, vol, pos0, vel0, cell0, qm, dt, spwt, steps, den_target, drift_dir, walk_cap=4096):
+def adjoint(tets, ef, vol, pos0, vel0, cell0, qm, dt, spwt, steps, den_target, drift_dir, walk_cap=4096,
+            wall_mode=0):
     tets = np.ascontiguousarray(tets, dtype=TET_DTYPE)
     ef = np.ascontiguousarray(ef, dtype=np.float64)
     vol = np.ascontiguousarray(vol, dtype=np.float64)
@@ -306,8 +307,8 @@ def adjoint(tets, ef, vol, pos0, vel0, cell0, qm, dt, spwt, steps, den_target, d
     dldd = C.c_double()
     g = np.zeros((len(tets), 3), np.float64)
     rc = lib().or_adjoint(_ptr(tets), _ptr(ef), len(tets), _ptr(vol), len(vol), _ptr(pos0), _ptr(vel0), _ptr(cell0),
-                          len(cell0), qm, dt, spwt, steps, _ptr(tgt), _ptr(dd), walk_cap, C.byref(loss), _ptr(g),
-                          C.byref(dldd))
+                          len(cell0), qm, dt, spwt, steps, _ptr(tgt), _ptr(dd), wall_mode, walk_cap, C.byref(loss),
+                          _ptr(g), C.byref(dldd))
     if rc != 0:
         raise RuntimeError(f"or_adjoint failed: {rc}")
     return float(loss.value), g, float(dldd.value)

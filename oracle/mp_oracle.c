@@ -16,8 +16,8 @@ enum { OR_INSIDE = 0, OR_LOST = 1, OR_CAPPED = 2 };
  * the cap counts face moves (and reflections); at the cap the particle is lost.  A
  * reflection mirrors position and velocity about the plane lc_min = 0, whose inward
  * normal is g = (-beta, gamma, -delta) of that vertex. */
-static int or_walk(const or_tet* tets, int32_t* cell, double* x, double* v, double lc[4], int wall,
-                   int cap, int* moves, int* refl) {
+static int or_walk_rec(const or_tet* tets, int32_t* cell, double* x, double* v, double lc[4], int wall,
+                       int cap, int* moves, int* refl, double* rec, int rec_max, int* n_rec) {
     for (;;) {
         const or_tet* t = &tets[*cell];
         int inside = 1;
@@ -51,8 +51,20 @@ static int or_walk(const or_tet* tets, int32_t* cell, double* x, double* v, doub
         v[0] = v[0] - w * gx;
         v[1] = v[1] - w * gy;
         v[2] = v[2] - w * gz;
+        if (rec) { /* the reflection's plane normal, in order (the adjoint replays them backwards) */
+            if (*n_rec >= rec_max) return -1;
+            rec[3 * *n_rec + 0] = gx;
+            rec[3 * *n_rec + 1] = gy;
+            rec[3 * *n_rec + 2] = gz;
+            (*n_rec)++;
+        }
         (*refl)++;
     }
+}
+
+static int or_walk(const or_tet* tets, int32_t* cell, double* x, double* v, double lc[4], int wall,
+                   int cap, int* moves, int* refl) {
+    return or_walk_rec(tets, cell, x, v, lc, wall, cap, moves, refl, NULL, 0, NULL);
 }
 
 static void add_stats(int64_t* stats, int lost, int crossed, int moves, int capped, int refl) {
@@ -193,16 +205,27 @@ void or_normalise(double* den, const double* vol, int32_t nn, double spwt) {
  *     for k = K-1 .. 0:  lambda_v(k+1) += lambda_x(k+1) dt ;  lambda_x(k) = lambda_x(k+1)
  *                        dL/dE[c_k] += qm dt lambda_v(k+1) ;   lambda_v(k) = lambda_v(k+1)
  *     dL/d drift = lambda_v(0) . drift_dir   (v_0 = u + drift * drift_dir)
+ * Reflecting walls (wall_mode 1): a reflection in step k maps x -> R x + b, v -> R v with
+ * R = I - 2 g g^T / |g|^2 (g the wall's plane normal, b fixed by the tet), so before the
+ * update above step k's reflections R_m .. R_1 (last one first) are applied to lambda_x and
+ * lambda_v (R is symmetric).  The reverse sweep finds them by re-running step k from its
+ * stored start state.
  * Particles lost before step K contribute nothing.  This oracle stores the full cell
- * path (K ints per particle); it is meant for small n.
+ * path (K ints per particle; with reflecting walls also the K start states); it is meant
+ * for small n.
  */
 int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const double* vol, int32_t nn,
                const double* pos0, const double* vel0, const int32_t* cell0, int64_t n, double qm,
                double dt, double spwt, int32_t steps, const double* den_target,
-               const double drift_dir[3], int32_t walk_cap, double* loss, double* dL_dE,
+               const double drift_dir[3], int32_t wall_mode, int32_t walk_cap, double* loss, double* dL_dE,
                double* dL_ddrift) {
     (void)n_tets;
     if (steps < 1) return 1;
+    double* states = NULL;
+    if (wall_mode) {
+        states = (double*)malloc(sizeof(double) * 6 * (size_t)n * (size_t)steps);
+        if (!states) return 2;
+    }
     int32_t* path = (int32_t*)malloc(sizeof(int32_t) * (size_t)n * (size_t)steps);
     double* x = (double*)malloc(sizeof(double) * 3 * (size_t)n);
     double* v = (double*)malloc(sizeof(double) * 3 * (size_t)n);
@@ -210,7 +233,7 @@ int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const doubl
     double* lc = (double*)malloc(sizeof(double) * 4 * (size_t)n);
     double* den = (double*)calloc((size_t)nn, sizeof(double));
     if (!path || !x || !v || !c || !lc || !den) {
-        free(path); free(x); free(v); free(c); free(lc); free(den);
+        free(path); free(x); free(v); free(c); free(lc); free(den); free(states);
         return 2;
     }
     memcpy(x, pos0, sizeof(double) * 3 * (size_t)n);
@@ -218,8 +241,13 @@ int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const doubl
     memcpy(c, cell0, sizeof(int32_t) * (size_t)n);
     for (int32_t k = 0; k < steps; k++) {
         for (int64_t p = 0; p < n; p++) path[(size_t)k * n + p] = c[p];
+        if (states)
+            for (int64_t p = 0; p < n; p++) {
+                memcpy(states + 6 * ((size_t)k * n + p), x + 3 * p, 3 * sizeof(double));
+                memcpy(states + 6 * ((size_t)k * n + p) + 3, v + 3 * p, 3 * sizeof(double));
+            }
         memset(den, 0, sizeof(double) * (size_t)nn);
-        or_move(tets, ef, x, v, c, lc, n, qm, dt, den, 0, walk_cap, NULL);
+        or_move(tets, ef, x, v, c, lc, n, qm, dt, den, wall_mode, walk_cap, NULL);
     }
     or_normalise(den, vol, nn, spwt);
     double L = 0.0;
@@ -244,6 +272,32 @@ int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const doubl
         double lv[3] = {0.0, 0.0, 0.0};
         for (int32_t k = steps - 1; k >= 0; k--) {
             const int32_t ck = path[(size_t)k * n + p];
+            if (states) {
+                /* step k again from its start state (or_move's arithmetic), recording the
+                 * reflections' plane normals */
+                const double* s0 = states + 6 * ((size_t)k * n + p);
+                double xs[3] = {s0[0], s0[1], s0[2]}, vs[3] = {s0[3], s0[4], s0[5]}, l4[4], gr[3 * 8];
+                for (int i = 0; i < 3; i++) vs[i] += qm * ef[3 * (int64_t)ck + i] * dt;
+                for (int i = 0; i < 3; i++) xs[i] += vs[i] * dt;
+                int32_t cw = ck;
+                int moves = 0, refl = 0, nr = 0;
+                if (or_walk_rec(tets, &cw, xs, vs, l4, 1, walk_cap, &moves, &refl, gr, 8, &nr) < 0) {
+                    free(path); free(x); free(v); free(c); free(lc); free(den); free(states);
+                    return 3;
+                }
+                for (int j = nr - 1; j >= 0; j--) {
+                    const double gx = gr[3 * j], gy = gr[3 * j + 1], gz = gr[3 * j + 2];
+                    const double gg = (gx * gx + gy * gy) + gz * gz;
+                    const double ax = (2.0 * ((lx[0] * gx + lx[1] * gy) + lx[2] * gz)) / gg;
+                    lx[0] = lx[0] - ax * gx;
+                    lx[1] = lx[1] - ax * gy;
+                    lx[2] = lx[2] - ax * gz;
+                    const double av = (2.0 * ((lv[0] * gx + lv[1] * gy) + lv[2] * gz)) / gg;
+                    lv[0] = lv[0] - av * gx;
+                    lv[1] = lv[1] - av * gy;
+                    lv[2] = lv[2] - av * gz;
+                }
+            }
             for (int d = 0; d < 3; d++) {
                 lv[d] += lx[d] * dt;
                 dL_dE[3 * (int64_t)ck + d] += (qm * dt) * lv[d];
@@ -252,6 +306,6 @@ int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const doubl
         dd += (lv[0] * drift_dir[0] + lv[1] * drift_dir[1]) + lv[2] * drift_dir[2];
     }
     *dL_ddrift = dd;
-    free(path); free(x); free(v); free(c); free(lc); free(den);
+    free(path); free(x); free(v); free(c); free(lc); free(den); free(states);
     return 0;
 }

@@ -68,13 +68,15 @@ void or_normalise(double* den, const double* vol, int32_t nn, double spwt);
 
 /* Adjoint of K frozen-field steps (see mp_oracle.c for the derivation).  Inputs are the
  * initial particles (pos0, vel0 [n][3], cell0 [n]); den_target [nn] and the loss
- * L = sum_n (den_K[n] - den_target[n])^2 with den_K normalised by spwt / vol.
+ * L = sum_n (den_K[n] - den_target[n])^2 with den_K normalised by spwt / vol.  wall_mode 0
+ * absorbs at boundary faces, 1 reflects.
  * Outputs: loss, dL/dE [T][3] (added to, caller zeroes) and dL/d(drift along drift_dir):
- * sum over particles of lambda_v0 . drift_dir. */
+ * sum over particles of lambda_v0 . drift_dir.  Returns 3 if a step reflects a particle more
+ * than 8 times. */
 int or_adjoint(const or_tet* tets, const double* ef, int32_t n_tets, const double* vol, int32_t nn,
                const double* pos0, const double* vel0, const int32_t* cell0, int64_t n, double qm,
                double dt, double spwt, int32_t steps, const double* den_target,
-               const double drift_dir[3], int32_t walk_cap, double* loss, double* dL_dE,
+               const double drift_dir[3], int32_t wall_mode, int32_t walk_cap, double* loss, double* dL_dE,
                double* dL_ddrift);
 
 #ifdef __cplusplus

@@ -69,6 +69,40 @@ def test_adjoint_deterministic_same_bits_and_matches_oracle(mp, orc):
     assert abs(gd - gdr) <= 1e-10 * abs(gdr)
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["atomic", "deterministic"])
+@pytest.mark.parametrize("ckpt", [1, 5])
+def test_adjoint_reflecting_walls_matches_oracle(mp, orc, mode, ckpt):
+    """Reflecting walls (a 2 cm box, so that most particles reach a wall in the 12 steps and
+    some reflect twice): the loss, dL/dE and dL/d(drift) against the oracle, whose reverse
+    sweep applies each step's reflections found by re-walking the step (pinned against central
+    differences in the CPU suite); the deterministic mode gives the same bits on two runs."""
+    side = 0.02
+    mesh, ef, pos, u, cell, d, dt, steps, drift = problem(mp, side=side)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, u + drift * d, cell, wall_mode=1)
+    for _ in range(steps):
+        run.step(QM, dt, SPWT)
+    assert run.stats["reflections"] > len(pos) // 2 and run.stats["lost"] == 0
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(wall_mode=1)
+        mv.upload_field(ef)
+        mv.set_particles(pos, u + 1.05 * drift * d, cell)
+        for _ in range(steps):
+            mv.step(QM, dt, SPWT, 1)
+        tgt = mv.density()
+        outs = [mv.adjoint(pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d, checkpoint_every=ckpt, mode=mode)
+                for _ in range(2 if mode else 1)]
+    L, g, gd = outs[0]
+    if mode:
+        assert outs[1][0] == L and outs[1][2] == gd and outs[1][1].tobytes() == g.tobytes()
+    Lr, gr, gdr = orc.adjoint(mesh.tets, ef, mesh.node_volume, pos, u + drift * d, cell, QM, dt, SPWT, steps, tgt, d,
+                              wall_mode=1)
+    assert abs(L - Lr) <= 1e-12 * abs(Lr)
+    scale = np.abs(gr).max()
+    assert scale > 0 and np.count_nonzero(g) > 0
+    np.testing.assert_allclose(g, gr, rtol=0, atol=1e-10 * scale)
+    assert abs(gd - gdr) <= 1e-10 * abs(gdr)
+
+
 @pytest.mark.parametrize("ckpt", [1, 4, 12])
 def test_adjoint_checkpointing_invariant(mp, ckpt):
     """The gradient does not depend on the checkpoint interval (replay is exact)."""

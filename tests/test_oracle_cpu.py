@@ -318,28 +318,36 @@ def test_rnd_golden(orc):
 
 # ---- adjoint oracle --------------------------------------------------------------------
 
-def test_adjoint_oracle_vs_finite_differences(mp, orc):
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_adjoint_oracle_vs_finite_differences(mp, orc, wall):
+    """The oracle's gradients against central differences of the serial forward run, with
+    absorbing walls and with reflecting ones (the particles are fast enough that many of them
+    reflect, several more than once; the perturbations keep every path and reflection)."""
     m = mp.Mesh.box_mesh(4, 1.0)
     rng = np.random.default_rng(21)
     n = 400
     pos = rng.random((n, 3)) * 0.8 + 0.1
     cell = m.locate_box(pos)
     drift = np.array([1.0, 0.0, 0.0])
-    u = rng.normal(size=(n, 3)) * 0.02
+    u = rng.normal(size=(n, 3)) * (0.02 if wall == 0 else 0.2)
     ef = m.field_sinusoid(1.0)
     qm, dt, spwt, K = 0.05, 0.5, 1.0, 6
     tgt = rng.random(m.n_nodes) * 5
 
     def fwd(e, dr):
-        run = orc.Run(m.tets, e, m.node_volume, pos, u + dr * drift, cell)
+        run = orc.Run(m.tets, e, m.node_volume, pos, u + dr * drift, cell, wall_mode=wall)
         for _ in range(K):
             den = run.step(qm, dt, spwt)
+        fwd.refl = run.stats["reflections"]
         return float(((den - tgt) ** 2).sum()), run.cell.copy(), run.pid.copy()
 
-    L0, g, gd = orc.adjoint(m.tets, ef, m.node_volume, pos, u + 0.1 * drift, cell, qm, dt, spwt, K, tgt, drift)
+    L0, g, gd = orc.adjoint(m.tets, ef, m.node_volume, pos, u + 0.1 * drift, cell, qm, dt, spwt, K, tgt, drift,
+                            wall_mode=wall)
     Lf, c0, pid0 = fwd(ef, 0.1)
     assert abs(L0 - Lf) <= 1e-12 * Lf
-    h = 1e-7
+    if wall:
+        assert fwd.refl > n // 4 and len(c0) == n  # many reflections, nobody lost
+    h = 1e-6  # rounding error eps * L / h stays far below the tolerance at L ~ 1e7
     idx = rng.choice(np.flatnonzero(np.abs(g).sum(1) > 0), 12, replace=False)
     for e in idx:
         for d in range(3):
@@ -351,6 +359,6 @@ def test_adjoint_oracle_vs_finite_differences(mp, orc):
             assert np.array_equal(cp, c0) and np.array_equal(cm, c0)  # same cell paths
             fd = (Lp - Lm) / (2 * h)
             assert abs(fd - g[e, d]) <= 1e-5 * max(1.0, abs(g[e, d])), (e, d, fd, g[e, d])
-    hp = 1e-7
+    hp = 1e-6
     fd = (fwd(ef, 0.1 + hp)[0] - fwd(ef, 0.1 - hp)[0]) / (2 * hp)
     assert abs(fd - gd) <= 1e-5 * max(1.0, abs(gd))
