@@ -149,6 +149,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
     c->timing = false;
     Saved saved;
     int rc = save(c, saved);
+    const bool have_saved = rc == MP_OK;
     Trial tr{c, &saved, q_over_m, dt, spwt, mode, multi, min_steps, {nullptr, nullptr}, {}};
     if (!rc && (hipEventCreate(&tr.ev[0]) != hipSuccess || hipEventCreate(&tr.ev[1]) != hipSuccess))
         rc = set_err(c, MP_ERR_HIP, "mp_autotune: hipEventCreate failed");
@@ -210,8 +211,12 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         }
     }
 
-    // leave the run as it was, with the chosen options
-    if (!rc) rc = restore(c, saved);
+    // leave the run as it was, with the chosen options; after a failed trial too (the state it
+    // left is undone, and its error is the one reported)
+    if (have_saved) {
+        const int rr = restore(c, saved);
+        if (!rc) rc = rr;
+    }
     c->opt = rc ? start : cur;
     c->timing = timing;
     for (hipEvent_t e : tr.ev)
