@@ -274,7 +274,7 @@ __device__ __forceinline__ void steps_finish(uint32_t* hk, double* hv, double* _
 // starts a step); a lane whose particle moves loads the next tet's record for its next
 // iteration, and a lane that finishes a step goes on with the next one while other lanes still
 // walk.  A wave thus takes about max over its lanes of (K + moves) iterations instead of the
-// sum over the steps of its slowest walk.  The arithmetic of every step is walk_regs' after the
+// sum over the steps of its slowest walk.  The arithmetic of every step is walk()'s after the
 // push, in the same order.  Step k's weights go to key0 + k * per_step + node: the LDS hash
 // (HASH) or direct atomics on den.  Counters as in k_push_soa.
 template <int WALL, bool HASH, int BITS = kStepBits>

@@ -240,52 +240,4 @@ __device__ __forceinline__ void load_tet_regs(const TetView t, TetRegs& r) {
     r.m3 = t.con(3);
 }
 
-// XtoLtet from cell c whose record t is in registers; each move loads the next tet's whole
-// record into t, so on return t is the record of the final tet c.  Same decisions, moves and
-// reflections as first_step + walk (cap >= 1).
-template <int WALL>
-__device__ __forceinline__ int walk_regs(const TetBlock* __restrict__ tets, TetRegs& t, int32_t& c, double& x,
-                                         double& y, double& z, double& vx, double& vy, double& vz, double& l0,
-                                         double& l1, double& l2, double& l3, int cap, WalkStats& st) {
-    while (true) {
-        const StartTet& s = t.g;
-        l0 = bary(s.a0, s.b0, s.g0, s.d0, x, y, z, s.vol);
-        l1 = bary(s.a1, s.b1, s.g1, s.d1, x, y, z, s.vol);
-        l2 = bary(s.a2, s.b2, s.g2, s.d2, x, y, z, s.vol);
-        l3 = bary(s.a3, s.b3, s.g3, s.d3, x, y, z, s.vol);
-        const bool out = (l0 < 0 || l0 > 1.0) || (l1 < 0 || l1 > 1.0) || (l2 < 0 || l2 > 1.0) ||
-                         (l3 < 0 || l3 > 1.0);
-        if (!out) return kInside;
-        int mi = 0;
-        double ml = l0;
-        if (l1 < ml) { ml = l1; mi = 1; }
-        if (l2 < ml) { ml = l2; mi = 2; }
-        if (l3 < ml) { ml = l3; mi = 3; }
-        const int32_t nb = mi == 0 ? s.n0 : mi == 1 ? s.n1 : mi == 2 ? s.n2 : s.n3;
-        if (st.moves >= cap) return kCapped;
-        st.moves++;
-        if (nb >= 0) {
-            c = nb;
-            load_tet_regs(tet_at(tets, nb), t);
-            continue;
-        }
-        if (WALL == 0) return kLost;
-        // reflection about the face plane, as in walk()
-        const double b = mi == 0 ? s.b0 : mi == 1 ? s.b1 : mi == 2 ? s.b2 : s.b3;
-        const double g = mi == 0 ? s.g0 : mi == 1 ? s.g1 : mi == 2 ? s.g2 : s.g3;
-        const double d = mi == 0 ? s.d0 : mi == 1 ? s.d1 : mi == 2 ? s.d2 : s.d3;
-        const double gx = -b, gy = g, gz = -d;
-        const double gg = (gx * gx + gy * gy) + gz * gz;
-        const double sc = ((2.0 * ml) * (6.0 * s.vol)) / gg;
-        x = x - sc * gx;
-        y = y - sc * gy;
-        z = z - sc * gz;
-        const double w = (2.0 * ((vx * gx + vy * gy) + vz * gz)) / gg;
-        vx = vx - w * gx;
-        vy = vy - w * gy;
-        vz = vz - w * gz;
-        st.refl++;
-    }
-}
-
 }  // namespace mpd

@@ -290,12 +290,6 @@ __global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int32
     }
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int q) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), q);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), q);
-    return __hiloint2double(hi, lo);
-}
-
 // Serial fold per node in ascending pid: ((0 + v1) + v2) + ..., as a serial MoveParticles
 // adds into den; then den *= spwt / node volume.  The fold is a dependent chain per node, so
 // the work is laid out to run many chains at once:
