@@ -84,6 +84,13 @@ typedef struct mp_options {
                                       centroids, >> sort_shift (buckets compact in space:
                                       fewer distinct nodes per workgroup where particles are
                                       sparse, a few per tet) */
+    int32_t det_algo;              /* deterministic scatter: 1 (default) the push writes each
+                                      slot's records in slot order and counts them per node;
+                                      they are grouped into node segments and each segment is
+                                      put in pid order on chip (within a wave up to 512
+                                      records, else a workgroup radix sort) and folded; 0 the
+                                      records are written in pid order and a device radix sort
+                                      on the node bits groups them */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
@@ -189,7 +196,7 @@ typedef struct mp_tune_trial {
     double     ms_per_step;
 } mp_tune_trial;
 /* Choose particles_per_thread, block_size, scatter_agg (atomic mode), sort_shift,
- * sort_order, sort_interval (and keep_pid_order in deterministic mode) for the resident particles and the given step parameters:
+ * sort_order, sort_interval (and det_algo, keep_pid_order in deterministic mode) for the resident particles and the given step parameters:
  * coordinate descent over the knobs, each candidate timed (HIP events) over a window of
  * max(interval, min_steps) steps that starts from the same saved state and holds one cell
  * sort per sort interval, as a long run does; a candidate replaces the incumbent only when

@@ -189,13 +189,15 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
                       base + (uint32_t)tf.con(2), base + (uint32_t)tf.con(3), l0, l1, l2, l3);
 }
 
-// The atomic scatter a kernel's template arguments select (nothing when DET: the records are
-// written where each walk ends): AGG 0 one atomic per weight, 2 LDS hash per workgroup (every
-// thread of the block calls it).
+// The atomic scatter a kernel's template arguments select (nothing when DET == 1: the records
+// are written where each walk ends; unit weights when DET == 2, which counts each node's
+// records): AGG 0 one atomic per weight, 2 LDS hash per workgroup (every thread of the block
+// calls it).
 template <int DET, int AGG>
 __device__ __forceinline__ void scatter_any(const TetBlock* __restrict__ tets, double* __restrict__ den, bool ok,
                                             int32_t fc, double l0, double l1, double l2, double l3) {
-    if (DET) return;
+    if (DET == 1) return;
+    if (DET == 2) l0 = l1 = l2 = l3 = 1.0;  // record counts per node
     if (AGG == 2) {
         scatter_lds(tets, den, ok, fc, l0, l1, l2, l3);
     } else if (ok) {

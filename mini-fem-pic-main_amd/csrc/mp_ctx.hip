@@ -108,6 +108,7 @@ mp_options default_options() {
     o.keep_pid_order = 0;
     o.gather_sort = 0;
     o.sort_order = 0;
+    o.det_algo = 1;
     return o;
 }
 
@@ -163,7 +164,7 @@ int reserve_tally(mp_ctx* c, int64_t n, int block) {
 
 namespace mpctx {
 
-int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
+int reserve_records(mp_ctx* c, int64_t m, int end_bit, bool need_sort) {
     if (m > c->rec_cap) {
         for (int k = 0; k < 2; k++) {
             dfree(c->keys[k]);
@@ -180,7 +181,7 @@ int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
                 return set_err(c, MP_ERR_ALLOC, "deterministic scatter buffers: allocation failed");
         c->rec_cap = nc;
     }
-    const size_t need = mpd::radix_sort_temp_bytes(c->rec_cap, end_bit);
+    const size_t need = need_sort ? mpd::radix_sort_temp_bytes(c->rec_cap, end_bit) : 0;
     if (need > c->sort_tmp_bytes) {
         dfree(c->sort_tmp);
         c->sort_tmp = nullptr;
@@ -191,6 +192,63 @@ int reserve_records(mp_ctx* c, int64_t m, int end_bit) {
     }
     if (!c->seg_bounds && dalloc(&c->seg_bounds, 3 * (int64_t)c->n_nodes + 1) != hipSuccess)
         return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
+    return MP_OK;
+}
+
+hipError_t pin_acquire(PinStage& ps, size_t bytes) {
+    hipError_t e = hipSuccess;
+    if (ps.pending) {
+        e = hipEventSynchronize(ps.done);
+        if (e != hipSuccess) return e;
+        ps.pending = false;
+    }
+    if (!ps.done && (e = hipEventCreateWithFlags(&ps.done, hipEventDisableTiming)) != hipSuccess) return e;
+    if (bytes > ps.bytes) {
+        if (ps.host) (void)hipHostFree(ps.host);
+        ps.host = nullptr;
+        ps.bytes = 0;
+        const size_t nb = std::max(bytes + bytes / 2, size_t(64) << 10);
+        if ((e = hipHostMalloc(&ps.host, nb, 0)) != hipSuccess) return e;
+        ps.bytes = nb;
+    }
+    return hipSuccess;
+}
+
+hipError_t pin_release(PinStage& ps, hipStream_t st) {
+    const hipError_t e = hipEventRecord(ps.done, st);
+    ps.pending = e == hipSuccess;
+    return e;
+}
+
+void pin_free(PinStage& ps) {
+    if (ps.host) (void)hipHostFree(ps.host);
+    if (ps.done) (void)hipEventDestroy(ps.done);
+    ps = PinStage{};
+}
+
+// Deterministic scatter by node segments: the m slot-ordered records of a det = 2 push (and
+// the per-node counts it left in den_raw) grouped by node and folded in pid order.
+int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
+    if (!c->seg_tab && dalloc(&c->seg_tab, mpd::seg_table_ints(c->n_nodes)) != hipSuccess)
+        return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
+    int rc = reserve_scan(c, (int64_t)c->n_nodes + 1);
+    if (rc) return rc;
+    mpd::SegFoldArgs a;
+    a.keys = c->keys[0];
+    a.vals = c->vals[0];
+    a.pid = c->soa[c->cur].pid;
+    a.m = m;
+    a.raw = c->den_raw;
+    a.nn = c->n_nodes;
+    a.tab = c->seg_tab;
+    a.scan_tmp = c->scan_tmp;
+    a.seg_pid = c->keys[1];
+    a.seg_val = c->vals[1];
+    a.nvol = c->node_vol;
+    a.spwt = spwt;
+    a.out = c->den_out;
+    a.pid_bits = bit_width((uint64_t)std::max<int64_t>(c->next_pid - 1, 1));
+    HIP_TRY(c, mpd::launch_seg_fold(a, c->stream));
     return MP_OK;
 }
 
@@ -339,7 +397,8 @@ bool good_options(const mp_options& o) {
            o.walk_cap >= 1 && (o.scatter_agg == 0 || o.scatter_agg == 2) && o.sort_shift >= 0 &&
            o.sort_shift <= 16 &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
-           (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1);
+           (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1) &&
+           (o.det_algo == 0 || o.det_algo == 1);
 }
 
 }  // namespace
@@ -500,9 +559,11 @@ void mp_destroy(mp_ctx* c) {
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
                     c->inj_tmp, c->cmp_buf, c->den_steps, c->perm, c->ef_stage,
-                    c->live_cnt, c->tet_rank};
+                    c->live_cnt, c->tet_rank, c->seg_tab};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
+    pin_free(c->pin_ef);
+    pin_free(c->pin_inj);
     mpstage::stager_free(c->stage);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -537,10 +598,23 @@ int mp_upload_field(mp_ctx* c, const double* ef, int32_t n_tets) {
     HIP_TRY(c, hipSetDevice(c->device));
     // the staging buffer is kept: a free per upload would synchronise the device every step
     if (!c->ef_stage) HIP_TRY(c, dalloc(&c->ef_stage, 3 * (int64_t)n_tets));
-    hipError_t e =
-        hipMemcpyAsync(c->ef_stage, ef, sizeof(double) * 3 * (size_t)n_tets, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = mpd::launch_set_field(c->tets, c->ef_stage, n_tets, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const size_t bytes = sizeof(double) * 3 * (size_t)n_tets;
+    hipError_t e;
+    if (bytes <= kPinStageMax) {
+        // a small field (a coupled loop's mesh) goes through pinned memory, and the call returns
+        // without waiting for the device: the step launched next queues behind the copy
+        e = pin_acquire(c->pin_ef, bytes);
+        if (e == hipSuccess) {
+            std::memcpy(c->pin_ef.host, ef, bytes);
+            e = hipMemcpyAsync(c->ef_stage, c->pin_ef.host, bytes, hipMemcpyHostToDevice, c->stream);
+        }
+        if (e == hipSuccess) e = pin_release(c->pin_ef, c->stream);
+        if (e == hipSuccess) e = mpd::launch_set_field(c->tets, c->ef_stage, n_tets, c->stream);
+    } else {
+        e = hipMemcpyAsync(c->ef_stage, ef, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = mpd::launch_set_field(c->tets, c->ef_stage, n_tets, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_upload_field: ") + hipGetErrorString(e));
     c->have_field = true;
     return MP_OK;
@@ -662,13 +736,29 @@ int mp_add_particles(mp_ctx* c, const double* pos, const double* vel, const int3
             c->inj_bytes = 2 * (2 * bp + bc);
         }
         char* tmp = (char*)c->inj_tmp;
-        hipError_t e = hipMemcpyAsync(tmp, pos, bp, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(tmp + bp, vel, bp, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(tmp + 2 * bp, cell, bc, hipMemcpyHostToDevice, c->stream);
+        const size_t all = 2 * bp + bc;
+        hipError_t e;
+        const bool small = all <= kPinStageMax;
+        if (small) {
+            // a per-step injection: one copy through pinned memory, no wait for the device
+            e = pin_acquire(c->pin_inj, all);
+            if (e == hipSuccess) {
+                char* h = (char*)c->pin_inj.host;
+                std::memcpy(h, pos, bp);
+                std::memcpy(h + bp, vel, bp);
+                std::memcpy(h + 2 * bp, cell, bc);
+                e = hipMemcpyAsync(tmp, h, all, hipMemcpyHostToDevice, c->stream);
+            }
+            if (e == hipSuccess) e = pin_release(c->pin_inj, c->stream);
+        } else {
+            e = hipMemcpyAsync(tmp, pos, bp, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(tmp + bp, vel, bp, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(tmp + 2 * bp, cell, bc, hipMemcpyHostToDevice, c->stream);
+        }
         if (e == hipSuccess)
             e = mpd::launch_unpack((double*)tmp, (double*)(tmp + bp), (int32_t*)(tmp + 2 * bp), n, c->next_pid,
                                    c->soa[c->cur], c->n_slots, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess && !small) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return set_err(c, MP_ERR_HIP, std::string("mp_add_particles: ") + hipGetErrorString(e));
     }
     c->n_slots += n;
@@ -690,12 +780,14 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         if (rc) return rc;
     }
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    // record slots 4*slot+v while the slots are in pid order, else 4*pid+v for every pid issued
-    const int64_t n_rec = c->pid_ordered ? 4 * c->n_slots : 4 * c->next_pid;
+    const bool seg = det && c->opt.det_algo == 1;
+    // node segments: records at 4*slot+v, every slot writing its own; radix sort: 4*slot+v
+    // while the slots are in pid order, else 4*pid+v for every pid issued (sentinels pre-filled)
+    const int64_t n_rec = (seg || c->pid_ordered) ? 4 * c->n_slots : 4 * c->next_pid;
     if (det) {
-        int rc = reserve_records(c, n_rec, bit_width((uint64_t)c->n_nodes));
+        int rc = reserve_records(c, n_rec, bit_width((uint64_t)c->n_nodes), !seg);
         if (rc) return rc;
-        HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, c->stream));
+        if (!seg) HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, c->stream));
     }
     {
         int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
@@ -714,7 +806,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;
     a.wall = c->opt.wall_mode;
-    a.det = det;
+    a.det = seg ? 2 : det ? 1 : 0;
     a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
     a.ppt = c->opt.particles_per_thread;
@@ -722,7 +814,10 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));
-    if (det) {
+    if (seg) {
+        int rc = det_finish_seg(c, n_rec, spwt);
+        if (rc) return rc;
+    } else if (det) {
         int rc = det_finish(c, n_rec, spwt);
         if (rc) return rc;
     } else {
@@ -796,7 +891,7 @@ int mp_step_many(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mod
         a.ctr = c->ctr;
         a.cap = c->opt.walk_cap;
         a.wall = c->opt.wall_mode;
-        a.det = false;
+        a.det = 0;
         a.agg = c->opt.scatter_agg;
         a.block = 256;
         a.ppt = 1;

@@ -11,6 +11,16 @@
 using mpd::TetBlock;
 using mpd::Soa;
 
+// A pinned host buffer that a small upload is copied through, so the call can return before
+// the device has read it; `done` is recorded after the copy out of it, and the buffer is
+// written again only once that event has completed.
+struct PinStage {
+    void* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
 struct mp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -50,6 +60,7 @@ struct mp_ctx {
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int32_t* seg_bounds = nullptr;
+    int32_t* seg_tab = nullptr;  // node-segment scatter: counts, starts, cursors, hot list
     // drop-in AoS scratch
     mp_particle* aos = nullptr;
     mp_particle* aos_out = nullptr;
@@ -61,6 +72,8 @@ struct mp_ctx {
     // staging of injected particles (mp_add_particles)
     void* inj_tmp = nullptr;
     size_t inj_bytes = 0;
+    // pinned host buffers of the small uploads (field, injected particles)
+    PinStage pin_ef, pin_inj;
     // per-step densities of the last mp_step_many, [steps][n_nodes]
     double* den_steps = nullptr;
     int64_t den_steps_n = 0;
@@ -84,9 +97,15 @@ void dfree(void* p);
 int reserve_scan(mp_ctx* c, int64_t n);
 // Deterministic-scatter records for m record slots (keys[0..1], vals[0..1], sort scratch and
 // segment table), and the sort + ordered fold of m records into den_out (normalised by spwt).
-int reserve_records(mp_ctx* c, int64_t m, int end_bit);
+int reserve_records(mp_ctx* c, int64_t m, int end_bit, bool need_sort = true);
 int det_finish(mp_ctx* c, int64_t m, double spwt);
 hipError_t alloc_soa(void** mem, Soa* s, int64_t n);
+// Uploads up to kPinStageMax bytes go through a PinStage: pin_acquire waits for the buffer's
+// previous copy and grows it to `bytes`; pin_release records the copy just enqueued.
+constexpr size_t kPinStageMax = size_t(8) << 20;
+hipError_t pin_acquire(PinStage& ps, size_t bytes);
+hipError_t pin_release(PinStage& ps, hipStream_t st);
+void pin_free(PinStage& ps);
 
 template <typename T>
 hipError_t dalloc(T** p, int64_t n) {

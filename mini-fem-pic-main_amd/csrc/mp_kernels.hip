@@ -94,7 +94,7 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                     s.cell[i] = c;
                     ok = true;
                     fc = c;
-                    if (DET) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
+                    if (DET) put_records(keys, vals, (DET == 2 || rec_by_slot) ? i : (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
                 } else {
                     s.cell[i] = -1;
                     n_lost++;
@@ -103,6 +103,9 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                 st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
+            // slot-indexed records: every slot writes its four, lost and empty slots the sentinel
+            if (DET == 2 && !ok)
+                *reinterpret_cast<uint4*>(keys + 4 * i) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
         }
         scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
@@ -754,12 +757,22 @@ __global__ void k_flag_soa(const int32_t* __restrict__ cell, int64_t n, int32_t*
 }
 
 // Live slots (cell >= 0), one atomic per wave; count was cleared by k_fill_u64.
-__global__ void k_count_live(const int32_t* __restrict__ cell, int64_t n, unsigned long long* __restrict__ count) {
+// One atomic per workgroup: same-address atomics are performed one after another at the
+// memory side, so one per wave (800 at 50k particles) took longer than the count itself.
+__global__ void __launch_bounds__(256) k_count_live(const int32_t* __restrict__ cell, int64_t n,
+                                                    unsigned long long* __restrict__ count) {
+    __shared__ unsigned long long part[256 / kWave];
     unsigned long long k = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         k += cell[i] >= 0;
     for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, kWave);
-    if (lane_id() == 0 && k) atomicAdd(count, k);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 256 / kWave; w++) t += part[w];
+        if (t) atomicAdd(count, t);
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, unsigned long long v) { *p = v; }
@@ -872,11 +885,13 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
             hipLaunchKernelGGL((k_push_soa<W, D, G, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n,  \
                                a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt, a.rec_by_slot);     \
     } while (0)
-#define MP_PUSH_W(W)                          \
-    do {                                      \
-        if (a.det) MP_PUSH(W, 1, 0);          \
-        else if (a.agg == 2) MP_PUSH(W, 0, 2); \
-        else MP_PUSH(W, 0, 0);                \
+#define MP_PUSH_W(W)                                      \
+    do {                                                  \
+        if (a.det == 2 && a.agg == 2) MP_PUSH(W, 2, 2);   \
+        else if (a.det == 2) MP_PUSH(W, 2, 0);            \
+        else if (a.det) MP_PUSH(W, 1, 0);                 \
+        else if (a.agg == 2) MP_PUSH(W, 0, 2);            \
+        else MP_PUSH(W, 0, 0);                            \
     } while (0)
     if (a.wall == 0) MP_PUSH_W(0);
     else MP_PUSH_W(1);
@@ -953,6 +968,12 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
     if (m > 0)
         hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for((m + 3) / 4, 256)), dim3(256), 0, st, keys, m, nn, start,
                            end);
+    return launch_fold_nodes(vals, start, end, nvol, nn, spwt, run_spwt, nn_mesh, out, long_list, long_count, st);
+}
+
+hipError_t launch_fold_nodes(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,
+                             int32_t nn, double spwt, const double* run_spwt, int32_t nn_mesh, double* out,
+                             int32_t* long_list, int32_t* long_count, hipStream_t st) {
     hipLaunchKernelGGL(k_fold, dim3(grid_for(nn, 256)), dim3(256), 0, st, vals, start, end, nvol, nn, spwt,
                        run_spwt, nn_mesh, out, long_list, long_count);
     // long segments exist only where a few thousand nodes hold most records: one wave each, so
@@ -965,6 +986,7 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
                        nn_mesh, out, long_list, long_count);
     return hipGetLastError();
 }
+
 
 int64_t scan_scratch_ints(int64_t n) {
     int64_t total = 0;
@@ -1026,7 +1048,8 @@ hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n,
 hipError_t launch_count_live(const int32_t* cell, int64_t n, unsigned long long* count, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, count, 0ull);
     if (n > 0) {
-        const unsigned g = (unsigned)std::min<int64_t>(1024, grid_for(n, 256));
+        // at least 4096 cells per workgroup, at most 1024 workgroups
+        const unsigned g = (unsigned)std::min<int64_t>(1024, (n + 4095) / 4096);
         hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, st, cell, n, count);
     }
     return hipGetLastError();

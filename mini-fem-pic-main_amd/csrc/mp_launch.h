@@ -31,7 +31,8 @@ struct PushArgs {
     unsigned long long* ctr;
     int cap;
     int wall;
-    bool det;
+    int det;  // 0 atomic scatter; 1 records only; 2 records at 4*slot+v (0xFFFFFFFF for a slot
+              // without one) and each node's record count added to den (unit weights)
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
@@ -67,6 +68,32 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
                            const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
                            const double* run_spwt = nullptr, int32_t nn_mesh = 0);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+// Deterministic scatter by node segments (mp_segfold.hip): the records of a det = 2 push
+// grouped by node and folded in ascending pid into out (normalised by spwt / node volume).
+struct SegFoldArgs {
+    uint32_t* keys;      // node of record 4*slot+v (0xFFFFFFFF: none); scratch afterwards
+    double* vals;        // its weight; scratch afterwards
+    const int32_t* pid;  // pid of each slot
+    int64_t m;           // records (4 * slots)
+    double* raw;         // per-node record counts summed by the push; cleared here
+    int32_t nn;
+    int32_t* tab;        // seg_table_ints(nn) ints
+    int32_t* scan_tmp;   // scan_scratch_ints(nn + 1) ints
+    uint32_t* seg_pid;   // m entries: the records grouped by node
+    double* seg_val;
+    const double* nvol;
+    double spwt;
+    double* out;
+    int pid_bits;        // bits of the largest pid
+};
+inline int64_t seg_table_ints(int32_t nn) { return 5 * (int64_t)nn + 4; }
+hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st);
+// The serial fold of every node's range [start[n], end[n]) of vals (start[n] < 0: none), in
+// order, normalised into out; ranges over 1024 are listed (long_list, long_count: zeroed by
+// the caller) and folded one wave per node.
+hipError_t launch_fold_nodes(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,
+                             int32_t nn, double spwt, const double* run_spwt, int32_t nn_mesh, double* out,
+                             int32_t* long_list, int32_t* long_count, hipStream_t st);
 int64_t scan_scratch_ints(int64_t n);
 hipError_t launch_excl_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* scratch, hipStream_t st);
 // stable compaction / pid reordering of the resident particles

@@ -6,6 +6,7 @@
 // in two passes over the records: 18-bit keys (a 60^3 box) take 2 passes of 9 bits instead
 // of the 3 passes of 8 bits rocPRIM's default width needs.  Each pass reads and writes 12 B per
 // record, so a pass saved is a third of the sort's HBM traffic.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -33,12 +34,38 @@ namespace {
 #define MP_SORT_ITEMS 20
 #endif
 
-template <unsigned Bits>
-using SortCfg = rocprim::radix_sort_config<
+// Largest record count rocPRIM sorts by block sort + merge passes instead of onesweep.  Its
+// default (1M) puts the coupled loop's 10k-200k records on a dozen merge-pass launches per
+// step; onesweep is a histogram and one launch per digit pass at any size.
+#ifndef MP_SORT_MERGE_LIMIT
+#define MP_SORT_MERGE_LIMIT 0
+#endif
+
+// Tiles for record counts up to MP_SORT_SMALL_MAX (the coupled loop's 10k-200k records): the
+// large tiles above leave a 200k-record pass on ~20 workgroups, each working through 10k
+// records, where 2k-record tiles spread it over ~100.
+#ifndef MP_SORT_SMALL_MAX
+#define MP_SORT_SMALL_MAX (1 << 22)
+#endif
+#ifndef MP_SORT_SMALL_THREADS
+#define MP_SORT_SMALL_THREADS 256
+#endif
+#ifndef MP_SORT_SMALL_ITEMS
+#define MP_SORT_SMALL_ITEMS 8
+#endif
+
+template <unsigned Bits, int HT, int HI, int ST, int SI>
+using OnesweepCfg = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<MP_SORT_HIST_THREADS, MP_SORT_HIST_ITEMS>,
-                                        rocprim::kernel_config<MP_SORT_THREADS, MP_SORT_ITEMS>, Bits,
-                                        rocprim::block_radix_rank_algorithm::match>>;
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<HT, HI>, rocprim::kernel_config<ST, SI>, Bits,
+                                        rocprim::block_radix_rank_algorithm::match>,
+    MP_SORT_MERGE_LIMIT>;
+
+template <unsigned Bits>
+using SortCfg = OnesweepCfg<Bits, MP_SORT_HIST_THREADS, MP_SORT_HIST_ITEMS, MP_SORT_THREADS, MP_SORT_ITEMS>;
+template <unsigned Bits>
+using SortCfgSmall =
+    OnesweepCfg<Bits, MP_SORT_SMALL_THREADS, MP_SORT_SMALL_ITEMS, MP_SORT_SMALL_THREADS, MP_SORT_SMALL_ITEMS>;
 
 // Digit width for an end_bit-wide key: the fewest passes of at most 11 bits, then the
 // narrowest width (>= 8) that keeps that pass count.
@@ -52,8 +79,10 @@ unsigned digit_bits(int end_bit) {
 template <unsigned Bits>
 hipError_t sort_bits(void* temp, size_t& bytes, const uint32_t* ki, uint32_t* ko, const double* vi,
                      double* vo, size_t n, int end_bit, hipStream_t st) {
-    return rocprim::radix_sort_pairs<SortCfg<Bits>>(temp, bytes, ki, ko, vi, vo, n, 0, (unsigned)end_bit,
-                                                    st);
+    if (n <= (size_t)MP_SORT_SMALL_MAX)
+        return rocprim::radix_sort_pairs<SortCfgSmall<Bits>>(temp, bytes, ki, ko, vi, vo, n, 0, (unsigned)end_bit,
+                                                             st);
+    return rocprim::radix_sort_pairs<SortCfg<Bits>>(temp, bytes, ki, ko, vi, vo, n, 0, (unsigned)end_bit, st);
 }
 
 hipError_t sort_any(void* temp, size_t& bytes, const uint32_t* ki, uint32_t* ko, const double* vi,
@@ -68,10 +97,15 @@ hipError_t sort_any(void* temp, size_t& bytes, const uint32_t* ki, uint32_t* ko,
 
 }  // namespace
 
+// Scratch for any count up to n: the small-tile configuration (more tiles, so more look-back
+// state per record) serves counts up to MP_SORT_SMALL_MAX, the large one the rest.
 size_t radix_sort_temp_bytes(int64_t n, int end_bit) {
-    size_t bytes = 0;
+    size_t bytes = 0, small = 0;
     (void)sort_any(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, (size_t)n, end_bit, nullptr);
-    return bytes;
+    if (n > (int64_t)MP_SORT_SMALL_MAX)
+        (void)sort_any(nullptr, small, nullptr, nullptr, nullptr, nullptr, (size_t)MP_SORT_SMALL_MAX, end_bit,
+                       nullptr);
+    return std::max(bytes, small);
 }
 
 hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
