@@ -91,6 +91,11 @@ typedef struct mp_options {
                                       records, else a workgroup radix sort) and folded; 0 the
                                       records are written in pid order and a device radix sort
                                       on the node bits groups them */
+    int32_t sort_pid_bits;         /* cell sort (sort_now) key: 0 (default) the bucket; b > 0
+                                      (up to 8) the bucket refined by the top b bits of a pid
+                                      digit, so the particles of a dense bucket lie in pid
+                                      ranges: the deterministic scatter's pid slabs of a dense
+                                      node then take few keys per tile */
 } mp_options;
 
 /* Counters accumulated on the device since mp_set_particles / mp_reset_counters. */
@@ -196,7 +201,7 @@ typedef struct mp_tune_trial {
     double     ms_per_step;
 } mp_tune_trial;
 /* Choose particles_per_thread, block_size, scatter_agg (atomic mode), sort_shift,
- * sort_order, sort_interval (and det_algo, keep_pid_order in deterministic mode) for the resident particles and the given step parameters:
+ * sort_order, sort_interval (and det_algo, keep_pid_order, sort_pid_bits in deterministic mode) for the resident particles and the given step parameters:
  * coordinate descent over the knobs, each candidate timed (HIP events) over a window of
  * max(interval, min_steps) steps that starts from the same saved state and holds one cell
  * sort per sort interval, as a long run does; a candidate replaces the incumbent only when

@@ -109,6 +109,7 @@ mp_options default_options() {
     o.gather_sort = 0;
     o.sort_order = 0;
     o.det_algo = 1;
+    o.sort_pid_bits = 0;
     return o;
 }
 
@@ -229,15 +230,24 @@ void pin_free(PinStage& ps) {
 // Deterministic scatter by node segments: the m slot-ordered records of a det = 2 push (and
 // the per-node counts it left in den_raw) grouped by node and folded in pid order.
 int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
-    if (!c->seg_tab && dalloc(&c->seg_tab, mpd::seg_table_ints(c->n_nodes)) != hipSuccess)
-        return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
-    int rc = reserve_scan(c, (int64_t)c->n_nodes + 1);
+    const int64_t need = mpd::seg_table_ints(c->n_nodes, m);
+    if (need > c->seg_tab_n) {
+        dfree(c->seg_tab);
+        c->seg_tab = nullptr;
+        c->seg_tab_n = 0;
+        // headroom: the record count grows with injections
+        const int64_t nc = need + need / 4;
+        if (dalloc(&c->seg_tab, nc) != hipSuccess) return set_err(c, MP_ERR_ALLOC, "segment table allocation failed");
+        c->seg_tab_n = nc;
+    }
+    int rc = reserve_scan(c, mpd::seg_scan_len(c->n_nodes, m));
     if (rc) return rc;
     mpd::SegFoldArgs a;
-    a.keys = c->keys[0];
-    a.vals = c->vals[0];
-    a.pid = c->soa[c->cur].pid;
-    a.m = m;
+    a.tets = c->tets;
+    a.soa = c->soa[c->cur];
+    a.n = c->n_slots;
+    a.tmp_pid = c->keys[0];
+    a.tmp_val = c->vals[0];
     a.raw = c->den_raw;
     a.nn = c->n_nodes;
     a.tab = c->seg_tab;
@@ -247,7 +257,9 @@ int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
     a.nvol = c->node_vol;
     a.spwt = spwt;
     a.out = c->den_out;
-    a.pid_bits = bit_width((uint64_t)std::max<int64_t>(c->next_pid - 1, 1));
+    const uint64_t p = (uint64_t)std::max<int64_t>(c->next_pid, 1);
+    a.pid_bits = bit_width(p - 1 > 0 ? p - 1 : 1);
+    a.mul = (1ull << 48) / p;
     HIP_TRY(c, mpd::launch_seg_fold(a, c->stream));
     return MP_OK;
 }
@@ -330,19 +342,32 @@ int sort_now(mp_ctx* c) {
     if (c->n_slots == 0) return MP_OK;
     const int shift = c->opt.sort_shift;
     const int64_t nb = (((int64_t)c->n_tets - 1) >> shift) + 1;
-    const int64_t nb1 = nb + 1;
+    // with pid bits every bucket splits into 2^db pid ranges
+    const int db = c->opt.sort_pid_bits;
+    const int64_t nbk = nb << db;
+    const int64_t nb1 = nbk + 1;
     int rc = reserve_scan(c, nb1);
     if (rc) return rc;
+    if (nb1 > c->cell_cap) {
+        dfree(c->cell_cnt);
+        dfree(c->cell_cur);
+        c->cell_cnt = c->cell_cur = nullptr;
+        c->cell_cap = 0;
+        if (dalloc(&c->cell_cnt, nb1) != hipSuccess || dalloc(&c->cell_cur, nb1) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "cell sort count allocation failed");
+        c->cell_cap = nb1;
+    }
+    const unsigned long long mul = (1ull << 48) / (unsigned long long)std::max<int64_t>(c->next_pid, 1);
     HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
     const Soa& src = c->soa[c->cur];
     const Soa& dst = c->soa[1 - c->cur];
     HIP_TRY(c, mpd::launch_cell_hist(src.cell, nullptr, c->n_slots, shift, (int32_t)nb, c->cell_cnt, c->stream,
-                                     sort_rank(c)));
+                                     sort_rank(c), src.pid, mul, db));
     HIP_TRY(c, mpd::launch_excl_scan(c->cell_cnt, nb1, c->cell_cur, c->scan_tmp, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nb, sizeof(int32_t), hipMemcpyDeviceToHost,
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32, c->cell_cur + nbk, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, mpd::launch_cell_scatter(src, c->n_slots, shift, (int32_t)nb, c->cell_cur, dst, c->stream,
-                                        sort_rank(c)));
+                                        sort_rank(c), mul, db));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->cur = 1 - c->cur;
     c->n_slots = *c->pinned_i32;
@@ -398,7 +423,7 @@ bool good_options(const mp_options& o) {
            o.sort_shift <= 16 &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
            (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1) &&
-           (o.det_algo == 0 || o.det_algo == 1);
+           (o.det_algo == 0 || o.det_algo == 1) && o.sort_pid_bits >= 0 && o.sort_pid_bits <= 8;
 }
 
 }  // namespace
@@ -524,6 +549,7 @@ int mp_create(const mp_tetra* tets, int32_t n_tets, const double* node_volume, i
         dalloc(&c->ctr, mpd::kNumCounters) != hipSuccess || dalloc(&c->cell_cnt, (int64_t)n_tets + 1) != hipSuccess ||
         dalloc(&c->cell_cur, (int64_t)n_tets + 1) != hipSuccess)
         return bail("Failed to allocate the mesh on the device");
+    c->cell_cap = (int64_t)n_tets + 1;
     if (hipHostMalloc((void**)&c->pinned_i32, 16 * sizeof(int32_t), 0) != hipSuccess)
         return bail("mp_create: pinned allocation failed");
     {
@@ -581,6 +607,9 @@ int mp_get_options(const mp_ctx* c, mp_options* o) {
 int mp_set_options(mp_ctx* c, const mp_options* o) {
     if (!c || !o) return MP_ERR_ARG;
     if (!good_options(*o)) return set_err(c, MP_ERR_ARG, "mp_set_options: unsupported option values");
+    // the pid-refined bucket keys are int32
+    if (((((int64_t)c->n_tets - 1) >> o->sort_shift) + 1) << o->sort_pid_bits >= ((int64_t)1 << 31))
+        return set_err(c, MP_ERR_ARG, "mp_set_options: sort_pid_bits too large for this mesh and sort_shift");
     c->opt = *o;
     return MP_OK;
 }

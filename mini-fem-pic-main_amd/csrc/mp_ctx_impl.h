@@ -51,6 +51,7 @@ struct mp_ctx {
     // counting-sort and scan scratch
     int32_t* cell_cnt = nullptr;
     int32_t* cell_cur = nullptr;
+    int64_t cell_cap = 0;  // entries of cell_cnt and cell_cur
     int32_t* scan_tmp = nullptr;
     int64_t scan_tmp_n = 0;
     // deterministic-scatter scratch
@@ -60,7 +61,8 @@ struct mp_ctx {
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int32_t* seg_bounds = nullptr;
-    int32_t* seg_tab = nullptr;  // node-segment scatter: counts, starts, cursors, hot list
+    int32_t* seg_tab = nullptr;  // node-segment scatter: counts, slab tables, node lists
+    int64_t seg_tab_n = 0;
     // drop-in AoS scratch
     mp_particle* aos = nullptr;
     mp_particle* aos_out = nullptr;

@@ -94,7 +94,7 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                     s.cell[i] = c;
                     ok = true;
                     fc = c;
-                    if (DET) put_records(keys, vals, (DET == 2 || rec_by_slot) ? i : (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
+                    if (DET == 1) put_records(keys, vals, rec_by_slot ? i : (int64_t)s.pid[i], tet_at(tets, c), l0, l1, l2, l3);
                 } else {
                     s.cell[i] = -1;
                     n_lost++;
@@ -103,9 +103,6 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                 st_(s.x + i, x); st_(s.y + i, y); st_(s.z + i, z);
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
-            // slot-indexed records: every slot writes its four, lost and empty slots the sentinel
-            if (DET == 2 && !ok)
-                *reinterpret_cast<uint4*>(keys + 4 * i) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
         }
         scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
@@ -495,6 +492,14 @@ __device__ __forceinline__ void sort_take(SortSlots& t, int32_t key, int& slot, 
     }
 }
 
+// The bucket key refined by a pid digit (db > 0): the top db bits of (pid * mul) >> 32, a
+// 16-bit digit non-decreasing in the pid (mul = floor(2^48 / P), every pid < P).
+__device__ __forceinline__ int32_t pid_refine(int32_t key, int32_t pid, unsigned long long mul, int db) {
+    if (db == 0 || key < 0) return key;
+    const uint32_t d = (uint32_t)(((unsigned long long)(uint32_t)pid * mul) >> 32);
+    return (key << db) | (int32_t)(d >> (16 - db));
+}
+
 // The histogram takes one slot per thread in 1024-thread workgroups and no wave-level
 // grouping: on near-sorted input this is faster than the tiled form (79-94 against 95-127 us
 // at 20M particles).
@@ -503,7 +508,8 @@ static_assert(kSortHash >= 2 * kHistThreads, "the LDS hash of a workgroup stays 
 
 __global__ void __launch_bounds__(kHistThreads)
 k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, const int32_t* __restrict__ trank,
-            int64_t n, int shift, int32_t nb, int32_t* __restrict__ cnt) {
+            int64_t n, int shift, int32_t nb, int32_t* __restrict__ cnt, const int32_t* __restrict__ pid,
+            unsigned long long mul, int db) {
     __shared__ SortSlots t;
     for (int q = threadIdx.x; q < kSortHash; q += kHistThreads) {
         t.key[q] = -1;
@@ -512,8 +518,9 @@ k_cell_hist(const int32_t* __restrict__ cell, const uint16_t* __restrict__ run, 
     const int64_t i = (int64_t)blockIdx.x * kHistThreads + threadIdx.x;
     const int32_t c = (i < n) ? cell[i] : -1;
     const int32_t r = (run && i < n) ? (int32_t)run[i] : 0;
+    const int32_t p = (db && i < n) ? pid[i] : 0;
     __syncthreads();
-    if (c >= 0) atomicAdd(&t.cnt[sort_slot(t, r * nb + (sort_pos(trank, c) >> shift))], 1);
+    if (c >= 0) atomicAdd(&t.cnt[sort_slot(t, pid_refine(r * nb + (sort_pos(trank, c) >> shift), p, mul, db))], 1);
     __syncthreads();
     for (int q = threadIdx.x; q < kSortHash; q += kHistThreads)
         if (t.key[q] >= 0) atomicAdd(cnt + t.key[q], t.cnt[q]);
@@ -544,7 +551,7 @@ __device__ __forceinline__ void stage_field(uint64_t* __restrict__ buf, const in
 // leave through LDS in bucket order (stage_field), field by field.
 __global__ void __launch_bounds__(kSortThreads)
 k_cell_scatter(Soa src, const int32_t* __restrict__ trank, int64_t n, int shift, int32_t nb,
-               int32_t* __restrict__ cursor, Soa dst) {
+               int32_t* __restrict__ cursor, Soa dst, unsigned long long mul, int db) {
     __shared__ SortSlots t;
     __shared__ int32_t gidx[kSortTile];
     __shared__ uint64_t buf[2][kSortTile];
@@ -574,8 +581,8 @@ k_cell_scatter(Soa src, const int32_t* __restrict__ trank, int64_t n, int shift,
     int slot[kSortItems], rank[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
-        const int32_t key =
-            c[j] >= 0 ? (src.run ? (int32_t)run[j] * nb : 0) + (sort_pos(trank, c[j]) >> shift) : -1;
+        const int32_t key = pid_refine(
+            c[j] >= 0 ? (src.run ? (int32_t)run[j] * nb : 0) + (sort_pos(trank, c[j]) >> shift) : -1, pid[j], mul, db);
         sort_take(t, key, slot[j], rank[j]);
     }
     __syncthreads();
@@ -988,6 +995,16 @@ hipError_t launch_fold_nodes(const double* vals, const int32_t* start, const int
 }
 
 
+hipError_t launch_fold_list(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,
+                            double spwt, double* out, const int32_t* list, const int32_t* count, int32_t max_count,
+                            hipStream_t st) {
+    const unsigned gl = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ((int64_t)max_count + 3) / 4));
+    // nodes of one mesh: node n's volume is nvol[n % INT32_MAX] = nvol[n]
+    hipLaunchKernelGGL(k_fold_long, dim3(gl), dim3(256), 0, st, vals, start, end, nvol, spwt, (const double*)nullptr,
+                       (int32_t)0x7FFFFFFF, out, list, count);
+    return hipGetLastError();
+}
+
 int64_t scan_scratch_ints(int64_t n) {
     int64_t total = 0;
     while (n > kScanTile) {
@@ -1021,18 +1038,19 @@ static unsigned sort_grid(int64_t n) {
 }
 
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cnt, hipStream_t st, const int32_t* trank) {
+                            int32_t* cnt, hipStream_t st, const int32_t* trank, const int32_t* pid,
+                            unsigned long long mul, int db) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n, kHistThreads)), dim3(kHistThreads), 0, st, cell, run, trank, n,
-                       shift, nb, cnt);
+                       shift, nb, cnt, pid, mul, db);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
-                               const Soa& dst, hipStream_t st, const int32_t* trank) {
+                               const Soa& dst, hipStream_t st, const int32_t* trank, unsigned long long mul, int db) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cell_scatter, dim3(sort_grid(n)), dim3(kSortThreads), 0, st, src, trank, n, shift,
-                       nb, cursor, dst);
+                       nb, cursor, dst, mul, db);
     return hipGetLastError();
 }
 

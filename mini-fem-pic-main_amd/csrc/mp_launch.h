@@ -31,8 +31,9 @@ struct PushArgs {
     unsigned long long* ctr;
     int cap;
     int wall;
-    int det;  // 0 atomic scatter; 1 records only; 2 records at 4*slot+v (0xFFFFFFFF for a slot
-              // without one) and each node's record count added to den (unit weights)
+    int det;  // 0 atomic scatter; 1 records only; 2 no records: each node's count of particles
+              // ending in its tets added to den (unit weights), the records being rebuilt from
+              // the pushed state afterwards (launch_seg_fold)
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
@@ -68,26 +69,36 @@ hipError_t launch_det_fold(const uint32_t* keys, const double* vals, int64_t m, 
                            const double* nvol, int32_t nn, double spwt, double* out, hipStream_t st,
                            const double* run_spwt = nullptr, int32_t nn_mesh = 0);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
-// Deterministic scatter by node segments (mp_segfold.hip): the records of a det = 2 push
-// grouped by node and folded in ascending pid into out (normalised by spwt / node volume).
+// Deterministic scatter by node segments (mp_segfold.hip): after a det = 2 push, every
+// particle's records rebuilt from the pushed state, grouped by node (hot nodes cut into slabs by
+// pid range) and folded in ascending pid into out (normalised by spwt / node volume).
 struct SegFoldArgs {
-    uint32_t* keys;      // node of record 4*slot+v (0xFFFFFFFF: none); scratch afterwards
-    double* vals;        // its weight; scratch afterwards
-    const int32_t* pid;  // pid of each slot
-    int64_t m;           // records (4 * slots)
+    const TetBlock* tets;
+    Soa soa;             // the pushed particles (cell < 0: none)
+    int64_t n;           // slots
+    uint32_t* tmp_pid;   // 4 * n entries of scratch (the fallback sort's second buffer)
+    double* tmp_val;
     double* raw;         // per-node record counts summed by the push; cleared here
     int32_t nn;
-    int32_t* tab;        // seg_table_ints(nn) ints
-    int32_t* scan_tmp;   // scan_scratch_ints(nn + 1) ints
-    uint32_t* seg_pid;   // m entries: the records grouped by node
+    int32_t* tab;        // seg_table_ints(nn, 4 * n) ints
+    int32_t* scan_tmp;   // scan_scratch_ints(seg_scan_len(nn, 4 * n)) ints
+    uint32_t* seg_pid;   // 4 * n entries: the records grouped by slab
     double* seg_val;
     const double* nvol;
     double spwt;
     double* out;
-    int pid_bits;        // bits of the largest pid
+    int pid_bits;            // bits of the largest pid
+    unsigned long long mul;  // floor(2^48 / P), P = the next pid to issue (>= 1)
 };
-inline int64_t seg_table_ints(int32_t nn) { return 5 * (int64_t)nn + 4; }
+int64_t seg_table_ints(int32_t nn, int64_t m);
+// the longest prefix sum launch_seg_fold takes
+int64_t seg_scan_len(int32_t nn, int64_t m);
 hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st);
+// One wave's serial fold of each listed node's range [start[n], end[n]) of vals, normalised
+// into out (at most max_count nodes listed; *count read on the device).
+hipError_t launch_fold_list(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,
+                            double spwt, double* out, const int32_t* list, const int32_t* count, int32_t max_count,
+                            hipStream_t st);
 // The serial fold of every node's range [start[n], end[n]) of vals (start[n] < 0: none), in
 // order, normalised into out; ranges over 1024 are listed (long_list, long_count: zeroed by
 // the caller) and folded one wave per node.
@@ -104,11 +115,14 @@ hipError_t launch_move_soa(const Soa& src, int64_t n, const int32_t* off, bool b
                            hipStream_t st);
 hipError_t launch_mark_pid(const int32_t* cell, const int32_t* pid, int64_t n, int32_t* present, hipStream_t st);
 // Counting sort by key = run * nb + (pos(cell) >> shift) (run term only when the Soa has runs),
-// pos(c) = trank[c] when a tet order is given, else c.
+// pos(c) = trank[c] when a tet order is given, else c; with db > 0 the key is refined to
+// (key << db) | (top db bits of the pid digit (pid * mul) >> 32), nb << db keys in all.
 hipError_t launch_cell_hist(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,
-                            int32_t* cnt, hipStream_t st, const int32_t* trank = nullptr);
+                            int32_t* cnt, hipStream_t st, const int32_t* trank = nullptr,
+                            const int32_t* pid = nullptr, unsigned long long mul = 0, int db = 0);
 hipError_t launch_cell_scatter(const Soa& src, int64_t n, int shift, int32_t nb, int32_t* cursor,
-                               const Soa& dst, hipStream_t st, const int32_t* trank = nullptr);
+                               const Soa& dst, hipStream_t st, const int32_t* trank = nullptr,
+                               unsigned long long mul = 0, int db = 0);
 // The rank pass alone: perm[d] = source slot of the particle that belongs at slot d (cursor =
 // bucket offsets, advanced); lost particles get no slot.  n < 2^31.
 hipError_t launch_cell_rank(const int32_t* cell, const uint16_t* run, int64_t n, int shift, int32_t nb,

@@ -1,22 +1,31 @@
 // Deterministic scatter by node segments.  The serial MoveParticles adds each node's weights
 // in ascending pid (src/mini-pic.cpp:297-304 run one particle after another), so the fold of
 // node n is ((0 + w_p1) + w_p2) + ... over the pids p1 < p2 < ... that end the step in a tet
-// of n.  Here the push writes every slot's four records (node, weight) at 4*slot+v, in
-// whatever order the slots are (cell-sorted for the gathers' locality), and counts the records
-// per node through its atomic scatter with unit weights.  Then:
-//   k_seg_counts   the counts as ints (the raw density buffer cleared for the next step);
-//   exclusive scan the segment of each node in the grouped record array;
-//   k_seg_bucket   each 2048-record tile groups its records by node in an LDS hash and claims
-//                  one range per node with one atomic, writing (pid, weight) into it;
-//   k_seg_sort     one wave per node: the segment's (pid, index) pairs sorted by pid in
-//                  registers (bitonic network, up to 512 records) and the weights written back
-//                  in that order;
-//   k_seg_hot_sort segments over 512 records (dense clusters): a workgroup sorts each by pid
-//                  (stable LSD radix on 8-bit digits through the record buffers);
-//   k_fold         the serial fold of each segment, one lane per node (k_fold_long: one wave
-//                  per node for the longest).
-// No pass waits on another workgroup's progress, so small record counts are not serialised
-// along a look-back chain as in a device radix sort.
+// of n.  Here the push counts each node's records through its atomic scatter with unit weights
+// and writes no records: the tile passes rebuild every particle's four records (node, weight)
+// from the pushed state, in slot order (cell-sorted for the gathers' locality).  Each node's
+// records are cut into slabs by pid range so that every slab fits one wave:
+//   k_seg_counts   the counts as ints (the raw density buffer cleared for the next step); a
+//                  node over 512 records ("hot") gets 2^b slabs, b the smallest with at most
+//                  128 records a slab on average; the nodes listed hot first, cold after;
+//   exclusive scan the first slab of each node (a cold node has one);
+//   k_slab_prep    per node (first slab, b); a cold node's slab count is its record count;
+//   k_seg_tile<1>  hot nodes only (it returns at once without): the records per slab, counted
+//                  in an LDS hash per 512-particle tile and added with one atomic per slab;
+//   exclusive scan the start of every slab in the grouped record array;
+//   k_seg_tile<0>  each 512-particle tile rebuilds its records, groups them by slab in an LDS
+//                  hash and claims one range per slab with one atomic, writing (pid, weight);
+//   k_seg_sortfold one wave per node, hot nodes first: slab after slab, each record ranked
+//                  by pid within a sub-bucket of the slab's pid range (a register bitonic
+//                  network where the pids bunch), the weights put in that order in LDS and
+//                  added by the first lane, so the fold runs through the node's records in
+//                  ascending pid; the normalised density is written at the end.
+// A slab's pids are those whose 16-bit digit d(pid) = (pid * floor(2^48 / P)) >> 32 (P = the
+// next pid to issue) has its top b bits equal to the slab's index: d is non-decreasing in the
+// pid, so slab order is pid order.  With the pids of a hot node spread over [0, P) a slab holds
+// 64..128 records on average; where they are bunched and a slab exceeds 256, the node goes to
+// a slower path (a workgroup LSD radix sort of its whole range, then a wave's serial fold).
+// No pass waits on another workgroup's progress.
 #include <algorithm>
 
 #include "mp_common.h"
@@ -26,84 +35,251 @@ namespace mpd {
 
 namespace {
 
-constexpr uint32_t kNoNode = 0xFFFFFFFFu;
-constexpr int kBucketItems = 8;
-constexpr int kBucketTile = 256 * kBucketItems;
-constexpr int kBucketBits = 12;  // LDS hash slots: twice the tile, so every key finds a slot
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+constexpr int kBucketBits = 12;  // LDS hash slots: twice a tile's records, so every key finds a slot
 constexpr int kSegWaveMax = 8 * kWave;
+constexpr int kSlabShift = 7;  // hot slabs: (64, 128] records on average
+constexpr int kBigBits = 8;    // nodes of 2^8 slabs and more (over 16k records) are listed first
+constexpr int kFineBits = 16;  // bits of the pid digit the slabs are cut from
 
-// cnt[n] = records of node n (the push's unit-weight sums, exact integers in FP64), raw[n]
-// cleared for the next step; cursors and the two list counts cleared; cnt[nn] = 0 makes the
-// scan's last entry the total.
-__global__ void k_seg_counts(double* __restrict__ raw, int32_t* __restrict__ cnt, int32_t* __restrict__ cur,
-                             int32_t* __restrict__ hot_count, int32_t* __restrict__ long_count, int32_t nn) {
+// The table of launch_seg_fold, carved from SegFoldArgs::tab (ints).
+struct SegTab {
+    int32_t* cnt;         // nn: records per node
+    int32_t* bits;        // nn: slab bits b (0: cold, one slab)
+    int32_t* nsl;         // nn + 1: 2^b, then 0
+    int32_t* sbase;       // nn + 1: first slab of each node (exclusive scan of nsl)
+    int32_t* order;       // nn: hot nodes under 2^kBigBits slabs from the front, cold nodes from the back
+    int32_t* big;         // nn: hot nodes of 2^kBigBits slabs and more
+    int32_t* ovf_list;    // nn: hot nodes with a slab over 256 records
+    int32_t* nstart;      // nn: record range of a listed node
+    int32_t* nend;        // nn
+    int2* sinfo;          // nn: (first slab, b)
+    int32_t* ctrs;        // 8: hot nodes (all), cold, overflowing, big hot, other hot, unused
+    int32_t* slab_cnt;    // smax + 1: records per slab
+    int32_t* cur;         // smax: claim cursors
+    int32_t* slab_start;  // smax + 1: exclusive scan of slab_cnt
+};
+
+// Slabs: one per cold node, and at most c / 64 for a hot node of c records, so at most
+// nn + m / 64 in all.
+int64_t slab_max(int32_t nn, int64_t m) { return (int64_t)nn + m / 64 + 1; }
+
+SegTab carve(int32_t* t, int32_t nn, int64_t m) {
+    SegTab s;
+    const int64_t n = nn, smax = slab_max(nn, m);
+    s.cnt = t;
+    s.bits = s.cnt + n;
+    s.nsl = s.bits + n;
+    s.sbase = s.nsl + n + 1;
+    s.order = s.sbase + n + 1;
+    s.big = s.order + n;
+    s.ovf_list = s.big + n;
+    s.nstart = s.ovf_list + n;
+    s.nend = s.nstart + n;
+    int32_t* p = s.nend + n;
+    p += (p - t) & 1;  // int2 alignment
+    s.sinfo = reinterpret_cast<int2*>(p);
+    s.ctrs = p + 2 * n;
+    s.slab_cnt = s.ctrs + 8;
+    s.cur = s.slab_cnt + smax + 1;
+    s.slab_start = s.cur + smax;
+    return s;
+}
+
+__device__ __forceinline__ int bit_width_u32(uint32_t v) { return v ? 32 - __clz((int)v) : 0; }
+
+__device__ __forceinline__ unsigned long long lanes_below() {
+    return (1ull << lane_id()) - 1;
+}
+
+// cnt = records of node n (the push's unit-weight sums, exact integers in FP64), raw cleared
+// for the next step; the node's slab bits; the node listed: big hot nodes in `big`, other hot
+// nodes at the front of `order`, cold nodes at its back (wave-aggregated list atomics).
+__global__ void __launch_bounds__(256)
+k_seg_counts(double* __restrict__ raw, int32_t nn, int32_t* __restrict__ cnt, int32_t* __restrict__ bits,
+             int32_t* __restrict__ nsl, int32_t* __restrict__ order, int32_t* __restrict__ big,
+             int32_t* __restrict__ ctrs) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nn) {
-        cnt[i] = (int32_t)raw[i];
+    const bool valid = i < nn;
+    int b = 0;
+    if (valid) {
+        const int32_t c = (int32_t)raw[i];
         raw[i] = 0.0;
-        cur[i] = 0;
+        cnt[i] = c;
+        if (c > kSegWaveMax) b = std::min(kFineBits, bit_width_u32((uint32_t)(c - 1) >> kSlabShift));
+        bits[i] = b;
+        nsl[i] = 1 << b;
     } else if (i == nn) {
-        cnt[nn] = 0;
-        *hot_count = 0;
-        *long_count = 0;
+        nsl[nn] = 0;
+    }
+    const bool isbig = valid && b >= kBigBits, hot = valid && b > 0 && b < kBigBits, cold = valid && b == 0;
+    const unsigned long long gm = __ballot(isbig), hm = __ballot(hot), cm = __ballot(cold);
+    int32_t gb = 0, hb = 0, cb = 0;
+    if (lane_id() == 0) {
+        if (gm | hm) atomicAdd(ctrs + 0, __popcll(gm | hm));
+        if (gm) gb = atomicAdd(ctrs + 3, __popcll(gm));
+        if (hm) hb = atomicAdd(ctrs + 4, __popcll(hm));
+        if (cm) cb = atomicAdd(ctrs + 1, __popcll(cm));
+    }
+    gb = __shfl(gb, 0, kWave);
+    hb = __shfl(hb, 0, kWave);
+    cb = __shfl(cb, 0, kWave);
+    if (isbig) big[gb + __popcll(gm & lanes_below())] = (int32_t)i;
+    if (hot) order[hb + __popcll(hm & lanes_below())] = (int32_t)i;
+    if (cold) order[nn - 1 - (cb + __popcll(cm & lanes_below()))] = (int32_t)i;
+}
+
+__global__ void __launch_bounds__(256)
+k_slab_prep(const int32_t* __restrict__ cnt, const int32_t* __restrict__ bits, const int32_t* __restrict__ sbase,
+            int32_t nn, int2* __restrict__ sinfo, int32_t* __restrict__ slab_cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int32_t b = bits[i], s = sbase[i];
+    sinfo[i] = make_int2(s, b);
+    if (b == 0) slab_cnt[s] = cnt[i];
+}
+
+// The slab of a record of a node with (first slab, b) = si.
+__device__ __forceinline__ uint32_t slab_of(int2 si, uint32_t pid, unsigned long long mul) {
+    if (si.y == 0) return (uint32_t)si.x;
+    const uint32_t d = (uint32_t)(((unsigned long long)pid * mul) >> 32);
+    return (uint32_t)si.x + (d >> (kFineBits - si.y));
+}
+
+constexpr int kTileParts = 512;  // particles of a tile (2048 records)
+constexpr int kTileItems = kTileParts / 256;
+
+// The LDS hash slot of key (at most 2048 distinct keys in 4096 slots: the probe ends at the key
+// or an empty slot).
+__device__ __forceinline__ int tile_slot(uint32_t* hk, uint32_t key) {
+    constexpr int kSlots = 1 << kBucketBits;
+    uint32_t h = (key * 0x9E3779B1u) >> (32 - kBucketBits);
+    for (int q = 0; q < kSlots; q++) {
+        const uint32_t prev = atomicCAS(&hk[h], kNoKey, key);
+        if (prev == kNoKey || prev == key) break;
+        h = (h + 1) & (kSlots - 1);
+    }
+    return (int)h;
+}
+
+// Slot and rank of this lane's key (kNoKey: none) among the tile's keys.  Called by every lane
+// of the wave: the lanes holding the wave's first valid key (on cell-sorted particles, most
+// lanes: neighbouring particles share the tet and so the node of vertex v) share the leader's
+// slot lookup and one LDS atomic, ranked by lane; the others take their own.
+__device__ __forceinline__ void tile_take(uint32_t* hk, int32_t* hc, uint32_t key, int& slot, int& rank) {
+    slot = -1;
+    rank = 0;
+    const unsigned long long valid = __ballot(key != kNoKey);
+    if (valid == 0) return;
+    const int lane = lane_id();
+    const int lead = __ffsll((long long)valid) - 1;
+    const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)key, lead);
+    const unsigned long long same = __ballot(key == k0);
+    int s0 = 0, r0 = 0;
+    if (lane == lead) {
+        s0 = tile_slot(hk, k0);
+        r0 = atomicAdd(&hc[s0], __popcll(same));
+    }
+    s0 = __builtin_amdgcn_readlane(s0, lead);
+    r0 = __builtin_amdgcn_readlane(r0, lead);
+    if (key == k0) {
+        slot = s0;
+        rank = r0 + __popcll(same & lanes_below());
+    } else if (key != kNoKey) {
+        slot = tile_slot(hk, key);
+        rank = atomicAdd(&hc[slot], 1);
     }
 }
 
-// Records [t0, t0 + 2048) of the slot-ordered array into their node segments.  A tile of
-// sorted slots covers a few hundred nodes, so the global cursors see one returning atomic per
-// node and tile; within a node's range the order is the LDS atomics' (k_seg_sort sorts it).
+// The records of particles [t0, t0 + 512) of the pushed state, by slab.  A particle still in the
+// domain (cell >= 0) ends its step in tet c at (x, y, z); its records are (con_v(c),
+// bary_v(c; x, y, z)), v = 0..3, the weights the walk computed there (same arithmetic, same
+// operands).  COUNT: hot nodes' records only, their counts added to slab_cnt (nothing to do
+// without a hot node).  Else: every record, written as (pid, weight) into its slab's range; a
+// tile of cell-sorted particles covers a few dozen tets, so the global cursors see one
+// returning atomic per slab and tile; within a slab the order is the LDS atomics' (the sort
+// puts it right).  Tiles are mapped to XCDs in contiguous eighths (xcd_block): the tiles that
+// fill one slab's range, neighbours in cell-sorted order, run on one XCD and its L2 merges their
+// partial lines.
+template <bool COUNT>
 __global__ void __launch_bounds__(256)
-k_seg_bucket(const uint32_t* __restrict__ keys, const double* __restrict__ vals, const int32_t* __restrict__ pid,
-             int64_t m, const int32_t* __restrict__ start, int32_t* __restrict__ cur,
-             uint32_t* __restrict__ out_pid, double* __restrict__ out_val) {
+k_seg_tile(const TetBlock* __restrict__ tets, Soa st, int64_t n, const int2* __restrict__ sinfo,
+           unsigned long long mul, const int32_t* __restrict__ ctrs, int32_t* __restrict__ slab_cnt,
+           const int32_t* __restrict__ slab_start, int32_t* __restrict__ cur, uint32_t* __restrict__ out_pid,
+           double* __restrict__ out_val) {
+    const bool hot = ctrs[0] != 0;
+    if (COUNT && !hot) return;
     constexpr int kSlots = 1 << kBucketBits;
     __shared__ uint32_t hk[kSlots];
     __shared__ int32_t hc[kSlots];
+    const int64_t i0 = xcd_block() * kTileParts + threadIdx.x;
+    int32_t c[kTileItems];
+    uint32_t p[kTileItems];
+    double x[kTileItems], y[kTileItems], z[kTileItems];
+#pragma unroll
+    for (int j = 0; j < kTileItems; j++) {
+        const int64_t i = i0 + j * 256;
+        c[j] = i < n ? st.cell[i] : -1;
+        p[j] = i < n ? (uint32_t)st.pid[i] : 0u;
+        x[j] = y[j] = z[j] = 0.0;
+        if (!COUNT && i < n) {
+            x[j] = ld(st.x + i);
+            y[j] = ld(st.y + i);
+            z[j] = ld(st.z + i);
+        }
+    }
     for (int q = threadIdx.x; q < kSlots; q += blockDim.x) {
-        hk[q] = kNoNode;
+        hk[q] = kNoKey;
         hc[q] = 0;
     }
-    const int64_t t0 = (int64_t)blockIdx.x * kBucketTile;
-    uint32_t k[kBucketItems];
-    double v[kBucketItems];
-    int32_t p[kBucketItems];
+    uint32_t k[4 * kTileItems];
+    double w[4 * kTileItems];
 #pragma unroll
-    for (int it = 0; it < kBucketItems; it++) {
-        const int64_t r = t0 + it * 256 + threadIdx.x;
-        const bool in = r < m;
-        k[it] = in ? keys[r] : kNoNode;
-        v[it] = in ? vals[r] : 0.0;
-        p[it] = in ? pid[r >> 2] : 0;
+    for (int j = 0; j < kTileItems; j++) {
+        const bool live = c[j] >= 0;
+        const TetView t = tet_at(tets, live ? c[j] : 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            k[4 * j + v] = live ? (uint32_t)t.con(v) : kNoKey;
+            w[4 * j + v] = 0.0;
+        }
+        if (!COUNT && live) {
+            const double vol = t.vol();
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                w[4 * j + v] = bary(t.c(v, 0), t.c(v, 1), t.c(v, 2), t.c(v, 3), x[j], y[j], z[j], vol);
+        }
+    }
+    // without a hot node every node has one slab and slab n is node n's (the scan of ones)
+    if (hot) {
+#pragma unroll
+        for (int r = 0; r < 4 * kTileItems; r++) {
+            if (k[r] == kNoKey) continue;
+            const int2 si = sinfo[k[r]];
+            k[r] = (COUNT && si.y == 0) ? kNoKey : slab_of(si, p[r / 4], mul);
+        }
     }
     __syncthreads();
-    int32_t slot[kBucketItems], rank[kBucketItems];
+    int slot[4 * kTileItems], rank[4 * kTileItems];
 #pragma unroll
-    for (int it = 0; it < kBucketItems; it++) {
-        slot[it] = -1;
-        rank[it] = 0;
-        if (k[it] == kNoNode) continue;
-        uint32_t h = (k[it] * 0x9E3779B1u) >> (32 - kBucketBits);
-        // at most 2048 distinct keys in 4096 slots: the probe ends at the key or an empty slot
-        for (int q = 0; q < kSlots; q++) {
-            const uint32_t prev = atomicCAS(&hk[h], kNoNode, k[it]);
-            if (prev == kNoNode || prev == k[it]) break;
-            h = (h + 1) & (kSlots - 1);
-        }
-        slot[it] = (int32_t)h;
-        rank[it] = atomicAdd(&hc[h], 1);
-    }
+    for (int r = 0; r < 4 * kTileItems; r++) tile_take(hk, hc, k[r], slot[r], rank[r]);
     __syncthreads();
     for (int q = threadIdx.x; q < kSlots; q += blockDim.x) {
-        const uint32_t node = hk[q];
-        if (node != kNoNode) hc[q] = start[node] + atomicAdd(&cur[node], hc[q]);
+        const uint32_t key = hk[q];
+        if (key == kNoKey) continue;
+        if (COUNT)
+            atomicAdd(slab_cnt + key, hc[q]);
+        else
+            hc[q] = slab_start[key] + atomicAdd(&cur[key], hc[q]);
     }
+    if (COUNT) return;
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kBucketItems; it++) {
-        if (slot[it] < 0) continue;
-        const int64_t pos = (int64_t)hc[slot[it]] + rank[it];
-        out_pid[pos] = (uint32_t)p[it];
-        out_val[pos] = v[it];
+    for (int r = 0; r < 4 * kTileItems; r++) {
+        if (slot[r] < 0) continue;
+        const int64_t pos = (int64_t)hc[slot[r]] + rank[r];
+        out_pid[pos] = p[r / 4];
+        out_val[pos] = w[r];
     }
 }
 
@@ -191,65 +367,290 @@ __device__ __forceinline__ void wave_bitonic(uint32_t (&key)[E], uint32_t (&idx)
     }
 }
 
-// Sorts the segment [s, s + m) of (pid, weight) records by pid and writes the weights back in
-// that order (the pids stay: only the order of the weights is read afterwards).
-template <int E>
-__device__ __forceinline__ void seg_sort_wave(const uint32_t* __restrict__ spid, double* __restrict__ sval,
-                                              int64_t s, int m, int lane, double* lv) {
+// The first lane adds lv[0..m) to acc in order, the next eight reads in flight while eight are
+// added.
+__device__ __forceinline__ double lds_fold(const double* lv, int m, double acc) {
+    int r = 0;
+    if (m >= 8) {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) w[q] = lv[q];
+        for (r = 8; r + 8 <= m; r += 8) {
+            double x[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) x[q] = lv[r + q];
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc = acc + w[q];
+#pragma unroll
+            for (int q = 0; q < 8; q++) w[q] = x[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = acc + w[q];
+    }
+    for (; r < m; r++) acc = acc + lv[r];
+    return acc;
+}
+
+// Adds a slab of 1 < m <= 64 * E records to acc in ascending pid.  Record g = e * 64 + lane
+// is (key[e], val[e]) (padding: key 0xFFFFFFFF); the wave sorts the (pid, index) pairs in
+// registers, puts the weights in sorted order in its LDS slice, and the first lane adds them
+// one after the other (acc is meaningful in lane 0 only).
+template <int E, int R>
+__device__ __forceinline__ double slab_fold_regs(const uint32_t (&key_in)[R], const double (&val)[R], int m, int lane,
+                                                 double* lv, double acc) {
     uint32_t key[E], idx[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int g = e * kWave + lane;
+        key[e] = key_in[e];
         idx[e] = (uint32_t)g;
-        if (g < m) {
-            key[e] = spid[s + g];
-            lv[g] = sval[s + g];
-        } else {
-            key[e] = kNoNode;
-        }
+        if (g < m) lv[g] = val[e];
     }
     wave_bitonic<E>(key, idx, lane);
     wave_sync_lds();
+    // sorted position g = e * 64 + lane holds record idx[e]; the pids are under 2^31, so the
+    // padding sorts behind the m records
+    double v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = e * kWave + lane < m ? lv[idx[e]] : 0.0;
+    wave_sync_lds();
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        if (e * kWave + lane < m) lv[e * kWave + lane] = v[e];
+    wave_sync_lds();
+    if (lane == 0) acc = lds_fold(lv, m, acc);
+    // lane 0's reads complete before the next slab overwrites the slice
+    wave_sync_lds();
+    return acc;
+}
+
+// Per-wave LDS of the rank sort: the pids grouped by sub-bucket, and the sub-buckets' counts
+// and starts (up to 128 each).
+struct RankLds {
+    uint32_t pid[kSegWaveMax];
+    int32_t cnt[2 * kWave];
+    int32_t base[2 * kWave];
+};
+
+// A sub-bucket longer than this sends the slab to the bitonic network instead.
+constexpr int kRankMaxBucket = 24;
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+    return v;
+}
+
+// Adds a slab of 1 < m <= 64 * E records to acc in ascending pid, by rank: the pids' range
+// [lo, hi] is cut into 16 * E sub-buckets by b = floor((pid - lo) * 16E / (hi - lo + 1)) (an
+// FP64 product of exact integers by one positive scale, so b is non-decreasing in the pid:
+// sub-bucket order is pid order); the records are grouped by sub-bucket through LDS counters,
+// and each record's rank is the number of smaller pids in its own sub-bucket (2 to 4 records on
+// average).  Its weight goes to LDS at (sub-bucket start + rank), and the first lane adds them
+// in order.  Returns false, having changed nothing, when a sub-bucket holds more than 24
+// records (pids bunched within the range): the caller then sorts the slab with the bitonic
+// network.  Record g = e * 64 + lane is (key[e], val[e]).
+template <int E, int R>
+__device__ __forceinline__ bool slab_fold_rank(const uint32_t (&key)[R], const double (&val)[R], int m, int lane,
+                                               RankLds& rl, double* lv, double& acc) {
+    constexpr int NB = 16 * E;            // sub-buckets
+    constexpr int NBL = NB > kWave ? 2 : 1;  // counters per lane
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        if (e * kWave + lane < m) {
+            lo = min(lo, key[e]);
+            hi = max(hi, key[e]);
+        }
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    const double scale = (double)NB / ((double)(hi - lo) + 1.0);
+#pragma unroll
+    for (int q = 0; q < NBL; q++)
+        if (q * kWave + lane < NB) rl.cnt[q * kWave + lane] = 0;
+    wave_sync_lds();
+    int b[E], slot[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
+        const bool valid = e * kWave + lane < m;
+        b[e] = valid ? min(NB - 1, (int)((double)(key[e] - lo) * scale)) : 0;
+        slot[e] = valid ? atomicAdd(&rl.cnt[b[e]], 1) : 0;
+    }
+    wave_sync_lds();
+    // lane l owns sub-buckets l * NBL .. l * NBL + NBL - 1
+    int c[NBL], t = 0, cmax = 0;
+#pragma unroll
+    for (int q = 0; q < NBL; q++) {
+        c[q] = lane * NBL + q < NB ? rl.cnt[lane * NBL + q] : 0;
+        cmax = max(cmax, c[q]);
+        t += c[q];
+    }
+    // exclusive scan of the counts, NBL consecutive buckets per lane
+    int inc = t;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += y;
+    }
+    cmax = (int)wave_max_u32((uint32_t)cmax);
+    if (cmax > kRankMaxBucket) return false;
+    int run = inc - t;
+#pragma unroll
+    for (int q = 0; q < NBL; q++) {
+        if (lane * NBL + q < NB) rl.base[lane * NBL + q] = run;
+        run += c[q];
+    }
+    wave_sync_lds();
+    int s0[E], n0[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const bool valid = e * kWave + lane < m;
+        s0[e] = rl.base[b[e]];
+        n0[e] = valid ? rl.cnt[b[e]] : 0;
+        if (valid) rl.pid[s0[e] + slot[e]] = key[e];
+    }
+    wave_sync_lds();
+    int r[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) r[e] = 0;
+    for (int j = 0; j < cmax; j++) {
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            if (j < n0[e]) r[e] += rl.pid[s0[e] + j] < key[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        if (e * kWave + lane < m) lv[s0[e] + r[e]] = val[e];
+    wave_sync_lds();
+    if (lane == 0) acc = lds_fold(lv, m, acc);
+    // lane 0's reads complete before the next slab overwrites the slices
+    wave_sync_lds();
+    return true;
+}
+
+// Loads the slab [s, s + m) into registers: record g = e * 64 + lane in (key[e], val[e]).
+template <int R>
+__device__ __forceinline__ void slab_load(const uint32_t* __restrict__ spid, const double* __restrict__ sval,
+                                          int64_t s, int m, int lane, uint32_t (&key)[R], double (&val)[R]) {
+#pragma unroll
+    for (int e = 0; e < R; e++) {
         const int g = e * kWave + lane;
-        if (g < m) sval[s + g] = lv[idx[e]];
+        key[e] = g < m ? spid[s + g] : kNoKey;
+        val[e] = g < m ? sval[s + g] : 0.0;
     }
 }
 
-// One wave per node (a scalar node index, so the branches on the segment length are
-// wave-uniform).  Segments of more than 512 records are listed for the hot path.
+// Adds the slab (already in registers) to acc: the smallest register count that holds it, by
+// rank, or by the bitonic network where the rank sort declines.
+template <int E, int R>
+__device__ __forceinline__ double slab_fold_e(const uint32_t (&key)[R], const double (&val)[R], int m, int lane,
+                                             RankLds& rl, double* lv, double acc) {
+    if (slab_fold_rank<E>(key, val, m, lane, rl, lv, acc)) return acc;
+    return slab_fold_regs<E>(key, val, m, lane, lv, acc);
+}
+
+template <int R>
+__device__ __forceinline__ double slab_fold(const uint32_t (&key)[R], const double (&val)[R], int m, int lane,
+                                           RankLds& rl, double* lv, double acc, const double* __restrict__ sval,
+                                           int64_t s) {
+    if (R > 4 && m > 4 * kWave) return slab_fold_e<(R > 4 ? 8 : 4)>(key, val, m, lane, rl, lv, acc);
+    if (m > 2 * kWave) return slab_fold_e<4>(key, val, m, lane, rl, lv, acc);
+    if (m > kWave) return slab_fold_e<2>(key, val, m, lane, rl, lv, acc);
+    if (m > 1) return slab_fold_e<1>(key, val, m, lane, rl, lv, acc);
+    if (m == 1 && lane == 0) acc = acc + sval[s];
+    return acc;
+}
+
+// One wave per node: the big hot nodes first, then the other hot nodes, then the cold ones (or,
+// without a hot node, wave q takes node q).  The node index is a scalar, so every branch on a
+// slab's length is wave-uniform.  Slab j + 1 is loaded into registers while slab j is sorted
+// and folded.  A hot node with a slab over 256 records is listed for the fallback with its
+// record range instead.
 __global__ void __launch_bounds__(256)
-k_seg_sort(const uint32_t* __restrict__ spid, double* __restrict__ sval, const int32_t* __restrict__ start,
-           int32_t nn, int32_t* __restrict__ hot_list, int32_t* __restrict__ hot_count) {
+k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval, const int32_t* __restrict__ sbase,
+               const int32_t* __restrict__ slab_start, const int32_t* __restrict__ order,
+               const int32_t* __restrict__ big, int32_t* __restrict__ ctrs, int32_t nn,
+               const double* __restrict__ nvol, double spwt, double* __restrict__ out,
+               int32_t* __restrict__ ovf_list, int32_t* __restrict__ nstart, int32_t* __restrict__ nend) {
     __shared__ double lval[256 / kWave][kSegWaveMax];
+    __shared__ RankLds lrank[256 / kWave];
     const int lane = lane_id();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-    const int64_t n = (int64_t)blockIdx.x * (256 / kWave) + w;
-    if (n >= nn) return;
-    const int64_t s = start[n];
-    const int m = start[n + 1] - (int32_t)s;
-    if (m > kSegWaveMax) {
-        if (lane == 0) hot_list[atomicAdd(hot_count, 1)] = (int32_t)n;
-    } else if (m > 4 * kWave) {
-        seg_sort_wave<8>(spid, sval, s, m, lane, lval[w]);
-    } else if (m > 2 * kWave) {
-        seg_sort_wave<4>(spid, sval, s, m, lane, lval[w]);
-    } else if (m > kWave) {
-        seg_sort_wave<2>(spid, sval, s, m, lane, lval[w]);
-    } else if (m > 1) {
-        seg_sort_wave<1>(spid, sval, s, m, lane, lval[w]);
+    const int32_t q = (int32_t)((int64_t)blockIdx.x * (256 / kWave) + w);
+    if (q >= nn) return;
+    const int32_t n_hot = __builtin_amdgcn_readfirstlane(ctrs[0]);
+    int32_t n = q;
+    if (n_hot) {
+        const int32_t n_big = __builtin_amdgcn_readfirstlane(ctrs[3]);
+        n = q < n_big ? big[q] : q < n_hot ? order[q - n_big] : order[q];
+        n = __builtin_amdgcn_readfirstlane(n);
     }
+    const int32_t b0 = __builtin_amdgcn_readfirstlane(sbase[n]);
+    const int32_t ns = __builtin_amdgcn_readfirstlane(sbase[n + 1]) - b0;
+    double* lv = lval[w];
+    RankLds& rl = lrank[w];
+    double acc = 0.0;
+    if (ns == 1) {
+        const int64_t s = __builtin_amdgcn_readfirstlane(slab_start[b0]);
+        const int m = __builtin_amdgcn_readfirstlane(slab_start[b0 + 1]) - (int32_t)s;
+        uint32_t key[8];
+        double val[8];
+        slab_load(spid, sval, s, m, lane, key, val);
+        acc = slab_fold(key, val, m, lane, rl, lv, acc, sval, s);
+    } else if (ns > 1) {
+        // a hot node's slabs hold 64..128 records on average; one over 256 (pids bunched within
+        // the node's range) sends the node to the fallback
+        bool over = false;
+        for (int32_t j = lane; j < ns; j += kWave) over |= slab_start[b0 + j + 1] - slab_start[b0 + j] > 4 * kWave;
+        if (__ballot(over)) {
+            if (lane == 0) {
+                ovf_list[atomicAdd(ctrs + 2, 1)] = n;
+                nstart[n] = slab_start[b0];
+                nend[n] = slab_start[b0 + ns];
+            }
+            return;
+        }
+        // slab j + 1 is loaded into the prefetch registers while slab j is folded
+        uint32_t pk[4];
+        double pv[4];
+        int64_t ps = __builtin_amdgcn_readfirstlane(slab_start[b0]);
+        int pm = __builtin_amdgcn_readfirstlane(slab_start[b0 + 1]) - (int32_t)ps;
+        slab_load(spid, sval, ps, pm, lane, pk, pv);
+        for (int32_t j = 0; j < ns; j++) {
+            uint32_t ck[4];
+            double cv[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                ck[e] = pk[e];
+                cv[e] = pv[e];
+            }
+            const int64_t s = ps;
+            const int m = pm;
+            if (j + 1 < ns) {
+                ps = __builtin_amdgcn_readfirstlane(slab_start[b0 + j + 1]);
+                pm = __builtin_amdgcn_readfirstlane(slab_start[b0 + j + 2]) - (int32_t)ps;
+                slab_load(spid, sval, ps, pm, lane, pk, pv);
+            }
+            acc = slab_fold(ck, cv, m, lane, rl, lv, acc, sval, s);
+        }
+    }
+    if (lane == 0) out[n] = acc * (spwt / nvol[n]);
 }
 
-// Each workgroup takes listed hot segments in turn and sorts each by pid: `passes` (even) LSD
-// passes of 8 bits, 256 records per round, stable within a round by wave ballots (rank among
-// the lanes with the same digit) and per-wave digit counts in LDS.  Even passes end in
-// (pa, va); (pb, vb) is scratch at the same offsets.
+// The fallback for a hot node with a slab over 256 records (pids bunched within its range):
+// each workgroup takes listed nodes in turn and sorts the node's whole range [start, end) by
+// pid with `passes` (even) LSD passes of 8 bits, 256 records per round, stable within a round
+// by wave ballots (rank among the lanes with the same digit) and per-wave digit counts in LDS.
+// Even passes end in (pa, va); (pb, vb) is scratch at the same offsets.
 __global__ void __launch_bounds__(256)
 k_seg_hot_sort(uint32_t* __restrict__ pa, double* __restrict__ va, uint32_t* __restrict__ pb,
-               double* __restrict__ vb, const int32_t* __restrict__ start, const int32_t* __restrict__ hot_list,
-               const int32_t* __restrict__ hot_count, int passes) {
+               double* __restrict__ vb, const int32_t* __restrict__ start, const int32_t* __restrict__ end,
+               const int32_t* __restrict__ hot_list, const int32_t* __restrict__ hot_count, int passes) {
     __shared__ int32_t base[256];
     __shared__ int32_t wcnt[256 / kWave][256];
     const int tid = threadIdx.x, lane = lane_id();
@@ -258,7 +659,7 @@ k_seg_hot_sort(uint32_t* __restrict__ pa, double* __restrict__ va, uint32_t* __r
     for (int32_t h = blockIdx.x; h < count; h += gridDim.x) {
         const int32_t n = hot_list[h];
         const int64_t s = start[n];
-        const int64_t m = start[n + 1] - s;
+        const int64_t m = end[n] - s;
         uint32_t *sp = pa + s, *dp = pb + s;
         double *sv = va + s, *dv = vb + s;
         for (int pass = 0; pass < passes; pass++) {
@@ -332,36 +733,52 @@ unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
 
+int64_t seg_table_ints(int32_t nn, int64_t m) {
+    return 12 * (int64_t)nn + 16 + 3 * (slab_max(nn, m) + 1);
+}
+
+int64_t seg_scan_len(int32_t nn, int64_t m) { return slab_max(nn, m) + 1; }
+
 hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st) {
     const int32_t nn = a.nn;
-    int32_t* cnt = a.tab;
-    int32_t* start = cnt + (int64_t)nn + 1;
-    int32_t* cur = start + (int64_t)nn + 1;
-    int32_t* hot_list = cur + nn;
-    int32_t* hot_count = hot_list + nn;
-    int32_t* long_list = hot_count + 1;
-    int32_t* long_count = long_list + nn;
-    hipLaunchKernelGGL(k_seg_counts, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, a.raw, cnt, cur,
-                       hot_count, long_count, nn);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = launch_excl_scan(cnt, (int64_t)nn + 1, start, a.scan_tmp, st);
+    const int64_t m = 4 * a.n;
+    const int64_t smax = slab_max(nn, m);
+    const SegTab t = carve(a.tab, nn, m);
+    // counters, slab counts and cursors start at zero (one contiguous range)
+    hipError_t e = launch_fill_i32(t.ctrs, 8 + (smax + 1) + smax, 0, st);
     if (e != hipSuccess) return e;
-    if (a.m > 0)
-        hipLaunchKernelGGL(k_seg_bucket, dim3(grid_for(a.m, kBucketTile)), dim3(256), 0, st, a.keys, a.vals, a.pid,
-                           a.m, start, cur, a.seg_pid, a.seg_val);
-    hipLaunchKernelGGL(k_seg_sort, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid, a.seg_val, start,
-                       nn, hot_list, hot_count);
-    // hot segments: as many workgroups as a few per CU, each taking listed nodes in turn (none
-    // listed: they return at once)
-    const int passes = std::max(2, ((a.pid_bits + 7) / 8 + 1) / 2 * 2);
-    const unsigned gh = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nn));
-    hipLaunchKernelGGL(k_seg_hot_sort, dim3(gh), dim3(256), 0, st, a.seg_pid, a.seg_val, a.keys, a.vals, start,
-                       hot_list, hot_count, passes);
+    hipLaunchKernelGGL(k_seg_counts, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, a.raw, nn, t.cnt, t.bits,
+                       t.nsl, t.order, t.big, t.ctrs);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_excl_scan(t.nsl, (int64_t)nn + 1, t.sbase, a.scan_tmp, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_slab_prep, dim3(grid_for(nn, 256)), dim3(256), 0, st, t.cnt, t.bits, t.sbase, nn, t.sinfo,
+                       t.slab_cnt);
+    // tiles of 512 particles, the grid a multiple of the XCD count (xcd_block)
+    const unsigned gt = (unsigned)((grid_for(a.n, kTileParts) + kXcds - 1) / kXcds * kXcds);
+    if (a.n > 0)
+        hipLaunchKernelGGL(k_seg_tile<true>, dim3(gt), dim3(256), 0, st, a.tets, a.soa, a.n, t.sinfo, a.mul, t.ctrs,
+                           t.slab_cnt, t.slab_start, t.cur, a.seg_pid, a.seg_val);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_excl_scan(t.slab_cnt, smax + 1, t.slab_start, a.scan_tmp, st);
+    if (e != hipSuccess) return e;
+    if (a.n > 0)
+        hipLaunchKernelGGL(k_seg_tile<false>, dim3(gt), dim3(256), 0, st, a.tets, a.soa, a.n, t.sinfo, a.mul, t.ctrs,
+                           t.slab_cnt, t.slab_start, t.cur, a.seg_pid, a.seg_val);
+    hipLaunchKernelGGL(k_seg_sortfold, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid, a.seg_val,
+                       t.sbase, t.slab_start, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list,
+                       t.nstart, t.nend);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // every segment now holds its weights in ascending pid: the serial folds
-    return launch_fold_nodes(a.seg_val, start, start + 1, a.nvol, nn, a.spwt, nullptr, nn, a.out, long_list,
-                             long_count, st);
+    // the fallback (none listed: its workgroups return at once): sort each listed node's range
+    // by pid, then fold it with one wave
+    const int passes = std::max(2, ((a.pid_bits + 7) / 8 + 1) / 2 * 2);
+    const unsigned gh = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nn));
+    hipLaunchKernelGGL(k_seg_hot_sort, dim3(gh), dim3(256), 0, st, a.seg_pid, a.seg_val, a.tmp_pid, a.tmp_val, t.nstart,
+                       t.nend, t.ovf_list, t.ctrs + 2, passes);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fold_list(a.seg_val, t.nstart, t.nend, a.nvol, a.spwt, a.out, t.ovf_list, t.ctrs + 2, nn, st);
 }
 
 }  // namespace mpd

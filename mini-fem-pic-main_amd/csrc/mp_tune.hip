@@ -193,6 +193,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         if (mode == MP_SCATTER_DETERMINISTIC) {
             sweep([](mp_options& o, int v) { o.det_algo = v; }, {0, 1});
             sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
+            sweep([](mp_options& o, int v) { o.sort_pid_bits = v; }, {0, 2, 4, 6});
         }
         sweep([](mp_options& o, int v) { o.sort_interval = v; }, {4, 6, 8, 10, 14, 20, 30});
     }

@@ -49,7 +49,8 @@ class MpOptions(C.Structure):
     _fields_ = [("sort_interval", C.c_int32), ("wall_mode", C.c_int32), ("block_size", C.c_int32),
                 ("particles_per_thread", C.c_int32), ("walk_cap", C.c_int32), ("scatter_agg", C.c_int32),
                 ("sort_shift", C.c_int32), ("keep_pid_order", C.c_int32),
-                ("gather_sort", C.c_int32), ("sort_order", C.c_int32), ("det_algo", C.c_int32)]
+                ("gather_sort", C.c_int32), ("sort_order", C.c_int32), ("det_algo", C.c_int32),
+                ("sort_pid_bits", C.c_int32)]
 
 
 class MpTuneTrial(C.Structure):

@@ -48,7 +48,7 @@ def test_struct_layouts(mp):
     assert mp.PARTICLE_DTYPE.itemsize == 88 and mp.PARTICLE_DTYPE.fields["cell_index"][1] == 80
     assert mp.PARTICLE_DTYPE.fields["lc"][1] == 48
     assert mp.FACE_DTYPE.itemsize == 104
-    assert ctypes.sizeof(mp.MpOptions) == 44 and ctypes.sizeof(mp.MpTuneTrial) == 56
+    assert ctypes.sizeof(mp.MpOptions) == 48 and ctypes.sizeof(mp.MpTuneTrial) == 56
 
 
 def test_create_rejects_bad_tables(mp):
