@@ -110,11 +110,20 @@ constexpr int kHashSlots = 1024;  // tables keyed by tet (one insert per particl
 #define MP_NODE_HASH_BITS 8
 #endif
 constexpr int kNodeSlots = 1 << MP_NODE_HASH_BITS;
+// 1 (default): the hash probe loops stay loops.  Fully unrolled, each of the 32 probes nests
+// an exec-mask save: the push then needs 106 SGPRs and spills 264 of them to VGPR lanes (38
+// SGPRs and no spill rolled, and 1% faster).
+#ifndef MP_PROBE_ROLLED
+#define MP_PROBE_ROLLED 1
+#endif
 constexpr int kNodeProbe = 32;
 
 __device__ __forceinline__ void lds_insert(int32_t* hk, double* hv, double* __restrict__ den, int32_t node,
                                            double w) {
     uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - MP_NODE_HASH_BITS);
+#if MP_PROBE_ROLLED
+#pragma unroll 1
+#endif
     for (int p = 0; p < kNodeProbe; p++) {
         const int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
@@ -187,6 +196,106 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
                                             double l1, double l2, double l3, uint32_t base = 0) {
     put_records_nodes(keys, vals, pid, base + (uint32_t)tf.con(0), base + (uint32_t)tf.con(1),
                       base + (uint32_t)tf.con(2), base + (uint32_t)tf.con(3), l0, l1, l2, l3);
+}
+
+// Deterministic records placed by the push (det_algo 2): node n's records go straight into its
+// segment [base[n], base[n + 1]) of the grouped record arrays, sized from the previous step's
+// count with slack.  Every thread of the block calls it.  The block groups its records by node
+// in an LDS hash (one insert per wave for the lanes that share the wave's first node, as
+// neighbouring cell-sorted particles do), claims one range per node with one returning atomic
+// on the node's cursor, and writes (pid, weight) at base + claim + rank; a record that finds no
+// slot within the probe limit claims its place alone.  A record past its segment's capacity is
+// not written and raises flags[0]: the caller then rebuilds the step's records exactly.  The
+// cursors end as the nodes' record counts.
+#ifndef MP_PLACE_BITS
+#define MP_PLACE_BITS 10
+#endif
+constexpr int kPlaceBits = MP_PLACE_BITS;
+constexpr int kPlaceSlots = 1 << kPlaceBits;
+constexpr int kPlaceProbe = 32;
+
+__device__ __forceinline__ int place_slot(int32_t* hk, int32_t node) {
+    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - kPlaceBits);
+#if MP_PROBE_ROLLED
+#pragma unroll 1
+#endif
+    for (int p = 0; p < kPlaceProbe; p++) {
+        const int32_t prev = atomicCAS(&hk[h], -1, node);
+        if (prev == -1 || prev == node) return (int)h;
+        h = (h + 1) & (kPlaceSlots - 1);
+    }
+    return -1;
+}
+
+// Slot (-1: none) and rank of this lane's node (-1: no record) in the block's table; every lane
+// of the wave calls it.
+__device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t node, int& slot, int& rank) {
+    slot = -1;
+    rank = 0;
+    const unsigned long long valid = __ballot(node >= 0);
+    if (valid == 0) return;
+    const int lane = lane_id();
+    const int lead = __ffsll((long long)valid) - 1;
+    const int32_t k0 = __builtin_amdgcn_readlane(node, lead);
+    const unsigned long long same = __ballot(node == k0);
+    int s0 = -1, r0 = 0;
+    if (lane == lead) {
+        s0 = place_slot(hk, k0);
+        if (s0 >= 0) r0 = atomicAdd(&hc[s0], __popcll(same));
+    }
+    s0 = __builtin_amdgcn_readlane(s0, lead);
+    r0 = __builtin_amdgcn_readlane(r0, lead);
+    if (node == k0) {
+        if (s0 >= 0) {
+            slot = s0;
+            rank = r0 + __popcll(same & ((1ull << lane) - 1ull));
+        }
+    } else if (node >= 0) {
+        slot = place_slot(hk, node);
+        if (slot >= 0) rank = atomicAdd(&hc[slot], 1);
+    }
+}
+
+__device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets, bool ok, int32_t fc, uint32_t pid,
+                                              double l0, double l1, double l2, double l3, const PlaceArgs& pl,
+                                              uint32_t* __restrict__ out_pid, double* __restrict__ out_val) {
+    __shared__ int32_t hk[kPlaceSlots];
+    __shared__ int32_t hc[kPlaceSlots];
+    for (int s = threadIdx.x; s < kPlaceSlots; s += blockDim.x) {
+        hk[s] = -1;
+        hc[s] = 0;
+    }
+    int32_t nd[4] = {-1, -1, -1, -1};
+    if (ok) {
+        const TetView t = tet_at(tets, fc);
+        nd[0] = t.con(0);
+        nd[1] = t.con(1);
+        nd[2] = t.con(2);
+        nd[3] = t.con(3);
+    }
+    __syncthreads();
+    int slot[4], rank[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) place_take(hk, hc, nd[v], slot[v], rank[v]);
+    __syncthreads();
+    for (int s = threadIdx.x; s < kPlaceSlots; s += blockDim.x) {
+        const int32_t node = hk[s];
+        if (node >= 0) hc[s] = atomicAdd(pl.cur + node, hc[s]);
+    }
+    __syncthreads();
+    if (!ok) return;
+    const double w[4] = {l0, l1, l2, l3};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int32_t g = slot[v] >= 0 ? hc[slot[v]] + rank[v] : atomicAdd(pl.cur + nd[v], 1);
+        const int32_t b = pl.base[nd[v]];
+        if (g < pl.base[nd[v] + 1] - b) {
+            out_pid[b + g] = pid;
+            out_val[b + g] = w[v];
+        } else {
+            pl.flags[0] = 1;
+        }
+    }
 }
 
 // The atomic scatter a kernel's template arguments select (nothing when DET == 1: the records

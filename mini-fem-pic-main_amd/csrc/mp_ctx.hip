@@ -108,7 +108,7 @@ mp_options default_options() {
     o.keep_pid_order = 0;
     o.gather_sort = 0;
     o.sort_order = 0;
-    o.det_algo = 1;
+    o.det_algo = 2;
     o.sort_pid_bits = 0;
     return o;
 }
@@ -261,6 +261,100 @@ int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
     a.pid_bits = bit_width(p - 1 > 0 ? p - 1 : 1);
     a.mul = (1ull << 48) / p;
     HIP_TRY(c, mpd::launch_seg_fold(a, c->stream));
+    return MP_OK;
+}
+
+// Records placed by the push (det_algo 2).  place_mem: cursors (nn), flags (8), capacities
+// (nn + 1), then the two segment layouts.
+namespace {
+struct PlaceBufs {
+    int32_t* cur;
+    int32_t* flags;
+    int32_t* cap;
+    int32_t* base[2];
+};
+PlaceBufs place_bufs(mp_ctx* c) {
+    const int64_t nn = c->n_nodes;
+    PlaceBufs p;
+    p.cur = c->place_mem;
+    p.flags = p.cur + nn;
+    p.cap = p.flags + 8;
+    p.base[0] = p.cap + nn + 1;
+    p.base[1] = p.base[0] + nn + 1;
+    return p;
+}
+// pinned_i32 slots of the placed path's flags: 8 no fit, 9 a node over 512, 10 the next
+// layout's size, 11 the exact path's hot count
+constexpr int kPinPlace = 8;
+}  // namespace
+
+int reserve_place(mp_ctx* c) {
+    if (c->place_mem) return MP_OK;
+    const int64_t nn = c->n_nodes;
+    if (dalloc(&c->place_mem, nn + 8 + 3 * (nn + 1)) != hipSuccess)
+        return set_err(c, MP_ERR_ALLOC, "placed-record tables: allocation failed");
+    if (!c->place_ev && hipEventCreateWithFlags(&c->place_ev, hipEventDisableTiming) != hipSuccess)
+        return set_err(c, MP_ERR_HIP, "placed-record event creation failed");
+    return reserve_scan(c, nn + 1);
+}
+
+// Whether the next step can place its records: the last layout is known, no node of the last
+// step held over 512 records, and the layout fits the record buffers.
+bool place_usable(mp_ctx* c) {
+    if (c->place_pending) {
+        if (hipEventSynchronize(c->place_ev) != hipSuccess) return false;
+        c->place_pending = false;
+        c->place_total = c->pinned_i32[kPinPlace + 2];
+        c->place_ready = c->pinned_i32[kPinPlace + 3] == 0;
+    }
+    return c->place_ready && c->place_total <= c->rec_cap;
+}
+
+// After an exact (det_algo 1 path) step with det_algo 2: the next step's layout from its counts.
+int place_prepare(mp_ctx* c) {
+    int rc = reserve_place(c);
+    if (rc) return rc;
+    const PlaceBufs p = place_bufs(c);
+    const int32_t nn = c->n_nodes;
+    HIP_TRY(c, mpd::launch_place_layout(c->seg_tab, nn, p.cap, p.base[c->place_buf], c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, p.base[c->place_buf] + nn, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 3, mpd::seg_hot_count(c->seg_tab, nn), sizeof(int32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->place_ev, c->stream));
+    c->place_pending = true;
+    return MP_OK;
+}
+
+// After a placing push: the counts checked and the next layout scanned; the placed records
+// folded (on the device, while the host waits for the check); a step whose records did not all
+// fit, or with a node over 512 records, is rebuilt exactly.
+int place_finish(mp_ctx* c, double spwt) {
+    const PlaceBufs p = place_bufs(c);
+    const int32_t nn = c->n_nodes;
+    int32_t* bnow = p.base[c->place_buf];
+    int32_t* bnext = p.base[1 - c->place_buf];
+    HIP_TRY(c, mpd::launch_place_check(p.cur, p.flags, p.cap, nn, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(p.cap, (int64_t)nn + 1, bnext, c->scan_tmp, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace, p.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, bnext + nn, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipEventRecord(c->place_ev, c->stream));
+    HIP_TRY(c, mpd::launch_place_fold(c->keys[1], c->vals[1], bnow, p.cur, p.flags, nn, c->node_vol, spwt,
+                                      c->den_out, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->place_ev));
+    const bool no_fit = c->pinned_i32[kPinPlace] != 0, hot = c->pinned_i32[kPinPlace + 1] != 0;
+    c->place_buf = 1 - c->place_buf;
+    c->place_total = c->pinned_i32[kPinPlace + 2];
+    c->place_ready = !hot;
+    c->place_pending = false;
+    if (no_fit || hot) {
+        c->det_rebuilt++;
+        HIP_TRY(c, mpd::launch_place_to_raw(p.cur, c->den_raw, nn, c->stream));
+        return det_finish_seg(c, 4 * c->n_slots, spwt);
+    }
+    c->det_placed++;
     return MP_OK;
 }
 
@@ -423,7 +517,7 @@ bool good_options(const mp_options& o) {
            o.sort_shift <= 16 &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
            (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1) &&
-           (o.det_algo == 0 || o.det_algo == 1) && o.sort_pid_bits >= 0 && o.sort_pid_bits <= 8;
+           (o.det_algo >= 0 && o.det_algo <= 2) && o.sort_pid_bits >= 0 && o.sort_pid_bits <= 8;
 }
 
 }  // namespace
@@ -585,13 +679,14 @@ void mp_destroy(mp_ctx* c) {
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
                     c->inj_tmp, c->cmp_buf, c->den_steps, c->perm, c->ef_stage,
-                    c->live_cnt, c->tet_rank, c->seg_tab};
+                    c->live_cnt, c->tet_rank, c->seg_tab, c->place_mem};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     pin_free(c->pin_ef);
     pin_free(c->pin_inj);
     mpstage::stager_free(c->stage);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->place_ev) (void)hipEventDestroy(c->place_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -737,6 +832,9 @@ int mp_set_particles(mp_ctx* c, const double* pos, const double* vel, const int3
     c->n_slots = 0;
     c->next_pid = 0;
     c->pid_ordered = true;
+    c->place_ready = false;
+    c->place_pending = false;
+    c->det_placed = c->det_rebuilt = 0;
     HIP_TRY(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters, c->stream));
     return mp_add_particles(c, pos, vel, cell, n);
 }
@@ -809,15 +907,17 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         if (rc) return rc;
     }
     const bool det = mode == MP_SCATTER_DETERMINISTIC;
-    const bool seg = det && c->opt.det_algo == 1;
-    // node segments: records at 4*slot+v, every slot writing its own; radix sort: 4*slot+v
-    // while the slots are in pid order, else 4*pid+v for every pid issued (sentinels pre-filled)
+    const bool seg = det && (c->opt.det_algo == 1 || c->opt.det_algo == 2);
+    // node segments: 4*slot+v record slots (rebuilt from the state, or placed by the push with
+    // det_algo 2, the segments sized from the last step's counts); radix sort: 4*slot+v while
+    // the slots are in pid order, else 4*pid+v for every pid issued (sentinels pre-filled)
     const int64_t n_rec = (seg || c->pid_ordered) ? 4 * c->n_slots : 4 * c->next_pid;
     if (det) {
         int rc = reserve_records(c, n_rec, bit_width((uint64_t)c->n_nodes), !seg);
         if (rc) return rc;
         if (!seg) HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, c->stream));
     }
+    const bool place = seg && c->opt.det_algo == 2 && place_usable(c);
     {
         int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
         if (rc) return rc;
@@ -829,13 +929,18 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.qm = q_over_m;
     a.dt = dt;
     a.den = c->den_raw;
-    a.keys = det ? c->keys[0] : nullptr;
-    a.vals = det ? c->vals[0] : nullptr;
+    a.keys = place ? c->keys[1] : det ? c->keys[0] : nullptr;
+    a.vals = place ? c->vals[1] : det ? c->vals[0] : nullptr;
     a.part = c->tally;
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;
     a.wall = c->opt.wall_mode;
-    a.det = seg ? 2 : det ? 1 : 0;
+    a.det = place ? 3 : seg ? 2 : det ? 1 : 0;
+    if (place) {
+        const PlaceBufs p = place_bufs(c);
+        a.place = mpd::PlaceArgs{p.base[c->place_buf], p.cur, p.flags};
+        HIP_TRY(c, mpd::launch_fill_i32(p.cur, (int64_t)c->n_nodes + 8, 0, c->stream));
+    }
     a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
     a.ppt = c->opt.particles_per_thread;
@@ -843,8 +948,13 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     if (c->timing) HIP_TRY(c, record_event(c));
     HIP_TRY(c, mpd::launch_push_soa(a, c->stream));
     if (c->timing) HIP_TRY(c, record_event(c));
-    if (seg) {
+    if (place) {
+        int rc = place_finish(c, spwt);
+        if (rc) return rc;
+    } else if (seg) {
+        c->det_rebuilt++;
         int rc = det_finish_seg(c, n_rec, spwt);
+        if (!rc && c->opt.det_algo == 2) rc = place_prepare(c);
         if (rc) return rc;
     } else if (det) {
         int rc = det_finish(c, n_rec, spwt);
@@ -1050,6 +1160,13 @@ int mp_get_density(mp_ctx* c, double* den) {
     return MP_OK;
 }
 
+int mp_get_det_paths(mp_ctx* c, int64_t* placed, int64_t* rebuilt) {
+    if (!c) return MP_ERR_ARG;
+    if (placed) *placed = c->det_placed;
+    if (rebuilt) *rebuilt = c->det_rebuilt;
+    return MP_OK;
+}
+
 int mp_get_counters(mp_ctx* c, mp_counters* out) {
     if (!c || !out) return MP_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
@@ -1067,6 +1184,7 @@ int mp_get_counters(mp_ctx* c, mp_counters* out) {
 
 int mp_reset_counters(mp_ctx* c) {
     if (!c) return MP_ERR_ARG;
+    c->det_placed = c->det_rebuilt = 0;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * mpd::kNumCounters, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

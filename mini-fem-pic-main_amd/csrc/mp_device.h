@@ -85,6 +85,13 @@ struct Soa {
     uint16_t* run;  // batch: run of each particle (nullptr for a single run)
 };
 
+// The node segments a det_algo 2 push places its records into (mp_common.h, place_records).
+struct PlaceArgs {
+    const int32_t* base;  // nn + 1: segment starts (capacity of n: base[n + 1] - base[n])
+    int32_t* cur;         // nn: claim cursors, zero before the push
+    int32_t* flags;       // [0]: a record did not fit
+};
+
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };
 
 // Most steps one multi-step launch takes (its 2048-slot LDS hash holds ~40 keys per step and

@@ -51,7 +51,7 @@ template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
-           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot) {
+           int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot, PlaceArgs pl) {
     const int ppt = MULTI ? ppt_arg : 1;
     const int64_t b0 = xcd_block() * ppt;
     int n_act = 0, n_lost = 0, n_cross = 0, n_moves = 0, n_cap = 0, n_refl = 0;
@@ -59,6 +59,7 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
         const int64_t i = (b0 + k) * blockDim.x + threadIdx.x;
         bool ok = false;
         int32_t fc = -1;
+        uint32_t pid = 0;
         double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
         if (i < n) {
             // The particle's state is loaded with its cell, not after it: only the start tet
@@ -66,6 +67,7 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
             // cell < 0, evaluates tet 0 with a cap of 0 and its results are dropped), so the
             // compiler cannot sink the state or start-tet loads behind a wait.
             const int32_t c_in = s.cell[i];
+            if (DET == 3) pid = (uint32_t)s.pid[i];
             double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
             const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
             // keep the six state loads issued ahead of the cell's first use (the scheduler
@@ -104,7 +106,10 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
         }
-        scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
+        if constexpr (DET == 3)
+            place_records(tets, ok, fc, pid, l0, l1, l2, l3, pl, keys, vals);
+        else
+            scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
     tally_counts(part, n_act, n_lost, n_cross, n_moves, n_cap, n_refl);
 }
@@ -887,14 +892,16 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     do {                                                                                                  \
         if (ppt == 1)                                                                                     \
             hipLaunchKernelGGL((k_push_soa<W, D, G, false>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n, \
-                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1, a.rec_by_slot);       \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, 1, a.rec_by_slot, a.place); \
         else                                                                                              \
             hipLaunchKernelGGL((k_push_soa<W, D, G, true>), dim3(g), dim3(bs), 0, st, a.tets, a.soa, a.n,  \
-                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt, a.rec_by_slot);     \
+                               a.qm, a.dt, a.den, a.keys, a.vals, a.part, a.cap, ppt, a.rec_by_slot,     \
+                               a.place);                                                                  \
     } while (0)
 #define MP_PUSH_W(W)                                      \
     do {                                                  \
-        if (a.det == 2 && a.agg == 2) MP_PUSH(W, 2, 2);   \
+        if (a.det == 3) MP_PUSH(W, 3, 0);                 \
+        else if (a.det == 2 && a.agg == 2) MP_PUSH(W, 2, 2); \
         else if (a.det == 2) MP_PUSH(W, 2, 0);            \
         else if (a.det) MP_PUSH(W, 1, 0);                 \
         else if (a.agg == 2) MP_PUSH(W, 0, 2);            \

@@ -33,11 +33,13 @@ struct PushArgs {
     int wall;
     int det;  // 0 atomic scatter; 1 records only; 2 no records: each node's count of particles
               // ending in its tets added to den (unit weights), the records being rebuilt from
-              // the pushed state afterwards (launch_seg_fold)
+              // the pushed state afterwards (launch_seg_fold); 3 the records placed into the node
+              // segments of `place` (keys: pids, vals: weights)
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
     bool rec_by_slot;  // deterministic records at 4*slot+v (slots in pid order), else 4*pid+v
+    PlaceArgs place{};  // det 3 only
     // launch_push_steps only: with perm set, particle i is read from slot perm[i] of src and
     // written to slot i of soa (the gather half of a cell sort)
     Soa src{};
@@ -94,6 +96,22 @@ int64_t seg_table_ints(int32_t nn, int64_t m);
 // the longest prefix sum launch_seg_fold takes
 int64_t seg_scan_len(int32_t nn, int64_t m);
 hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st);
+// Records placed by the push (det_algo 2), mp_segfold.hip.  After the push (the cursors cur
+// are the counts): flags[1] = a node over 512 records, cap = the next step's segment capacities
+// (nn + 1 ints, the last 0).
+hipError_t launch_place_check(const int32_t* cur, int32_t* flags, int32_t* cap, int32_t nn, hipStream_t st);
+// The counts into raw as FP64, the exact path's input, for a placed step that is rebuilt.
+hipError_t launch_place_to_raw(const int32_t* cur, double* raw, int32_t nn, hipStream_t st);
+// The next step's segment starts (nn + 1) from the counts launch_seg_fold left in its table.
+hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int32_t* cap, int32_t* base, int32_t* scan_tmp,
+                               hipStream_t st);
+// Where launch_seg_fold leaves its count of hot nodes (one int).
+const int32_t* seg_hot_count(const int32_t* tab, int32_t nn);
+// One wave per node: the placed records in pid order, folded into out; nothing when flags[0] or
+// flags[1] is set.
+hipError_t launch_place_fold(const uint32_t* spid, const double* sval, const int32_t* base, const int32_t* cnt,
+                             const int32_t* flags, int32_t nn, const double* nvol, double spwt, double* out,
+                             hipStream_t st);
 // One wave's serial fold of each listed node's range [start[n], end[n]) of vals, normalised
 // into out (at most max_count nodes listed; *count read on the device).
 hipError_t launch_fold_list(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,

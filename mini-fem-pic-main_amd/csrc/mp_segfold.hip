@@ -729,6 +729,64 @@ k_seg_hot_sort(uint32_t* __restrict__ pa, double* __restrict__ va, uint32_t* __r
     }
 }
 
+// ---- records placed by the push (det_algo 2) -------------------------------------------------
+
+// A node's segment for the next step: its count now, a quarter more and 32 (the counts of a node
+// change by a few per cent from one step to the next).
+__device__ __forceinline__ int32_t place_capacity(int32_t c) { return c + (c >> 2) + 32; }
+
+// After a placing push: the cursors are the counts; a node over 512 records raises flags[1],
+// and the next step's capacities are written (cap[nn] = 0 for the scan's total).
+__global__ void __launch_bounds__(256)
+k_place_check(const int32_t* __restrict__ cur, int32_t* __restrict__ flags, int32_t* __restrict__ cap, int32_t nn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) {
+        const int32_t c = cur[i];
+        if (c > kSegWaveMax) flags[1] = 1;
+        cap[i] = place_capacity(c);
+    } else if (i == nn) {
+        cap[nn] = 0;
+    }
+}
+
+// The counts as the exact path's input (FP64 unit sums), when a placed step is rebuilt.
+__global__ void __launch_bounds__(256)
+k_place_to_raw(const int32_t* __restrict__ cur, double* __restrict__ raw, int32_t nn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) raw[i] = (double)cur[i];
+}
+
+// The capacities from the exact path's counts.
+__global__ void __launch_bounds__(256)
+k_place_cap(const int32_t* __restrict__ cnt, int32_t* __restrict__ cap, int32_t nn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) cap[i] = place_capacity(cnt[i]);
+    else if (i == nn) cap[nn] = 0;
+}
+
+// One wave per node: its placed records [base[n], base[n] + cnt[n]) (at most 512) in ascending
+// pid and folded, the normalised density written.  Nothing is done when the step's records did
+// not all fit, or a node exceeds 512 (flags): the exact path rebuilds the step.
+__global__ void __launch_bounds__(256)
+k_place_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval,
+                 const int32_t* __restrict__ base, const int32_t* __restrict__ cnt, const int32_t* __restrict__ flags,
+                 int32_t nn, const double* __restrict__ nvol, double spwt, double* __restrict__ out) {
+    __shared__ double lval[256 / kWave][kSegWaveMax];
+    __shared__ RankLds lrank[256 / kWave];
+    if (flags[0] | flags[1]) return;
+    const int lane = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int64_t n = (int64_t)blockIdx.x * (256 / kWave) + w;
+    if (n >= nn) return;
+    const int64_t s = __builtin_amdgcn_readfirstlane(base[n]);
+    const int m = __builtin_amdgcn_readfirstlane(cnt[n]);
+    uint32_t key[8];
+    double val[8];
+    slab_load(spid, sval, s, m, lane, key, val);
+    const double acc = slab_fold(key, val, m, lane, lrank[w], lval[w], 0.0, sval, s);
+    if (lane == 0) out[n] = acc * (spwt / nvol[n]);
+}
+
 unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -779,6 +837,35 @@ hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st) {
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_fold_list(a.seg_val, t.nstart, t.nend, a.nvol, a.spwt, a.out, t.ovf_list, t.ctrs + 2, nn, st);
+}
+
+hipError_t launch_place_check(const int32_t* cur, int32_t* flags, int32_t* cap, int32_t nn, hipStream_t st) {
+    hipLaunchKernelGGL(k_place_check, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, cur, flags, cap, nn);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_to_raw(const int32_t* cur, double* raw, int32_t nn, hipStream_t st) {
+    hipLaunchKernelGGL(k_place_to_raw, dim3(grid_for(nn, 256)), dim3(256), 0, st, cur, raw, nn);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int32_t* cap, int32_t* base, int32_t* scan_tmp,
+                               hipStream_t st) {
+    // the counts are the first nn ints of the table (k_seg_counts)
+    hipLaunchKernelGGL(k_place_cap, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, tab, cap, nn);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_excl_scan(cap, (int64_t)nn + 1, base, scan_tmp, st);
+    return e;
+}
+
+const int32_t* seg_hot_count(const int32_t* tab, int32_t nn) { return carve(const_cast<int32_t*>(tab), nn, 0).ctrs; }
+
+hipError_t launch_place_fold(const uint32_t* spid, const double* sval, const int32_t* base, const int32_t* cnt,
+                             const int32_t* flags, int32_t nn, const double* nvol, double spwt, double* out,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_place_sortfold, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, spid, sval, base, cnt,
+                       flags, nn, nvol, spwt, out);
+    return hipGetLastError();
 }
 
 }  // namespace mpd

@@ -191,7 +191,7 @@ extern "C" int mp_autotune(mp_ctx* c, double q_over_m, double dt, double spwt, i
         sweep([](mp_options& o, int v) { o.block_size = v; }, {64, 128, 256});
         sweep([](mp_options& o, int v) { o.particles_per_thread = v; }, {1, 2, 4, 8});
         if (mode == MP_SCATTER_DETERMINISTIC) {
-            sweep([](mp_options& o, int v) { o.det_algo = v; }, {0, 1});
+            sweep([](mp_options& o, int v) { o.det_algo = v; }, {0, 1, 2});
             sweep([](mp_options& o, int v) { o.keep_pid_order = v; }, {0, 1});
             sweep([](mp_options& o, int v) { o.sort_pid_bits = v; }, {0, 2, 4, 6});
         }

@@ -97,6 +97,7 @@ SIGNATURES = {
     "mp_get_density": (C.c_int, [_P, _P]),
     "mp_get_counters": (C.c_int, [_P, C.POINTER(MpCounters)]),
     "mp_reset_counters": (C.c_int, [_P]),
+    "mp_get_det_paths": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mp_sort_now": (C.c_int, [_P]),
     "mp_stream": (_P, [_P]),
     "mp_set_timing": (C.c_int, [_P, _I32]),
@@ -624,6 +625,13 @@ class Mover:
 
     def reset_counters(self):
         self._check(lib().mp_reset_counters(self._h))
+
+    def det_paths(self):
+        """(placed, rebuilt): deterministic steps whose records the push placed into node segments
+        (det_algo 2), and steps whose records were rebuilt from the pushed state."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(lib().mp_get_det_paths(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def stream(self):
         return lib().mp_stream(self._h)

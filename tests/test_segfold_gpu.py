@@ -1,10 +1,13 @@
-"""GPU parity of the two deterministic scatters against the serial oracle.
+"""GPU parity of the deterministic scatters against the serial oracle.
 
-det_algo 1 (the default) writes the records by slot and puts each node's segment in pid
-order on chip: a wave's register bitonic network for up to 512 records (8 per lane), a
-workgroup LSD radix sort on 8-bit pid digits above that; det_algo 0 writes them in pid order
-and sorts them on the node bits with a device radix sort.  Either way the density must be
-the ascending-pid fold of the serial MoveParticles, bit for bit.
+det_algo 2 (the default) has the push place the records into node segments sized from the
+last step's counts (a step that does not fit is rebuilt as with 1).  det_algo 1 rebuilds each
+particle's records from the pushed state and groups them into node segments (dense nodes cut
+into pid slabs).  Either way each segment is put in pid order on chip: a wave ranks records
+within sub-buckets of the pid range (a register bitonic network where the pids bunch), and a
+node with a slab over 256 records goes to a workgroup LSD radix sort.  det_algo 0 writes the
+records in pid order and sorts them on the node bits with a device radix sort.  Every way the density must be the
+ascending-pid fold of the serial MoveParticles, bit for bit.
 """
 import numpy as np
 import pytest
@@ -37,11 +40,12 @@ def check_state(mv, run, k):
 
 @pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
 @pytest.mark.parametrize("pid_order", [0, 1], ids=["cell-sorted", "pid-order"])
-@pytest.mark.parametrize("algo", [0, 1], ids=["radix", "segments"])
+@pytest.mark.parametrize("algo", [0, 1, 2], ids=["radix", "segments", "placed"])
 def test_det_algos_match_oracle(mp, orc, algo, pid_order, wall):
-    """Both algorithms with cell-sorted slots and with pid-order compaction, 6 steps with a
-    periodic pass every 2 steps and an injection midway; the algorithm is switched for one
-    step in the middle."""
+    """Every algorithm with cell-sorted slots and with pid-order compaction, 6 steps with a
+    periodic pass every 2 steps and an injection midway (a third more particles: the placed
+    records of det_algo 2 overflow their segments and the step is rebuilt); the algorithm is
+    switched for one step in the middle."""
     mesh, ef, pos, vel, cell = box_case(mp, seed=71 + 2 * wall + pid_order)
     half = 45000
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos[:half], vel[:half], cell[:half], wall_mode=wall)
@@ -57,7 +61,7 @@ def test_det_algos_match_oracle(mp, orc, algo, pid_order, wall):
                 run.vel = np.ascontiguousarray(np.concatenate([run.vel, vel[half:]]))
                 run.cell = np.ascontiguousarray(np.concatenate([run.cell, cell[half:]]).astype(np.int32))
                 run.pid = np.concatenate([run.pid, np.arange(half, len(pos))])
-            mv.set_options(det_algo=1 - algo if k == 4 else algo)
+            mv.set_options(det_algo=(algo + 1) % 3 if k == 4 else algo)
             mv.step(QM, 6e-7, SPWT, DET)
             ref = run.step(QM, 6e-7, SPWT)
             assert mv.density().tobytes() == ref.tobytes(), f"step {k}: density not bit-exact"
@@ -89,16 +93,17 @@ def segment_case(mp, fillers, seed=3):
     return mesh, np.zeros((mesh.n_tets, 3)), pos, vel, mesh.locate_box(pos)
 
 
+@pytest.mark.parametrize("algo", [1, 2], ids=["segments", "placed"])
 @pytest.mark.parametrize("sort_interval", [0, 1], ids=["slots-in-pid-order", "cell-sorted"])
 @pytest.mark.parametrize("fillers", [0, 70000], ids=["pids-13bit", "pids-17bit"])
-def test_segment_lengths_at_every_boundary(mp, orc, fillers, sort_interval):
+def test_segment_lengths_at_every_boundary(mp, orc, fillers, sort_interval, algo):
     """Node segments of exactly 1, 63..65, 127..129, 255..257, 511..513, 700 and 3000 records, in
     slots left in pid order or cell-sorted; pids under 2^13 (two radix passes on the long
     segments) or from 70000 up (four)."""
     mesh, ef, pos, vel, cell = segment_case(mp, fillers)
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=sort_interval, det_algo=1)
+        mv.set_options(sort_interval=sort_interval, det_algo=algo)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         for k in range(3):
@@ -111,10 +116,12 @@ def test_segment_lengths_at_every_boundary(mp, orc, fillers, sort_interval):
     assert run.pid.min() == fillers and len(run.pid) == sum(LENGTHS)
 
 
-def test_segments_with_hot_nodes_and_empty_steps(mp, orc):
+@pytest.mark.parametrize("algo", [1, 2], ids=["segments", "placed"])
+def test_segments_with_hot_nodes_and_empty_steps(mp, orc, algo):
     """A dense cluster (dozens of nodes over 512 records, the hottest over 6 x 10^4) beside a
     smaller one, then every particle absorbed: the step after that has no records at all and
-    must leave a zero density."""
+    must leave a zero density.  With det_algo 2 the hot nodes keep the steps on the exact path
+    (the placed push is tried only after a step without them)."""
     side = 0.1
     mesh = mp.Mesh.box_mesh(10, side)
     ef = mesh.field_sinusoid(10.0)
@@ -123,7 +130,7 @@ def test_segments_with_hot_nodes_and_empty_steps(mp, orc):
     cell = mesh.locate_box(pos)
     run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
     with mp.Mover.from_mesh(mesh) as mv:
-        mv.set_options(sort_interval=2, det_algo=1)
+        mv.set_options(sort_interval=2, det_algo=algo)
         mv.upload_field(ef)
         mv.set_particles(pos, vel, cell)
         for k in range(4):
@@ -139,3 +146,38 @@ def test_segments_with_hot_nodes_and_empty_steps(mp, orc):
         assert mv.num_active() == 0
         mv.step(QM, 1e-6, SPWT, DET)
         assert not mv.density().any()
+
+
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_placed_records_match_oracle(mp, orc, wall):
+    """det_algo 2 on a sparse 10^3 box (about 200 records per interior node): the first step is
+    rebuilt (no segment layout yet), the next ones are placed by the push; an injection of a
+    third more particles overflows the segments once (that step is rebuilt) and placement
+    resumes.  Every step bit-exact against the serial fold."""
+    mesh = mp.Mesh.box_mesh(10, 0.1)
+    ef = mesh.field_sinusoid(10.0)
+    # no drift against a reflecting wall: particles would pile up there into nodes over 512
+    drift = [5000.0, 0, 0] if wall == 0 else [0.0, 0, 0]
+    pos, vel = mp.synth_uniform(91 + wall, 52000, [0, 0, 0], [0.1] * 3, 2000.0, drift)
+    cell = mesh.locate_box(pos)
+    half = 39000
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos[:half], vel[:half], cell[:half], wall_mode=wall)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=3, wall_mode=wall, det_algo=2)
+        mv.upload_field(ef)
+        mv.set_particles(pos[:half], vel[:half], cell[:half])
+        for k in range(9):
+            if k == 5:
+                mv.add_particles(pos[half:], vel[half:], cell[half:])
+                run.pos = np.ascontiguousarray(np.concatenate([run.pos, pos[half:]]))
+                run.vel = np.ascontiguousarray(np.concatenate([run.vel, vel[half:]]))
+                run.cell = np.ascontiguousarray(np.concatenate([run.cell, cell[half:]]).astype(np.int32))
+                run.pid = np.concatenate([run.pid, np.arange(half, len(pos))])
+            mv.step(QM, 6e-7, SPWT, DET)
+            ref = run.step(QM, 6e-7, SPWT)
+            assert mv.density().tobytes() == ref.tobytes(), f"step {k}: density not bit-exact"
+            check_state(mv, run, k)
+        placed, rebuilt = mv.det_paths()
+        per_node = np.bincount(mesh.tets["con"][run.cell].ravel(), minlength=mesh.n_nodes)
+    assert per_node.max() <= 512
+    assert placed >= 6 and rebuilt >= 2 and placed + rebuilt == 9, (placed, rebuilt)
