@@ -334,6 +334,9 @@ template <int BITS = kStepBits>
 __device__ __forceinline__ void step_insert(uint32_t* hk, double* hv, double* __restrict__ den, uint32_t key,
                                             double w) {
     uint32_t h = (key * 0x9E3779B1u) >> (32 - BITS);
+#if MP_PROBE_ROLLED
+#pragma unroll 1
+#endif
     for (int p = 0; p < kStepProbe; p++) {
         const uint32_t prev = atomicCAS(&hk[h], 0xFFFFFFFFu, key);
         if (prev == 0xFFFFFFFFu || prev == key) {

@@ -154,6 +154,9 @@ constexpr int kTileItems = kTileParts / 256;
 __device__ __forceinline__ int tile_slot(uint32_t* hk, uint32_t key) {
     constexpr int kSlots = 1 << kBucketBits;
     uint32_t h = (key * 0x9E3779B1u) >> (32 - kBucketBits);
+#if MP_PROBE_ROLLED
+#pragma unroll 1
+#endif
     for (int q = 0; q < kSlots; q++) {
         const uint32_t prev = atomicCAS(&hk[h], kNoKey, key);
         if (prev == kNoKey || prev == key) break;
