@@ -438,15 +438,42 @@ struct RankLds {
 // A sub-bucket longer than this sends the slab to the bitonic network instead.
 constexpr int kRankMaxBucket = 24;
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+// Wave-wide scans through DPP (VALU lane moves, no LDS round trip): within each row of 16
+// lanes by row_shr 1, 2, 4 and 8, then across rows by row_bcast:15 (lane 15 of each row into
+// the next; rows 1 and 3) and row_bcast:31 (lane 31 into rows 2 and 3).  A lane whose source
+// lies outside its row, or whose row is masked, takes the identity id.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t id, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+template <class Op>
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t v, uint32_t id, Op op) {
+    v = op(v, dpp_u32<0x111>(id, v));
+    v = op(v, dpp_u32<0x112>(id, v));
+    v = op(v, dpp_u32<0x114>(id, v));
+    v = op(v, dpp_u32<0x118>(id, v));
+    v = op(v, dpp_u32<0x142, 0xA>(id, v));
+    v = op(v, dpp_u32<0x143, 0xC>(id, v));
     return v;
 }
+
+struct OpMin {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return min(a, b); }
+};
+struct OpMax {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return max(a, b); }
+};
+struct OpAdd {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+
+// The wave's minimum / maximum (uniform: the last lane's inclusive scan).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32(v, 0xFFFFFFFFu, OpMin{}), kWave - 1);
+}
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, kWave));
-    return v;
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32(v, 0u, OpMax{}), kWave - 1);
 }
 
 // Adds a slab of 1 < m <= 64 * E records to acc in ascending pid, by rank: the pids' range
@@ -494,12 +521,7 @@ __device__ __forceinline__ bool slab_fold_rank(const uint32_t (&key)[R], const d
         t += c[q];
     }
     // exclusive scan of the counts, NBL consecutive buckets per lane
-    int inc = t;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(inc, o, kWave);
-        if (lane >= o) inc += y;
-    }
+    const int inc = (int)wave_scan_u32((uint32_t)t, 0u, OpAdd{});
     cmax = (int)wave_max_u32((uint32_t)cmax);
     if (cmax > kRankMaxBucket) return false;
     int run = inc - t;
