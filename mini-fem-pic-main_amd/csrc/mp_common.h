@@ -198,15 +198,28 @@ __device__ __forceinline__ void put_records(uint32_t* __restrict__ keys, double*
                       base + (uint32_t)tf.con(2), base + (uint32_t)tf.con(3), l0, l1, l2, l3);
 }
 
-// Deterministic records placed by the push (det_algo 2): node n's records go straight into its
-// segment [base[n], base[n + 1]) of the grouped record arrays, sized from the previous step's
-// count with slack.  Every thread of the block calls it.  The block groups its records by node
-// in an LDS hash (one insert per wave for the lanes that share the wave's first node, as
-// neighbouring cell-sorted particles do), claims one range per node with one returning atomic
-// on the node's cursor, and writes (pid, weight) at base + claim + rank; a record that finds no
+// Slabs of a node's records: a node with (first slab, b) = si owns slabs si.x .. si.x + 2^b - 1;
+// a record of pid p falls in slab si.x + (d(p) >> (kSlabFineBits - b)), d(p) = (p * mul) >> 32
+// the 16-bit pid digit (mul = floor(2^48 / P) for pids below P).  d is non-decreasing in p, so
+// slab order is pid order within a node.
+constexpr int kSlabFineBits = 16;
+
+__device__ __forceinline__ uint32_t slab_of(int2 si, uint32_t pid, unsigned long long mul) {
+    if (si.y == 0) return (uint32_t)si.x;
+    const uint32_t d = (uint32_t)(((unsigned long long)pid * mul) >> 32);
+    return (uint32_t)si.x + (d >> (kSlabFineBits - si.y));
+}
+
+// Deterministic records placed by the push (det_algo 2): a record goes straight into its slab's
+// segment [base[s], base[s + 1]) of the grouped record arrays, sized from the previous step's
+// count with slack (a cold node's one slab is the node; a hot node's slab is picked by the pid
+// digit, pl.sinfo).  Every thread of the block calls it.  The block groups its records by slab
+// in an LDS hash (one insert per wave for the lanes that share the wave's first slab, as
+// neighbouring cell-sorted particles do), claims one range per slab with one returning atomic on
+// the slab's cursor, and writes (pid, weight) at base + claim + rank; a record that finds no
 // slot within the probe limit claims its place alone.  A record past its segment's capacity is
 // not written and raises flags[0]: the caller then rebuilds the step's records exactly.  The
-// cursors end as the nodes' record counts.
+// cursors end as the slabs' record counts.
 #ifndef MP_PLACE_BITS
 #define MP_PLACE_BITS 10
 #endif
@@ -214,30 +227,30 @@ constexpr int kPlaceBits = MP_PLACE_BITS;
 constexpr int kPlaceSlots = 1 << kPlaceBits;
 constexpr int kPlaceProbe = 32;
 
-__device__ __forceinline__ int place_slot(int32_t* hk, int32_t node) {
-    uint32_t h = ((uint32_t)node * 0x9E3779B1u) >> (32 - kPlaceBits);
+__device__ __forceinline__ int place_slot(int32_t* hk, int32_t key) {
+    uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - kPlaceBits);
 #if MP_PROBE_ROLLED
 #pragma unroll 1
 #endif
     for (int p = 0; p < kPlaceProbe; p++) {
-        const int32_t prev = atomicCAS(&hk[h], -1, node);
-        if (prev == -1 || prev == node) return (int)h;
+        const int32_t prev = atomicCAS(&hk[h], -1, key);
+        if (prev == -1 || prev == key) return (int)h;
         h = (h + 1) & (kPlaceSlots - 1);
     }
     return -1;
 }
 
-// Slot (-1: none) and rank of this lane's node (-1: no record) in the block's table; every lane
-// of the wave calls it.
-__device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t node, int& slot, int& rank) {
+// Slot (-1: none) and rank of this lane's slab key (-1: no record) in the block's table; every
+// lane of the wave calls it.
+__device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t key, int& slot, int& rank) {
     slot = -1;
     rank = 0;
-    const unsigned long long valid = __ballot(node >= 0);
+    const unsigned long long valid = __ballot(key >= 0);
     if (valid == 0) return;
     const int lane = lane_id();
     const int lead = __ffsll((long long)valid) - 1;
-    const int32_t k0 = __builtin_amdgcn_readlane(node, lead);
-    const unsigned long long same = __ballot(node == k0);
+    const int32_t k0 = __builtin_amdgcn_readlane(key, lead);
+    const unsigned long long same = __ballot(key == k0);
     int s0 = -1, r0 = 0;
     if (lane == lead) {
         s0 = place_slot(hk, k0);
@@ -245,13 +258,13 @@ __device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t nod
     }
     s0 = __builtin_amdgcn_readlane(s0, lead);
     r0 = __builtin_amdgcn_readlane(r0, lead);
-    if (node == k0) {
+    if (key == k0) {
         if (s0 >= 0) {
             slot = s0;
             rank = r0 + __popcll(same & ((1ull << lane) - 1ull));
         }
-    } else if (node >= 0) {
-        slot = place_slot(hk, node);
+    } else if (key >= 0) {
+        slot = place_slot(hk, key);
         if (slot >= 0) rank = atomicAdd(&hc[slot], 1);
     }
 }
@@ -272,6 +285,10 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
         nd[1] = t.con(1);
         nd[2] = t.con(2);
         nd[3] = t.con(3);
+        if (pl.sinfo) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) nd[v] = (int32_t)slab_of(pl.sinfo[nd[v]], pid, pl.mul);
+        }
     }
     __syncthreads();
     int slot[4], rank[4];
@@ -279,8 +296,8 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
     for (int v = 0; v < 4; v++) place_take(hk, hc, nd[v], slot[v], rank[v]);
     __syncthreads();
     for (int s = threadIdx.x; s < kPlaceSlots; s += blockDim.x) {
-        const int32_t node = hk[s];
-        if (node >= 0) hc[s] = atomicAdd(pl.cur + node, hc[s]);
+        const int32_t key = hk[s];
+        if (key >= 0) hc[s] = atomicAdd(pl.cur + key, hc[s]);
     }
     __syncthreads();
     if (!ok) return;

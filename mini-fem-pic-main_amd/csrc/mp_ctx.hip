@@ -227,6 +227,11 @@ void pin_free(PinStage& ps) {
     ps = PinStage{};
 }
 
+// The pid digit multiplier of the slab cut, floor(2^48 / P) for the pids issued so far (below P).
+unsigned long long pid_digit_mul(const mp_ctx* c) {
+    return (1ull << 48) / (unsigned long long)std::max<int64_t>(c->next_pid, 1);
+}
+
 // Deterministic scatter by node segments: the m slot-ordered records of a det = 2 push (and
 // the per-node counts it left in den_raw) grouped by node and folded in pid order.
 int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
@@ -259,101 +264,135 @@ int det_finish_seg(mp_ctx* c, int64_t m, double spwt) {
     a.out = c->den_out;
     const uint64_t p = (uint64_t)std::max<int64_t>(c->next_pid, 1);
     a.pid_bits = bit_width(p - 1 > 0 ? p - 1 : 1);
-    a.mul = (1ull << 48) / p;
+    a.mul = pid_digit_mul(c);
     HIP_TRY(c, mpd::launch_seg_fold(a, c->stream));
     return MP_OK;
 }
 
-// Records placed by the push (det_algo 2).  place_mem: cursors (nn), flags (8), capacities
-// (nn + 1), then the two segment layouts.
+// Records placed by the push (det_algo 2), in the slab layout of the last exact step.
+// place_mem: flags (8), then cursors, capacities, limits and the two segment layouts, smax + 1
+// ints each (smax = seg_slab_max of that step).
 namespace {
 struct PlaceBufs {
-    int32_t* cur;
     int32_t* flags;
+    int32_t* cur;
     int32_t* cap;
+    int32_t* lim;
     int32_t* base[2];
 };
 PlaceBufs place_bufs(mp_ctx* c) {
-    const int64_t nn = c->n_nodes;
+    const int64_t l = c->place_smax + 1;
     PlaceBufs p;
-    p.cur = c->place_mem;
-    p.flags = p.cur + nn;
-    p.cap = p.flags + 8;
-    p.base[0] = p.cap + nn + 1;
-    p.base[1] = p.base[0] + nn + 1;
+    p.flags = c->place_mem;
+    p.cur = p.flags + 8;
+    p.cap = p.cur + l;
+    p.lim = p.cap + l;
+    p.base[0] = p.lim + l;
+    p.base[1] = p.base[0] + l;
     return p;
 }
-// pinned_i32 slots of the placed path's flags: 8 no fit, 9 a node over 512, 10 the next
+// pinned_i32 slots of the placed path's flags: 8 no fit, 9 a slab over its limit, 10 the next
 // layout's size, 11 the exact path's hot count
 constexpr int kPinPlace = 8;
 }  // namespace
 
-int reserve_place(mp_ctx* c) {
-    if (c->place_mem) return MP_OK;
-    const int64_t nn = c->n_nodes;
-    if (dalloc(&c->place_mem, nn + 8 + 3 * (nn + 1)) != hipSuccess)
-        return set_err(c, MP_ERR_ALLOC, "placed-record tables: allocation failed");
+int reserve_place(mp_ctx* c, int64_t smax) {
+    const int64_t need = 8 + 5 * (smax + 1);
+    if (need > c->place_mem_n) {
+        dfree(c->place_mem);
+        c->place_mem = nullptr;
+        c->place_mem_n = 0;
+        const int64_t nc = need + need / 4;  // headroom: the slab count grows with the records
+        if (dalloc(&c->place_mem, nc) != hipSuccess)
+            return set_err(c, MP_ERR_ALLOC, "placed-record tables: allocation failed");
+        c->place_mem_n = nc;
+    }
     if (!c->place_ev && hipEventCreateWithFlags(&c->place_ev, hipEventDisableTiming) != hipSuccess)
         return set_err(c, MP_ERR_HIP, "placed-record event creation failed");
-    return reserve_scan(c, nn + 1);
+    return reserve_scan(c, smax + 1);
 }
 
-// Whether the next step can place its records: the last layout is known, no node of the last
-// step held over 512 records, and the layout fits the record buffers.
+// Whether the next step can place its records: a layout from the last step's counts exists and
+// fits the record buffers.
 bool place_usable(mp_ctx* c) {
     if (c->place_pending) {
         if (hipEventSynchronize(c->place_ev) != hipSuccess) return false;
         c->place_pending = false;
         c->place_total = c->pinned_i32[kPinPlace + 2];
-        c->place_ready = c->pinned_i32[kPinPlace + 3] == 0;
+        c->place_hot = c->pinned_i32[kPinPlace + 3] != 0;
     }
     return c->place_ready && c->place_total <= c->rec_cap;
 }
 
-// After an exact (det_algo 1 path) step with det_algo 2: the next step's layout from its counts.
-int place_prepare(mp_ctx* c) {
-    int rc = reserve_place(c);
-    if (rc) return rc;
-    const PlaceBufs p = place_bufs(c);
+// After an exact (det_algo 1 path) step of m records with det_algo 2: its slab layout kept for the
+// next steps, and the next step's segments from its slab counts.
+int place_prepare(mp_ctx* c, int64_t m) {
     const int32_t nn = c->n_nodes;
-    HIP_TRY(c, mpd::launch_place_layout(c->seg_tab, nn, p.cap, p.base[c->place_buf], c->scan_tmp, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, p.base[c->place_buf] + nn, sizeof(int32_t),
+    const int64_t smax = mpd::seg_slab_max(nn, m);
+    int rc = reserve_place(c, smax);
+    if (rc) return rc;
+    c->place_m = m;
+    c->place_smax = smax;
+    const PlaceBufs p = place_bufs(c);
+    HIP_TRY(c, mpd::launch_place_layout(c->seg_tab, nn, m, p.lim, p.cap, p.base[c->place_buf], c->scan_tmp,
+                                        c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, p.base[c->place_buf] + smax, sizeof(int32_t),
                               hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 3, mpd::seg_hot_count(c->seg_tab, nn), sizeof(int32_t),
                               hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipEventRecord(c->place_ev, c->stream));
     c->place_pending = true;
+    c->place_ready = true;
     return MP_OK;
 }
 
-// After a placing push: the counts checked and the next layout scanned; the placed records
+// After a placing push: the slab counts checked and the next layout scanned; the placed records
 // folded (on the device, while the host waits for the check); a step whose records did not all
-// fit, or with a node over 512 records, is rebuilt exactly.
+// fit, or with a hot node's slab over its limit, is rebuilt exactly, and its counts give the next
+// layout.
 int place_finish(mp_ctx* c, double spwt) {
     const PlaceBufs p = place_bufs(c);
     const int32_t nn = c->n_nodes;
+    const int64_t smax = c->place_smax;
     int32_t* bnow = p.base[c->place_buf];
     int32_t* bnext = p.base[1 - c->place_buf];
-    HIP_TRY(c, mpd::launch_place_check(p.cur, p.flags, p.cap, nn, c->stream));
-    HIP_TRY(c, mpd::launch_excl_scan(p.cap, (int64_t)nn + 1, bnext, c->scan_tmp, c->stream));
+    HIP_TRY(c, mpd::launch_place_check(c->seg_tab, nn, smax, p.cur, p.lim, p.flags, p.cap, c->stream));
+    HIP_TRY(c, mpd::launch_excl_scan(p.cap, smax + 1, bnext, c->scan_tmp, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace, p.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, bnext + nn, sizeof(int32_t), hipMemcpyDeviceToHost,
+    HIP_TRY(c, hipMemcpyAsync(c->pinned_i32 + kPinPlace + 2, bnext + smax, sizeof(int32_t), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipEventRecord(c->place_ev, c->stream));
-    HIP_TRY(c, mpd::launch_place_fold(c->keys[1], c->vals[1], bnow, p.cur, p.flags, nn, c->node_vol, spwt,
-                                      c->den_out, c->stream));
+    mpd::PlaceFoldArgs f;
+    f.tab = c->seg_tab;
+    f.nn = nn;
+    f.spid = c->keys[1];
+    f.sval = c->vals[1];
+    f.tmp_pid = c->keys[0];
+    f.tmp_val = c->vals[0];
+    f.base = bnow;
+    f.cur = p.cur;
+    f.flags = p.flags;
+    f.nvol = c->node_vol;
+    f.spwt = spwt;
+    f.out = c->den_out;
+    const uint64_t np = (uint64_t)std::max<int64_t>(c->next_pid, 1);
+    f.pid_bits = bit_width(np - 1 > 0 ? np - 1 : 1);
+    HIP_TRY(c, mpd::launch_place_fold(f, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->place_ev));
-    const bool no_fit = c->pinned_i32[kPinPlace] != 0, hot = c->pinned_i32[kPinPlace + 1] != 0;
+    const bool no_fit = c->pinned_i32[kPinPlace] != 0, over = c->pinned_i32[kPinPlace + 1] != 0;
+    c->place_pending = false;
+    if (no_fit || over) {
+        c->det_rebuilt++;
+        // the node counts from the slab counts (the capacities are scratch now), then the exact
+        // path, whose counts give the next layout
+        HIP_TRY(c, mpd::launch_place_to_raw(c->seg_tab, nn, smax, p.cur, p.cap, c->scan_tmp, c->den_raw, c->stream));
+        const int64_t m = 4 * c->n_slots;
+        int rc = det_finish_seg(c, m, spwt);
+        return rc ? rc : place_prepare(c, m);
+    }
     c->place_buf = 1 - c->place_buf;
     c->place_total = c->pinned_i32[kPinPlace + 2];
-    c->place_ready = !hot;
-    c->place_pending = false;
-    if (no_fit || hot) {
-        c->det_rebuilt++;
-        HIP_TRY(c, mpd::launch_place_to_raw(p.cur, c->den_raw, nn, c->stream));
-        return det_finish_seg(c, 4 * c->n_slots, spwt);
-    }
     c->det_placed++;
     return MP_OK;
 }
@@ -451,7 +490,7 @@ int sort_now(mp_ctx* c) {
             return set_err(c, MP_ERR_ALLOC, "cell sort count allocation failed");
         c->cell_cap = nb1;
     }
-    const unsigned long long mul = (1ull << 48) / (unsigned long long)std::max<int64_t>(c->next_pid, 1);
+    const unsigned long long mul = pid_digit_mul(c);
     HIP_TRY(c, mpd::launch_fill_i32(c->cell_cnt, nb1, 0, c->stream));
     const Soa& src = c->soa[c->cur];
     const Soa& dst = c->soa[1 - c->cur];
@@ -938,8 +977,11 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.det = place ? 3 : seg ? 2 : det ? 1 : 0;
     if (place) {
         const PlaceBufs p = place_bufs(c);
-        a.place = mpd::PlaceArgs{p.base[c->place_buf], p.cur, p.flags};
-        HIP_TRY(c, mpd::launch_fill_i32(p.cur, (int64_t)c->n_nodes + 8, 0, c->stream));
+        a.place = mpd::PlaceArgs{p.base[c->place_buf], p.cur, p.flags,
+                                 c->place_hot ? mpd::seg_slab_info(c->seg_tab, c->n_nodes) : nullptr,
+                                 pid_digit_mul(c)};
+        // flags and cursors (contiguous) start at zero
+        HIP_TRY(c, mpd::launch_fill_i32(p.flags, 8 + c->place_smax + 1, 0, c->stream));
     }
     a.agg = c->opt.scatter_agg;
     a.block = c->opt.block_size;
@@ -954,7 +996,7 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     } else if (seg) {
         c->det_rebuilt++;
         int rc = det_finish_seg(c, n_rec, spwt);
-        if (!rc && c->opt.det_algo == 2) rc = place_prepare(c);
+        if (!rc && c->opt.det_algo == 2) rc = place_prepare(c, n_rec);
         if (rc) return rc;
     } else if (det) {
         int rc = det_finish(c, n_rec, spwt);

@@ -63,12 +63,17 @@ struct mp_ctx {
     int32_t* seg_bounds = nullptr;
     int32_t* seg_tab = nullptr;  // node-segment scatter: counts, slab tables, node lists
     int64_t seg_tab_n = 0;
-    // records placed by the push (det_algo 2): cursors (nn), flags (8), capacities (nn + 1) and
-    // two segment layouts (nn + 1 each: the step's and the next step's)
+    // records placed by the push (det_algo 2), in the slab layout of the last exact step: flags
+    // (8), cursors, capacities, limits and two segment layouts (the step's and the next step's),
+    // place_smax + 1 ints each
     int32_t* place_mem = nullptr;
+    int64_t place_mem_n = 0;    // ints allocated
+    int64_t place_m = 0;        // records of the exact step the slab layout comes from
+    int64_t place_smax = 0;     // seg_slab_max(nn, place_m)
     int place_buf = 0;          // which layout the next placing push uses
     bool place_ready = false;   // that layout holds the last step's counts
     bool place_pending = false; // its hot count and total are still on their way (place_ev)
+    bool place_hot = false;     // the slab layout has hot nodes (records keyed by pid slab)
     int64_t place_total = 0;    // its size in records
     hipEvent_t place_ev = nullptr;
     int64_t det_placed = 0, det_rebuilt = 0;  // mp_get_det_paths

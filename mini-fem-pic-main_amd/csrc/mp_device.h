@@ -85,11 +85,14 @@ struct Soa {
     uint16_t* run;  // batch: run of each particle (nullptr for a single run)
 };
 
-// The node segments a det_algo 2 push places its records into (mp_common.h, place_records).
+// The slab segments a det_algo 2 push places its records into (mp_common.h, place_records).  A
+// node owns one slab, or 2^b slabs cut from its pid range when it is hot (sinfo).
 struct PlaceArgs {
-    const int32_t* base;  // nn + 1: segment starts (capacity of n: base[n + 1] - base[n])
-    int32_t* cur;         // nn: claim cursors, zero before the push
-    int32_t* flags;       // [0]: a record did not fit
+    const int32_t* base;      // slabs + 1: segment starts (capacity of s: base[s + 1] - base[s])
+    int32_t* cur;             // slabs: claim cursors, zero before the push
+    int32_t* flags;           // [0]: a record did not fit
+    const int2* sinfo;        // nn: (first slab, b) per node; nullptr: no hot node, slab n = node n
+    unsigned long long mul;   // pid digit multiplier, floor(2^48 / next pid)
 };
 
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };

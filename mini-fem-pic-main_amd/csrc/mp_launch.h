@@ -96,22 +96,47 @@ int64_t seg_table_ints(int32_t nn, int64_t m);
 // the longest prefix sum launch_seg_fold takes
 int64_t seg_scan_len(int32_t nn, int64_t m);
 hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st);
-// Records placed by the push (det_algo 2), mp_segfold.hip.  After the push (the cursors cur
-// are the counts): flags[1] = a node over 512 records, cap = the next step's segment capacities
-// (nn + 1 ints, the last 0).
-hipError_t launch_place_check(const int32_t* cur, int32_t* flags, int32_t* cap, int32_t nn, hipStream_t st);
-// The counts into raw as FP64, the exact path's input, for a placed step that is rebuilt.
-hipError_t launch_place_to_raw(const int32_t* cur, double* raw, int32_t nn, hipStream_t st);
-// The next step's segment starts (nn + 1) from the counts launch_seg_fold left in its table.
-hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int32_t* cap, int32_t* base, int32_t* scan_tmp,
-                               hipStream_t st);
-// Where launch_seg_fold leaves its count of hot nodes (one int).
+// Records placed by the push (det_algo 2), mp_segfold.hip.  The placement keeps the slab layout
+// of the last exact step (launch_seg_fold's table tab for nn nodes and m records): a cold node's
+// one slab, a hot node's 2^b pid slabs.  Its arrays have smax + 1 = seg_slab_max(nn, m) + 1
+// entries (lim, cap, base, cur; the entries past the layout's slabs stay 0).
+int64_t seg_slab_max(int32_t nn, int64_t m);
+// (first slab, b) per node, as the placing push keys its records by.
+const int2* seg_slab_info(const int32_t* tab, int32_t nn);
+// Where launch_seg_fold leaves its count of hot nodes (one int) and of slabs (one int).
 const int32_t* seg_hot_count(const int32_t* tab, int32_t nn);
-// One wave per node: the placed records in pid order, folded into out; nothing when flags[0] or
-// flags[1] is set.
-hipError_t launch_place_fold(const uint32_t* spid, const double* sval, const int32_t* base, const int32_t* cnt,
-                             const int32_t* flags, int32_t nn, const double* nvol, double spwt, double* out,
-                             hipStream_t st);
+const int32_t* seg_slab_count(const int32_t* tab, int32_t nn);
+// After an exact step: each slab's limit (lim: 256 for a hot node's slab, 0 = none for a cold
+// node's), capacity (cap) and segment start (base, smax + 1 entries, base[smax] the total).
+hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int64_t m, int32_t* lim, int32_t* cap, int32_t* base,
+                               int32_t* scan_tmp, hipStream_t st);
+// After a placing push (the cursors cur are the slab counts): flags[1] = a slab over its limit,
+// cap = the next step's capacities; the table's fallback count cleared for the placed fold.
+hipError_t launch_place_check(const int32_t* tab, int32_t nn, int64_t smax, const int32_t* cur, const int32_t* lim,
+                              int32_t* flags, int32_t* cap, hipStream_t st);
+// The node counts into raw as FP64, the exact path's input, for a placed step that is rebuilt
+// (scur: smax + 1 ints of scratch).
+hipError_t launch_place_to_raw(const int32_t* tab, int32_t nn, int64_t smax, const int32_t* cur, int32_t* scur,
+                               int32_t* scan_tmp, double* raw, hipStream_t st);
+// One wave per node: the placed records of each slab in pid order, folded into out; nothing
+// when flags[0] or flags[1] is set.  A cold node over 512 records is sorted in place by a
+// workgroup (tmp_pid / tmp_val: scratch at the same offsets) and folded by one wave.
+struct PlaceFoldArgs {
+    const int32_t* tab;  // launch_seg_fold's table of the layout's exact step
+    int32_t nn;
+    uint32_t* spid;      // the placed records
+    double* sval;
+    uint32_t* tmp_pid;   // as many entries as spid
+    double* tmp_val;
+    const int32_t* base;  // the step's segment starts
+    const int32_t* cur;   // the slab counts
+    const int32_t* flags;
+    const double* nvol;
+    double spwt;
+    double* out;
+    int pid_bits;         // bits of the largest pid
+};
+hipError_t launch_place_fold(const PlaceFoldArgs& a, hipStream_t st);
 // One wave's serial fold of each listed node's range [start[n], end[n]) of vals, normalised
 // into out (at most max_count nodes listed; *count read on the device).
 hipError_t launch_fold_list(const double* vals, const int32_t* start, const int32_t* end, const double* nvol,

@@ -40,7 +40,6 @@ constexpr int kBucketBits = 12;  // LDS hash slots: twice a tile's records, so e
 constexpr int kSegWaveMax = 8 * kWave;
 constexpr int kSlabShift = 7;  // hot slabs: (64, 128] records on average
 constexpr int kBigBits = 8;    // nodes of 2^8 slabs and more (over 16k records) are listed first
-constexpr int kFineBits = 16;  // bits of the pid digit the slabs are cut from
 
 // The table of launch_seg_fold, carved from SegFoldArgs::tab (ints).
 struct SegTab {
@@ -106,7 +105,7 @@ k_seg_counts(double* __restrict__ raw, int32_t nn, int32_t* __restrict__ cnt, in
         const int32_t c = (int32_t)raw[i];
         raw[i] = 0.0;
         cnt[i] = c;
-        if (c > kSegWaveMax) b = std::min(kFineBits, bit_width_u32((uint32_t)(c - 1) >> kSlabShift));
+        if (c > kSegWaveMax) b = std::min(kSlabFineBits, bit_width_u32((uint32_t)(c - 1) >> kSlabShift));
         bits[i] = b;
         nsl[i] = 1 << b;
     } else if (i == nn) {
@@ -137,13 +136,6 @@ k_slab_prep(const int32_t* __restrict__ cnt, const int32_t* __restrict__ bits, c
     const int32_t b = bits[i], s = sbase[i];
     sinfo[i] = make_int2(s, b);
     if (b == 0) slab_cnt[s] = cnt[i];
-}
-
-// The slab of a record of a node with (first slab, b) = si.
-__device__ __forceinline__ uint32_t slab_of(int2 si, uint32_t pid, unsigned long long mul) {
-    if (si.y == 0) return (uint32_t)si.x;
-    const uint32_t d = (uint32_t)(((unsigned long long)pid * mul) >> 32);
-    return (uint32_t)si.x + (d >> (kFineBits - si.y));
 }
 
 constexpr int kTileParts = 512;  // particles of a tile (2048 records)
@@ -594,16 +586,22 @@ __device__ __forceinline__ double slab_fold(const uint32_t (&key)[R], const doub
 // One wave per node: the big hot nodes first, then the other hot nodes, then the cold ones (or,
 // without a hot node, wave q takes node q).  The node index is a scalar, so every branch on a
 // slab's length is wave-uniform.  Slab j + 1 is loaded into registers while slab j is sorted
-// and folded.  A hot node with a slab over 256 records is listed for the fallback with its
-// record range instead.
+// and folded.  Slab j's records: [sstart[j], sstart[j + 1]) grouped by the tile pass, or, PLACED,
+// [sstart[j], sstart[j] + scnt[j]) placed by the push (nothing is done when flags report a record
+// that did not fit or a hot slab over its limit: the step is rebuilt).  A hot node with a slab
+// over 256 records (never PLACED: the check before it flags such a slab), or a placed cold node
+// over 512, is listed for the fallback with its record range instead.
+template <bool PLACED>
 __global__ void __launch_bounds__(256)
 k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval, const int32_t* __restrict__ sbase,
-               const int32_t* __restrict__ slab_start, const int32_t* __restrict__ order,
-               const int32_t* __restrict__ big, int32_t* __restrict__ ctrs, int32_t nn,
-               const double* __restrict__ nvol, double spwt, double* __restrict__ out,
-               int32_t* __restrict__ ovf_list, int32_t* __restrict__ nstart, int32_t* __restrict__ nend) {
+               const int32_t* __restrict__ sstart, const int32_t* __restrict__ scnt,
+               const int32_t* __restrict__ order, const int32_t* __restrict__ big, int32_t* __restrict__ ctrs,
+               int32_t nn, const double* __restrict__ nvol, double spwt, double* __restrict__ out,
+               int32_t* __restrict__ ovf_list, int32_t* __restrict__ nstart, int32_t* __restrict__ nend,
+               const int32_t* __restrict__ flags) {
     __shared__ double lval[256 / kWave][kSegWaveMax];
     __shared__ RankLds lrank[256 / kWave];
+    if (PLACED && (flags[0] | flags[1])) return;
     const int lane = lane_id();
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int32_t q = (int32_t)((int64_t)blockIdx.x * (256 / kWave) + w);
@@ -615,14 +613,31 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
         n = q < n_big ? big[q] : q < n_hot ? order[q - n_big] : order[q];
         n = __builtin_amdgcn_readfirstlane(n);
     }
+    // the record range of slab j
+    auto range = [&](int32_t j, int64_t& s, int& m) {
+        s = __builtin_amdgcn_readfirstlane(sstart[j]);
+        m = PLACED ? __builtin_amdgcn_readfirstlane(scnt[j])
+                   : __builtin_amdgcn_readfirstlane(sstart[j + 1]) - (int32_t)s;
+    };
     const int32_t b0 = __builtin_amdgcn_readfirstlane(sbase[n]);
     const int32_t ns = __builtin_amdgcn_readfirstlane(sbase[n + 1]) - b0;
     double* lv = lval[w];
     RankLds& rl = lrank[w];
     double acc = 0.0;
     if (ns == 1) {
-        const int64_t s = __builtin_amdgcn_readfirstlane(slab_start[b0]);
-        const int m = __builtin_amdgcn_readfirstlane(slab_start[b0 + 1]) - (int32_t)s;
+        int64_t s;
+        int m;
+        range(b0, s, m);
+        if (PLACED && m > kSegWaveMax) {
+            // a cold node's placed slab over one wave's 512: its range is contiguous, so the
+            // workgroup fallback sorts it
+            if (lane == 0) {
+                ovf_list[atomicAdd(ctrs + 2, 1)] = n;
+                nstart[n] = (int32_t)s;
+                nend[n] = (int32_t)s + m;
+            }
+            return;
+        }
         uint32_t key[8];
         double val[8];
         slab_load(spid, sval, s, m, lane, key, val);
@@ -630,21 +645,25 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
     } else if (ns > 1) {
         // a hot node's slabs hold 64..128 records on average; one over 256 (pids bunched within
         // the node's range) sends the node to the fallback
-        bool over = false;
-        for (int32_t j = lane; j < ns; j += kWave) over |= slab_start[b0 + j + 1] - slab_start[b0 + j] > 4 * kWave;
-        if (__ballot(over)) {
-            if (lane == 0) {
-                ovf_list[atomicAdd(ctrs + 2, 1)] = n;
-                nstart[n] = slab_start[b0];
-                nend[n] = slab_start[b0 + ns];
+        if (!PLACED) {
+            bool over = false;
+            for (int32_t j = lane; j < ns; j += kWave)
+                over |= sstart[b0 + j + 1] - sstart[b0 + j] > 4 * kWave;
+            if (__ballot(over)) {
+                if (lane == 0) {
+                    ovf_list[atomicAdd(ctrs + 2, 1)] = n;
+                    nstart[n] = sstart[b0];
+                    nend[n] = sstart[b0 + ns];
+                }
+                return;
             }
-            return;
         }
         // slab j + 1 is loaded into the prefetch registers while slab j is folded
         uint32_t pk[4];
         double pv[4];
-        int64_t ps = __builtin_amdgcn_readfirstlane(slab_start[b0]);
-        int pm = __builtin_amdgcn_readfirstlane(slab_start[b0 + 1]) - (int32_t)ps;
+        int64_t ps;
+        int pm;
+        range(b0, ps, pm);
         slab_load(spid, sval, ps, pm, lane, pk, pv);
         for (int32_t j = 0; j < ns; j++) {
             uint32_t ck[4];
@@ -657,8 +676,7 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
             const int64_t s = ps;
             const int m = pm;
             if (j + 1 < ns) {
-                ps = __builtin_amdgcn_readfirstlane(slab_start[b0 + j + 1]);
-                pm = __builtin_amdgcn_readfirstlane(slab_start[b0 + j + 2]) - (int32_t)ps;
+                range(b0 + j + 1, ps, pm);
                 slab_load(spid, sval, ps, pm, lane, pk, pv);
             }
             acc = slab_fold(ck, cv, m, lane, rl, lv, acc, sval, s);
@@ -756,60 +774,64 @@ k_seg_hot_sort(uint32_t* __restrict__ pa, double* __restrict__ va, uint32_t* __r
 
 // ---- records placed by the push (det_algo 2) -------------------------------------------------
 
-// A node's segment for the next step: its count now, a quarter more and 32 (the counts of a node
-// change by a few per cent from one step to the next).
-__device__ __forceinline__ int32_t place_capacity(int32_t c) { return c + (c >> 2) + 32; }
+// A slab's segment for the next step: its count now and some slack (the counts change by a few
+// per cent from one step to the next).  A cold node's one slab (lim 0) gets a quarter more and
+// 32; a hot node's slab (lim 256, 64..128 records when cut) an eighth more and 16, which keeps
+// the placed arrays near 1.25 x the records where most records fall on hot nodes.
+__device__ __forceinline__ int32_t place_capacity(int32_t c, int32_t lim) {
+    return lim == 0 ? c + (c >> 2) + 32 : c + (c >> 3) + 16;
+}
 
-// After a placing push: the cursors are the counts; a node over 512 records raises flags[1],
-// and the next step's capacities are written (cap[nn] = 0 for the scan's total).
+// The most records a slab may hold for the placed fold: a hot node's slabs 256 (the prefetching
+// slab loop); a cold node's one slab has no limit (lim 0: over 512 records its node is sorted
+// by the workgroup fallback, its range being contiguous).  One thread per node.
 __global__ void __launch_bounds__(256)
-k_place_check(const int32_t* __restrict__ cur, int32_t* __restrict__ flags, int32_t* __restrict__ cap, int32_t nn) {
+k_place_lim(const int32_t* __restrict__ sbase, const int32_t* __restrict__ bits, int32_t nn,
+            int32_t* __restrict__ lim) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nn) {
-        const int32_t c = cur[i];
-        if (c > kSegWaveMax) flags[1] = 1;
-        cap[i] = place_capacity(c);
-    } else if (i == nn) {
-        cap[nn] = 0;
+    if (i >= nn) return;
+    const int32_t s0 = sbase[i], b = bits[i];
+    if (b == 0) {
+        lim[s0] = 0;
+        return;
+    }
+    for (int32_t j = 0; j < (1 << b); j++) lim[s0 + j] = 4 * kWave;
+}
+
+// Capacities (smax + 1 entries, 0 past the *ns slabs) from the exact path's slab counts.
+__global__ void __launch_bounds__(256)
+k_place_cap(const int32_t* __restrict__ cnt, const int32_t* __restrict__ lim, const int32_t* __restrict__ ns,
+            int64_t smax, int32_t* __restrict__ cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > smax) return;
+    cap[i] = i < *ns ? place_capacity(cnt[i], lim[i]) : 0;
+}
+
+// After a placing push: the cursors are the slab counts; a hot node's slab over its limit raises
+// flags[1], and the next step's capacities are written (0 past the *ns slabs, for the scan's
+// total).  The fallback list's count (ctrs[2]) is cleared for the placed fold.
+__global__ void __launch_bounds__(256)
+k_place_check(const int32_t* __restrict__ cur, const int32_t* __restrict__ lim, const int32_t* __restrict__ ns,
+              int64_t smax, int32_t* __restrict__ flags, int32_t* __restrict__ cap, int32_t* __restrict__ ctrs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > smax) return;
+    if (i == 0) ctrs[2] = 0;
+    if (i < *ns) {
+        const int32_t c = cur[i], l = lim[i];
+        if (l && c > l) flags[1] = 1;
+        cap[i] = place_capacity(c, l);
+    } else {
+        cap[i] = 0;
     }
 }
 
-// The counts as the exact path's input (FP64 unit sums), when a placed step is rebuilt.
+// The node counts as the exact path's input (FP64 unit sums), when a placed step is rebuilt:
+// node n's slabs [sbase[n], sbase[n + 1]) summed through the exclusive scan of the slab counts.
 __global__ void __launch_bounds__(256)
-k_place_to_raw(const int32_t* __restrict__ cur, double* __restrict__ raw, int32_t nn) {
+k_place_to_raw(const int32_t* __restrict__ scur, const int32_t* __restrict__ sbase, int32_t nn,
+               double* __restrict__ raw) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nn) raw[i] = (double)cur[i];
-}
-
-// The capacities from the exact path's counts.
-__global__ void __launch_bounds__(256)
-k_place_cap(const int32_t* __restrict__ cnt, int32_t* __restrict__ cap, int32_t nn) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nn) cap[i] = place_capacity(cnt[i]);
-    else if (i == nn) cap[nn] = 0;
-}
-
-// One wave per node: its placed records [base[n], base[n] + cnt[n]) (at most 512) in ascending
-// pid and folded, the normalised density written.  Nothing is done when the step's records did
-// not all fit, or a node exceeds 512 (flags): the exact path rebuilds the step.
-__global__ void __launch_bounds__(256)
-k_place_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval,
-                 const int32_t* __restrict__ base, const int32_t* __restrict__ cnt, const int32_t* __restrict__ flags,
-                 int32_t nn, const double* __restrict__ nvol, double spwt, double* __restrict__ out) {
-    __shared__ double lval[256 / kWave][kSegWaveMax];
-    __shared__ RankLds lrank[256 / kWave];
-    if (flags[0] | flags[1]) return;
-    const int lane = lane_id();
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-    const int64_t n = (int64_t)blockIdx.x * (256 / kWave) + w;
-    if (n >= nn) return;
-    const int64_t s = __builtin_amdgcn_readfirstlane(base[n]);
-    const int m = __builtin_amdgcn_readfirstlane(cnt[n]);
-    uint32_t key[8];
-    double val[8];
-    slab_load(spid, sval, s, m, lane, key, val);
-    const double acc = slab_fold(key, val, m, lane, lrank[w], lval[w], 0.0, sval, s);
-    if (lane == 0) out[n] = acc * (spwt / nvol[n]);
+    if (i < nn) raw[i] = (double)(scur[sbase[i + 1]] - scur[sbase[i]]);
 }
 
 unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -848,9 +870,9 @@ hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st) {
     if (a.n > 0)
         hipLaunchKernelGGL(k_seg_tile<false>, dim3(gt), dim3(256), 0, st, a.tets, a.soa, a.n, t.sinfo, a.mul, t.ctrs,
                            t.slab_cnt, t.slab_start, t.cur, a.seg_pid, a.seg_val);
-    hipLaunchKernelGGL(k_seg_sortfold, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid, a.seg_val,
-                       t.sbase, t.slab_start, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list,
-                       t.nstart, t.nend);
+    hipLaunchKernelGGL(k_seg_sortfold<false>, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid,
+                       a.seg_val, t.sbase, t.slab_start, (const int32_t*)nullptr, t.order, t.big, t.ctrs, nn, a.nvol,
+                       a.spwt, a.out, t.ovf_list, t.nstart, t.nend, (const int32_t*)nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the fallback (none listed: its workgroups return at once): sort each listed node's range
@@ -864,33 +886,63 @@ hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st) {
     return launch_fold_list(a.seg_val, t.nstart, t.nend, a.nvol, a.spwt, a.out, t.ovf_list, t.ctrs + 2, nn, st);
 }
 
-hipError_t launch_place_check(const int32_t* cur, int32_t* flags, int32_t* cap, int32_t nn, hipStream_t st) {
-    hipLaunchKernelGGL(k_place_check, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, cur, flags, cap, nn);
-    return hipGetLastError();
-}
+int64_t seg_slab_max(int32_t nn, int64_t m) { return slab_max(nn, m); }
 
-hipError_t launch_place_to_raw(const int32_t* cur, double* raw, int32_t nn, hipStream_t st) {
-    hipLaunchKernelGGL(k_place_to_raw, dim3(grid_for(nn, 256)), dim3(256), 0, st, cur, raw, nn);
-    return hipGetLastError();
-}
-
-hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int32_t* cap, int32_t* base, int32_t* scan_tmp,
-                               hipStream_t st) {
-    // the counts are the first nn ints of the table (k_seg_counts)
-    hipLaunchKernelGGL(k_place_cap, dim3(grid_for((int64_t)nn + 1, 256)), dim3(256), 0, st, tab, cap, nn);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = launch_excl_scan(cap, (int64_t)nn + 1, base, scan_tmp, st);
-    return e;
+const int2* seg_slab_info(const int32_t* tab, int32_t nn) {
+    return carve(const_cast<int32_t*>(tab), nn, 0).sinfo;
 }
 
 const int32_t* seg_hot_count(const int32_t* tab, int32_t nn) { return carve(const_cast<int32_t*>(tab), nn, 0).ctrs; }
 
-hipError_t launch_place_fold(const uint32_t* spid, const double* sval, const int32_t* base, const int32_t* cnt,
-                             const int32_t* flags, int32_t nn, const double* nvol, double spwt, double* out,
-                             hipStream_t st) {
-    hipLaunchKernelGGL(k_place_sortfold, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, spid, sval, base, cnt,
-                       flags, nn, nvol, spwt, out);
+const int32_t* seg_slab_count(const int32_t* tab, int32_t nn) {
+    return carve(const_cast<int32_t*>(tab), nn, 0).sbase + nn;
+}
+
+hipError_t launch_place_layout(const int32_t* tab, int32_t nn, int64_t m, int32_t* lim, int32_t* cap, int32_t* base,
+                               int32_t* scan_tmp, hipStream_t st) {
+    const SegTab t = carve(const_cast<int32_t*>(tab), nn, m);
+    const int64_t smax = slab_max(nn, m);
+    hipLaunchKernelGGL(k_place_lim, dim3(grid_for(nn, 256)), dim3(256), 0, st, t.sbase, t.bits, nn, lim);
+    hipLaunchKernelGGL(k_place_cap, dim3(grid_for(smax + 1, 256)), dim3(256), 0, st, t.slab_cnt, lim,
+                       t.sbase + nn, smax, cap);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_excl_scan(cap, smax + 1, base, scan_tmp, st);
+    return e;
+}
+
+hipError_t launch_place_check(const int32_t* tab, int32_t nn, int64_t smax, const int32_t* cur, const int32_t* lim,
+                              int32_t* flags, int32_t* cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_place_check, dim3(grid_for(smax + 1, 256)), dim3(256), 0, st, cur, lim,
+                       seg_slab_count(tab, nn), smax, flags, cap,
+                       carve(const_cast<int32_t*>(tab), nn, 0).ctrs);
     return hipGetLastError();
+}
+
+hipError_t launch_place_to_raw(const int32_t* tab, int32_t nn, int64_t smax, const int32_t* cur, int32_t* scur,
+                               int32_t* scan_tmp, double* raw, hipStream_t st) {
+    hipError_t e = launch_excl_scan(cur, smax + 1, scur, scan_tmp, st);
+    if (e != hipSuccess) return e;
+    const SegTab t = carve(const_cast<int32_t*>(tab), nn, 0);
+    hipLaunchKernelGGL(k_place_to_raw, dim3(grid_for(nn, 256)), dim3(256), 0, st, scur, t.sbase, nn, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_fold(const PlaceFoldArgs& a, hipStream_t st) {
+    const int32_t nn = a.nn;
+    const SegTab t = carve(const_cast<int32_t*>(a.tab), nn, 0);
+    hipLaunchKernelGGL(k_seg_sortfold<true>, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.spid, a.sval,
+                       t.sbase, a.base, a.cur, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list, t.nstart,
+                       t.nend, a.flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the cold nodes listed over 512 records (usually none: the workgroups return at once)
+    const int passes = std::max(2, ((a.pid_bits + 7) / 8 + 1) / 2 * 2);
+    const unsigned gh = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nn));
+    hipLaunchKernelGGL(k_seg_hot_sort, dim3(gh), dim3(256), 0, st, a.spid, a.sval, a.tmp_pid, a.tmp_val, t.nstart,
+                       t.nend, t.ovf_list, t.ctrs + 2, passes);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fold_list(a.sval, t.nstart, t.nend, a.nvol, a.spwt, a.out, t.ovf_list, t.ctrs + 2, nn, st);
 }
 
 }  // namespace mpd

@@ -1,7 +1,8 @@
 """GPU parity of the deterministic scatters against the serial oracle.
 
-det_algo 2 (the default) has the push place the records into node segments sized from the
-last step's counts (a step that does not fit is rebuilt as with 1).  det_algo 1 rebuilds each
+det_algo 2 (the default) has the push place the records into slab segments sized from the
+last step's counts: a cold node's one slab, or a hot node's pid slabs (a step that does not fit,
+or with a slab over its limit, is rebuilt as with 1).  det_algo 1 rebuilds each
 particle's records from the pushed state and groups them into node segments (dense nodes cut
 into pid slabs).  Either way each segment is put in pid order on chip: a wave ranks records
 within sub-buckets of the pid range (a register bitonic network where the pids bunch), and a
@@ -120,8 +121,8 @@ def test_segment_lengths_at_every_boundary(mp, orc, fillers, sort_interval, algo
 def test_segments_with_hot_nodes_and_empty_steps(mp, orc, algo):
     """A dense cluster (dozens of nodes over 512 records, the hottest over 6 x 10^4) beside a
     smaller one, then every particle absorbed: the step after that has no records at all and
-    must leave a zero density.  With det_algo 2 the hot nodes keep the steps on the exact path
-    (the placed push is tried only after a step without them)."""
+    must leave a zero density.  With det_algo 2 the first step is exact and the next ones place
+    the hot nodes' records by pid slab."""
     side = 0.1
     mesh = mp.Mesh.box_mesh(10, side)
     ef = mesh.field_sinusoid(10.0)
@@ -181,3 +182,46 @@ def test_placed_records_match_oracle(mp, orc, wall):
         per_node = np.bincount(mesh.tets["con"][run.cell].ravel(), minlength=mesh.n_nodes)
     assert per_node.max() <= 512
     assert placed >= 6 and rebuilt >= 2 and placed + rebuilt == 9, (placed, rebuilt)
+
+
+@pytest.mark.parametrize("pid_bits", [0, 4], ids=["bucket-key", "pid-digit-key"])
+@pytest.mark.parametrize("wall", [0, 1], ids=["absorb", "reflect"])
+def test_placed_hot_slabs_match_oracle(mp, orc, wall, pid_bits):
+    """det_algo 2 where most records fall on hot nodes (a cluster with ~170 nodes over 512
+    records): after the first (exact) step the push places every record into its slab, a hot
+    node's slab picked by the pid digit; the cell sorts (with the pid digit in the key or
+    without) change the slots but not the slabs; cold nodes that grow past 512 records are
+    sorted by the workgroup fallback in their placed range (steps 4, 5, 8 and 9 here); the
+    injection of a fifth more pids changes the digit's multiplier and overflows segments once
+    (that step is rebuilt).  Every step bit-exact against the serial fold."""
+    side = 0.1
+    mesh = mp.Mesh.box_mesh(10, side)
+    ef = mesh.field_sinusoid(10.0)
+    centres = np.array([[0.05, 0.05, 0.05], [0.03, 0.07, 0.04]])
+    pos, vel = mp.synth_mixture(17 + wall, 120000, [0, 0, 0], [side] * 3, [0.6, 0.4], centres, 0.012, 1500.0,
+                                [0, 0, 0])
+    cell = mesh.locate_box(pos)
+    half = 100000
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos[:half], vel[:half], cell[:half], wall_mode=wall)
+    hot_steps = 0
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=3, wall_mode=wall, det_algo=2, sort_pid_bits=pid_bits)
+        mv.upload_field(ef)
+        mv.set_particles(pos[:half], vel[:half], cell[:half])
+        for k in range(10):
+            if k == 6:
+                mv.add_particles(pos[half:], vel[half:], cell[half:])
+                run.pos = np.ascontiguousarray(np.concatenate([run.pos, pos[half:]]))
+                run.vel = np.ascontiguousarray(np.concatenate([run.vel, vel[half:]]))
+                run.cell = np.ascontiguousarray(np.concatenate([run.cell, cell[half:]]).astype(np.int32))
+                run.pid = np.concatenate([run.pid, np.arange(half, len(pos))])
+            mv.step(QM, 1e-7, SPWT, DET)
+            ref = run.step(QM, 1e-7, SPWT)
+            assert mv.density().tobytes() == ref.tobytes(), f"step {k}: density not bit-exact"
+            per_node = np.bincount(mesh.tets["con"][run.cell].ravel(), minlength=mesh.n_nodes)
+            hot_steps += int((per_node > 512).sum() > 100)
+        check_state(mv, run, 10)
+        placed, rebuilt = mv.det_paths()
+    assert hot_steps == 10
+    # rebuilt: the first step (no layout yet) and the injection's
+    assert placed >= 7 and rebuilt >= 2 and placed + rebuilt == 10, (placed, rebuilt)
