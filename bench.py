@@ -226,8 +226,9 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
     (sigma 3000 m/s, no drift), frozen sinusoidal field, reflecting walls, cell sort every 8
     steps.  The same initial state is run in deterministic mode (the configuration's mode, its
     cell sort keyed by bucket and the top pid_bits of a pid digit, so that the dense nodes' pid
-    slabs stay few per tile) and in atomic mode; the difference of the two step times is the
-    cost of the sorted reduce.  A third run keeps the particles in pid order (compaction
+    slabs stay few per tile; det_algo 2: after an exact first step the push places each record
+    into its pid slab's segment, sized from the last step's counts) and in atomic mode; the
+    difference of the two step times is the cost of the sorted reduce.  A third run keeps the particles in pid order (compaction
     instead of the cell sort) with the device radix sort of pid-ordered records (det_algo 0)."""
     t = time.perf_counter()
     mesh = mp.Mesh.box_mesh(100, 1.0)
@@ -270,6 +271,9 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
                          "push_ms_per_launch": avg_s * 1e3, "push_gbs": bpl / avg_s / 1e9,
                          "crossing_fraction": c["crossed"] / max(c["particle_steps"], 1),
                          "reflections_per_step": c["reflections"] / steps}
+            if mode == mp.SCATTER_DETERMINISTIC:
+                placed, rebuilt = mv.det_paths()
+                res[name]["steps_placed"], res[name]["steps_rebuilt"] = placed, rebuilt
             log(f"contention {name}: {res[name]['value'] / 1e9:.3f} G particle-steps/s, "
                 f"{res[name]['ms_per_step']:.2f} ms/step, push {avg_s * 1e3:.3f} ms ({res[name]['push_gbs']:.0f} GB/s)")
     del pos, vel, cell

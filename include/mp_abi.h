@@ -85,17 +85,19 @@ typedef struct mp_options {
                                       fewer distinct nodes per workgroup where particles are
                                       sparse, a few per tet) */
     int32_t det_algo;              /* deterministic scatter: 2 (default) the push places each
-                                      particle's records into node segments sized from the last
-                                      step's counts, and each segment is put in pid order on
-                                      chip and folded (a step whose records do not fit, or with
-                                      a node over 512 records, is done as with 1); 1 the push
+                                      particle's records into slab segments sized from the last
+                                      step's counts (a node's one slab, or a dense node's pid
+                                      slabs as 1 cuts them), and each slab is put in pid order
+                                      on chip and folded (a step whose records do not fit, or
+                                      with a dense node's slab over 256 records, is done as
+                                      with 1); 1 the push
                                       counts each node's records, which are rebuilt from the
                                       pushed state and grouped into node segments (dense nodes
                                       cut into pid slabs); 0 the records are written in pid
                                       order and a device radix sort on the node bits groups
                                       them */
     int32_t sort_pid_bits;         /* cell sort (sort_now) key: 0 (default) the bucket; b > 0
-                                      (up to 8) the bucket refined by the top b bits of a pid
+                                      (up to 12) the bucket refined by the top b bits of a pid
                                       digit, so the particles of a dense bucket lie in pid
                                       ranges: the deterministic scatter's pid slabs of a dense
                                       node then take few keys per tile */
@@ -186,9 +188,9 @@ int  mp_get_density(mp_ctx* ctx, double* den);
 int  mp_get_counters(mp_ctx* ctx, mp_counters* out);
 int  mp_reset_counters(mp_ctx* ctx);
 /* Deterministic steps since mp_set_particles / mp_reset_counters by how their records were
- * grouped: *placed by the push into node segments sized from the last step (det_algo 2),
+ * grouped: *placed by the push into slab segments sized from the last step (det_algo 2),
  * *rebuilt from the pushed state (det_algo 1, and det_algo 2 steps whose records did not fit
- * or that had a node over 512 records).  Either pointer may be NULL. */
+ * or that had a dense node's slab over 256 records).  Either pointer may be NULL. */
 int  mp_get_det_paths(mp_ctx* ctx, int64_t* placed, int64_t* rebuilt);
 /* Sort the resident particles by cell (or, with keep_pid_order, compact them in pid order)
  * and drop the lost ones now. */

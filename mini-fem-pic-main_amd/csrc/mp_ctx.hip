@@ -556,7 +556,7 @@ bool good_options(const mp_options& o) {
            o.sort_shift <= 16 &&
            (o.keep_pid_order == 0 || o.keep_pid_order == 1) &&
            (o.gather_sort == 0 || o.gather_sort == 1) && (o.sort_order == 0 || o.sort_order == 1) &&
-           (o.det_algo >= 0 && o.det_algo <= 2) && o.sort_pid_bits >= 0 && o.sort_pid_bits <= 8;
+           (o.det_algo >= 0 && o.det_algo <= 2) && o.sort_pid_bits >= 0 && o.sort_pid_bits <= 12;
 }
 
 }  // namespace

@@ -627,7 +627,7 @@ class Mover:
         self._check(lib().mp_reset_counters(self._h))
 
     def det_paths(self):
-        """(placed, rebuilt): deterministic steps whose records the push placed into node segments
+        """(placed, rebuilt): deterministic steps whose records the push placed into slab segments
         (det_algo 2), and steps whose records were rebuilt from the pushed state."""
         a, b = C.c_int64(), C.c_int64()
         self._check(lib().mp_get_det_paths(self._h, C.byref(a), C.byref(b)))
