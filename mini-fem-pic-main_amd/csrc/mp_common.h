@@ -279,6 +279,7 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
         hc[s] = 0;
     }
     int32_t nd[4] = {-1, -1, -1, -1};
+    int2 seg[4];  // each record's segment [base[s], base[s + 1])
     if (ok) {
         const TetView t = tet_at(tets, fc);
         nd[0] = t.con(0);
@@ -295,9 +296,28 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
 #pragma unroll
     for (int v = 0; v < 4; v++) place_take(hk, hc, nd[v], slot[v], rank[v]);
     __syncthreads();
-    for (int s = threadIdx.x; s < kPlaceSlots; s += blockDim.x) {
-        const int32_t key = hk[s];
-        if (key >= 0) hc[s] = atomicAdd(pl.cur + key, hc[s]);
+    // the segment bounds (they do not depend on the claims) and the block's claims are all in
+    // flight together: one returning atomic per occupied slot, a thread's up to kPlaceClaim
+    // claims issued before any is waited for (a barrier waits on the loads issued before it)
+    if (ok) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) seg[v] = make_int2(pl.base[nd[v]], pl.base[nd[v] + 1]);
+    }
+    constexpr int kPlaceClaim = 4;
+    for (int s0 = threadIdx.x; s0 < kPlaceSlots; s0 += kPlaceClaim * blockDim.x) {
+        int32_t key[kPlaceClaim], got[kPlaceClaim];
+#pragma unroll
+        for (int u = 0; u < kPlaceClaim; u++) {
+            const int s = s0 + u * blockDim.x;
+            key[u] = s < kPlaceSlots ? hk[s] : -1;
+            got[u] = key[u] >= 0 ? hc[s] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPlaceClaim; u++)
+            if (key[u] >= 0) got[u] = atomicAdd(pl.cur + key[u], got[u]);
+#pragma unroll
+        for (int u = 0; u < kPlaceClaim; u++)
+            if (key[u] >= 0) hc[s0 + u * blockDim.x] = got[u];
     }
     __syncthreads();
     if (!ok) return;
@@ -305,8 +325,8 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const int32_t g = slot[v] >= 0 ? hc[slot[v]] + rank[v] : atomicAdd(pl.cur + nd[v], 1);
-        const int32_t b = pl.base[nd[v]];
-        if (g < pl.base[nd[v] + 1] - b) {
+        const int32_t b = seg[v].x;
+        if (g < seg[v].y - b) {
             out_pid[b + g] = pid;
             out_val[b + g] = w[v];
         } else {
