@@ -65,7 +65,9 @@ typedef struct mp_particle {
 typedef struct mp_options {
     int32_t sort_interval;         /* cell-sort + compact every k steps (0 = never)    */
     int32_t wall_mode;             /* MP_WALL_ABSORB / MP_WALL_REFLECT                 */
-    int32_t block_size;            /* threads per workgroup of the push kernel (64, 128, 256) */
+    int32_t block_size;            /* threads per workgroup of the push kernel (64, 128, 256;
+                                      the placing push of det_algo 2 runs its own: 512, or 256
+                                      with dense nodes cut into pid slabs) */
     int32_t particles_per_thread;  /* 1, 2, 4 or 8: rounds of block_size particles per thread */
     int32_t walk_cap;              /* max face moves per particle per step             */
     int32_t scatter_agg;           /* atomic-mode aggregation: 0 one atomic per weight,

@@ -220,28 +220,31 @@ __device__ __forceinline__ uint32_t slab_of(int2 si, uint32_t pid, unsigned long
 // slot within the probe limit claims its place alone.  A record past its segment's capacity is
 // not written and raises flags[0]: the caller then rebuilds the step's records exactly.  The
 // cursors end as the slabs' record counts.
-#ifndef MP_PLACE_BITS
-#define MP_PLACE_BITS 10
-#endif
-constexpr int kPlaceBits = MP_PLACE_BITS;
-constexpr int kPlaceSlots = 1 << kPlaceBits;
+// The placing push's workgroups: 256 threads (a 2^10-slot hash) or 512 (2^11): 4 slots per
+// thread.  512 doubles the records per slab claim, which pays where every node is one slab (9%
+// on the placing push at 20M uniform particles) and costs where the records of a workgroup
+// spread over many pid slabs (3% at 200M clustered ones).
+constexpr int place_bits(int threads) { return threads > 256 ? 11 : 10; }
 constexpr int kPlaceProbe = 32;
 
+template <int BITS>
 __device__ __forceinline__ int place_slot(int32_t* hk, int32_t key) {
-    uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - kPlaceBits);
+    uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - BITS);
+    constexpr uint32_t mask = (1u << BITS) - 1u;
 #if MP_PROBE_ROLLED
 #pragma unroll 1
 #endif
     for (int p = 0; p < kPlaceProbe; p++) {
         const int32_t prev = atomicCAS(&hk[h], -1, key);
         if (prev == -1 || prev == key) return (int)h;
-        h = (h + 1) & (kPlaceSlots - 1);
+        h = (h + 1) & mask;
     }
     return -1;
 }
 
 // Slot (-1: none) and rank of this lane's slab key (-1: no record) in the block's table; every
 // lane of the wave calls it.
+template <int BITS>
 __device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t key, int& slot, int& rank) {
     slot = -1;
     rank = 0;
@@ -253,7 +256,7 @@ __device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t key
     const unsigned long long same = __ballot(key == k0);
     int s0 = -1, r0 = 0;
     if (lane == lead) {
-        s0 = place_slot(hk, k0);
+        s0 = place_slot<BITS>(hk, k0);
         if (s0 >= 0) r0 = atomicAdd(&hc[s0], __popcll(same));
     }
     s0 = __builtin_amdgcn_readlane(s0, lead);
@@ -264,17 +267,25 @@ __device__ __forceinline__ void place_take(int32_t* hk, int32_t* hc, int32_t key
             rank = r0 + __popcll(same & ((1ull << lane) - 1ull));
         }
     } else if (key >= 0) {
-        slot = place_slot(hk, key);
+        slot = place_slot<BITS>(hk, key);
         if (slot >= 0) rank = atomicAdd(&hc[slot], 1);
     }
 }
 
+// THREADS: the workgroup size.  `again`: not the block's first call (several rounds per
+// thread): its records' reads of the claimed offsets (any slot) finish before the table is
+// cleared.
+template <int THREADS>
 __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets, bool ok, int32_t fc, uint32_t pid,
                                               double l0, double l1, double l2, double l3, const PlaceArgs& pl,
-                                              uint32_t* __restrict__ out_pid, double* __restrict__ out_val) {
-    __shared__ int32_t hk[kPlaceSlots];
-    __shared__ int32_t hc[kPlaceSlots];
-    for (int s = threadIdx.x; s < kPlaceSlots; s += blockDim.x) {
+                                              uint32_t* __restrict__ out_pid, double* __restrict__ out_val,
+                                              bool again) {
+    constexpr int kBits = place_bits(THREADS);
+    constexpr int slots = 1 << kBits;
+    __shared__ int32_t hk[slots];
+    __shared__ int32_t hc[slots];
+    if (again) __syncthreads();
+    for (int s = threadIdx.x; s < slots; s += blockDim.x) {
         hk[s] = -1;
         hc[s] = 0;
     }
@@ -294,7 +305,7 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
     __syncthreads();
     int slot[4], rank[4];
 #pragma unroll
-    for (int v = 0; v < 4; v++) place_take(hk, hc, nd[v], slot[v], rank[v]);
+    for (int v = 0; v < 4; v++) place_take<kBits>(hk, hc, nd[v], slot[v], rank[v]);
     __syncthreads();
     // the segment bounds (they do not depend on the claims) and the block's claims are all in
     // flight together: one returning atomic per occupied slot, a thread's up to kPlaceClaim
@@ -304,12 +315,12 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
         for (int v = 0; v < 4; v++) seg[v] = make_int2(pl.base[nd[v]], pl.base[nd[v] + 1]);
     }
     constexpr int kPlaceClaim = 4;
-    for (int s0 = threadIdx.x; s0 < kPlaceSlots; s0 += kPlaceClaim * blockDim.x) {
+    for (int s0 = threadIdx.x; s0 < slots; s0 += kPlaceClaim * blockDim.x) {
         int32_t key[kPlaceClaim], got[kPlaceClaim];
 #pragma unroll
         for (int u = 0; u < kPlaceClaim; u++) {
             const int s = s0 + u * blockDim.x;
-            key[u] = s < kPlaceSlots ? hk[s] : -1;
+            key[u] = s < slots ? hk[s] : -1;
             got[u] = key[u] >= 0 ? hc[s] : 0;
         }
 #pragma unroll

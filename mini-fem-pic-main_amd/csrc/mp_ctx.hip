@@ -974,7 +974,8 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;
     a.wall = c->opt.wall_mode;
-    a.det = place ? 3 : seg ? 2 : det ? 1 : 0;
+    // placing: 512-thread workgroups where every node is one slab, 256 with pid slabs
+    a.det = place ? (c->place_hot ? 3 : 4) : seg ? 2 : det ? 1 : 0;
     if (place) {
         const PlaceBufs p = place_bufs(c);
         a.place = mpd::PlaceArgs{p.base[c->place_buf], p.cur, p.flags,

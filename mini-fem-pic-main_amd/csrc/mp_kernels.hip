@@ -46,9 +46,10 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
 // Push + XtoLtet walk + scatter of the resident particles, one step.  AGG: 0 = one atomic per
 // weight, 2 = LDS hash per workgroup.  Each thread takes ppt particles, one per round, a round
 // being blockDim.x consecutive particles; MULTI == false fixes ppt = 1 at compile time (the
-// round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).
+// round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).  The placing push
+// runs fixed workgroups whatever the block size option: DET 3 256 threads, DET 4 512.
 template <int WALL, int DET, int AGG, bool MULTI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
+__global__ void __launch_bounds__(DET == 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
            double* __restrict__ den, uint32_t* __restrict__ keys, double* __restrict__ vals,
            int32_t* __restrict__ part, int cap, int ppt_arg, bool rec_by_slot, PlaceArgs pl) {
@@ -67,7 +68,7 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
             // cell < 0, evaluates tet 0 with a cap of 0 and its results are dropped), so the
             // compiler cannot sink the state or start-tet loads behind a wait.
             const int32_t c_in = s.cell[i];
-            if (DET == 3) pid = (uint32_t)s.pid[i];
+            if (DET >= 3) pid = (uint32_t)s.pid[i];
             double vx = ld(s.vx + i), vy = ld(s.vy + i), vz = ld(s.vz + i);
             const double x0 = ld(s.x + i), y0 = ld(s.y + i), z0 = ld(s.z + i);
             // keep the six state loads issued ahead of the cell's first use (the scheduler
@@ -106,8 +107,8 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
                 st_(s.vx + i, vx); st_(s.vy + i, vy); st_(s.vz + i, vz);
             }
         }
-        if constexpr (DET == 3)
-            place_records(tets, ok, fc, pid, l0, l1, l2, l3, pl, keys, vals);
+        if constexpr (DET >= 3)
+            place_records<DET == 4 ? 512 : 256>(tets, ok, fc, pid, l0, l1, l2, l3, pl, keys, vals, MULTI && k > 0);
         else
             scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }
@@ -885,7 +886,8 @@ hipError_t launch_normalise_steps(double* den, const double* nvol, int32_t nn, i
 
 hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-    const int bs = a.block;
+    // the placing push's workgroup size is fixed (its LDS hash is sized for it)
+    const int bs = a.det == 4 ? 512 : a.det == 3 ? 256 : a.block;
     const int ppt = a.ppt > 0 ? a.ppt : 1;
     const unsigned g = (grid_for(a.n, (int64_t)bs * ppt) + kXcds - 1) / kXcds * kXcds;
 #define MP_PUSH(W, D, G)                                                                                  \
@@ -900,7 +902,8 @@ hipError_t launch_push_soa(const PushArgs& a, hipStream_t st) {
     } while (0)
 #define MP_PUSH_W(W)                                      \
     do {                                                  \
-        if (a.det == 3) MP_PUSH(W, 3, 0);                 \
+        if (a.det == 4) MP_PUSH(W, 4, 0);                 \
+        else if (a.det == 3) MP_PUSH(W, 3, 0);            \
         else if (a.det == 2 && a.agg == 2) MP_PUSH(W, 2, 2); \
         else if (a.det == 2) MP_PUSH(W, 2, 0);            \
         else if (a.det) MP_PUSH(W, 1, 0);                 \

@@ -33,8 +33,9 @@ struct PushArgs {
     int wall;
     int det;  // 0 atomic scatter; 1 records only; 2 no records: each node's count of particles
               // ending in its tets added to den (unit weights), the records being rebuilt from
-              // the pushed state afterwards (launch_seg_fold); 3 the records placed into the node
-              // segments of `place` (keys: pids, vals: weights)
+              // the pushed state afterwards (launch_seg_fold); 3 and 4 the records placed into
+              // the slab segments of `place` (keys: pids, vals: weights) by 256- and 512-thread
+              // workgroups
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
