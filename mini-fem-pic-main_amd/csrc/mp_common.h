@@ -332,18 +332,28 @@ __device__ __forceinline__ void place_records(const TetBlock* __restrict__ tets,
     }
     __syncthreads();
     if (!ok) return;
+    // every offset first (the claimed ones from LDS, then the rare record that found no slot
+    // claims its own), the stores after: no wait on a memory result follows a store, so none
+    // waits for an earlier record's stores to complete
+    int32_t g[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) g[v] = slot[v] >= 0 ? hc[slot[v]] + rank[v] : -1;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        if (slot[v] < 0) g[v] = atomicAdd(pl.cur + nd[v], 1);
     const double w[4] = {l0, l1, l2, l3};
+    bool fit = true;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const int32_t g = slot[v] >= 0 ? hc[slot[v]] + rank[v] : atomicAdd(pl.cur + nd[v], 1);
         const int32_t b = seg[v].x;
-        if (g < seg[v].y - b) {
-            out_pid[b + g] = pid;
-            out_val[b + g] = w[v];
+        if (g[v] < seg[v].y - b) {
+            out_pid[b + g[v]] = pid;
+            out_val[b + g[v]] = w[v];
         } else {
-            pl.flags[0] = 1;
+            fit = false;
         }
     }
+    if (!fit) pl.flags[0] = 1;
 }
 
 // The atomic scatter a kernel's template arguments select (nothing when DET == 1: the records
