@@ -658,12 +658,27 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
                 return;
             }
         }
+        // the slab ranges 64 at a time, slab q + lane's in this lane (one load for 64 slabs,
+        // not two dependent scalar loads before each slab's data), read out by lane index
+        int32_t q = 0, rs = 0, rm = 0;
+        auto chunk = [&](int32_t q0) {
+            q = q0;
+            const int32_t j = q0 + lane;
+            rs = j < ns ? sstart[b0 + j] : 0;
+            rm = j >= ns ? 0 : PLACED ? scnt[b0 + j] : sstart[b0 + j + 1] - rs;
+        };
+        auto range_at = [&](int32_t j, int64_t& st, int& m) {
+            if (j - q >= kWave) chunk(j);
+            st = __builtin_amdgcn_readlane(rs, j - q);
+            m = __builtin_amdgcn_readlane(rm, j - q);
+        };
+        chunk(0);
         // slab j + 1 is loaded into the prefetch registers while slab j is folded
         uint32_t pk[4];
         double pv[4];
         int64_t ps;
         int pm;
-        range(b0, ps, pm);
+        range_at(0, ps, pm);
         slab_load(spid, sval, ps, pm, lane, pk, pv);
         for (int32_t j = 0; j < ns; j++) {
             uint32_t ck[4];
@@ -676,7 +691,7 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
             const int64_t s = ps;
             const int m = pm;
             if (j + 1 < ns) {
-                range(b0 + j + 1, ps, pm);
+                range_at(j + 1, ps, pm);
                 slab_load(spid, sval, ps, pm, lane, pk, pv);
             }
             acc = slab_fold(ck, cv, m, lane, rl, lv, acc, sval, s);
