@@ -535,10 +535,17 @@ __device__ __forceinline__ bool slab_fold_rank(const uint32_t (&key)[R], const d
     int r[E];
 #pragma unroll
     for (int e = 0; e < E; e++) r[e] = 0;
+    // branch-free: every lane reads entry j of its sub-bucket (or, past its end, an in-bounds
+    // entry the mask drops), so the unrolled iterations keep several LDS reads in flight; the
+    // tests are sign bits (pids are under 2^31: o < key is the sign of o - key; j < n0 that of
+    // j - n0), plain VALU with no lane masks
+#pragma unroll 4
     for (int j = 0; j < cmax; j++) {
 #pragma unroll
-        for (int e = 0; e < E; e++)
-            if (j < n0[e]) r[e] += rl.pid[s0[e] + j] < key[e];
+        for (int e = 0; e < E; e++) {
+            const uint32_t o = rl.pid[min(s0[e] + j, kSegWaveMax - 1)];
+            r[e] += (int)(((o - key[e]) & (uint32_t)(j - n0[e])) >> 31);
+        }
     }
 #pragma unroll
     for (int e = 0; e < E; e++)
