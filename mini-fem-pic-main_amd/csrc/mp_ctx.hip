@@ -312,6 +312,20 @@ int reserve_place(mp_ctx* c, int64_t smax) {
     return reserve_scan(c, smax + 1);
 }
 
+// The placed-record array for a layout of `total` records (a quarter of headroom: the layouts
+// follow the counts).
+int reserve_place_rec(mp_ctx* c, int64_t total) {
+    if (total <= c->place_rec_n) return MP_OK;
+    dfree(c->place_rec);
+    c->place_rec = nullptr;
+    c->place_rec_n = 0;
+    const int64_t nc = std::max<int64_t>(total + total / 4, 4096);
+    if (dalloc(&c->place_rec, mpd::kPlacedWords * nc) != hipSuccess)
+        return set_err(c, MP_ERR_ALLOC, "placed-record array allocation failed");
+    c->place_rec_n = nc;
+    return MP_OK;
+}
+
 // Whether the next step can place its records: a layout from the last step's counts exists and
 // fits the record buffers.
 bool place_usable(mp_ctx* c) {
@@ -366,6 +380,7 @@ int place_finish(mp_ctx* c, double spwt) {
     mpd::PlaceFoldArgs f;
     f.tab = c->seg_tab;
     f.nn = nn;
+    f.rec = c->place_hot ? c->place_rec : nullptr;  // the records' layout: see mp_step
     f.spid = c->keys[1];
     f.sval = c->vals[1];
     f.tmp_pid = c->keys[0];
@@ -718,7 +733,7 @@ void mp_destroy(mp_ctx* c) {
                     c->cell_cnt, c->cell_cur, c->scan_tmp, c->keys[0], c->keys[1], c->vals[0], c->vals[1],
                     c->sort_tmp, c->seg_bounds, c->aos, c->aos_out, c->flags, c->offs, c->tally,
                     c->inj_tmp, c->cmp_buf, c->den_steps, c->perm, c->ef_stage,
-                    c->live_cnt, c->tet_rank, c->seg_tab, c->place_mem};
+                    c->live_cnt, c->tet_rank, c->seg_tab, c->place_mem, c->place_rec};
     for (void* p : bufs) dfree(p);
     if (c->pinned_i32) (void)hipHostFree(c->pinned_i32);
     pin_free(c->pin_ef);
@@ -957,6 +972,10 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
         if (!seg) HIP_TRY(c, mpd::launch_fill_u32(c->keys[0], n_rec, 0xFFFFFFFFu, c->stream));
     }
     const bool place = seg && c->opt.det_algo == 2 && place_usable(c);
+    if (place && c->place_hot) {
+        int rc = reserve_place_rec(c, c->place_total);
+        if (rc) return rc;
+    }
     {
         int rc = reserve_tally(c, c->n_slots, c->opt.block_size);
         if (rc) return rc;
@@ -968,8 +987,11 @@ int mp_step(mp_ctx* c, double q_over_m, double dt, double spwt, int32_t mode) {
     a.qm = q_over_m;
     a.dt = dt;
     a.den = c->den_raw;
-    a.keys = place ? c->keys[1] : det ? c->keys[0] : nullptr;
-    a.vals = place ? c->vals[1] : det ? c->vals[0] : nullptr;
+    // placed records: PlacedRec records where hot nodes are cut into pid slabs (det 3), else
+    // pids and weights in the grouped arrays (det 4)
+    const bool place_aos = place && c->place_hot;
+    a.keys = place_aos ? c->place_rec : place ? c->keys[1] : det ? c->keys[0] : nullptr;
+    a.vals = place_aos ? nullptr : place ? c->vals[1] : det ? c->vals[0] : nullptr;
     a.part = c->tally;
     a.ctr = c->ctr;
     a.cap = c->opt.walk_cap;

@@ -75,6 +75,8 @@ struct mp_ctx {
     bool place_pending = false; // its hot count and total are still on their way (place_ev)
     bool place_hot = false;     // the slab layout has hot nodes (records keyed by pid slab)
     int64_t place_total = 0;    // its size in records
+    uint32_t* place_rec = nullptr;  // the placed records (PlacedRec: 3 words each)
+    int64_t place_rec_n = 0;        // records it holds
     hipEvent_t place_ev = nullptr;
     int64_t det_placed = 0, det_rebuilt = 0;  // mp_get_det_paths
     // drop-in AoS scratch

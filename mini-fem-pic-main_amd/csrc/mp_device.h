@@ -95,6 +95,9 @@ struct PlaceArgs {
     unsigned long long mul;   // pid digit multiplier, floor(2^48 / next pid)
 };
 
+// Words (4 B) per placed record (mp_common.h, PlacedRec).
+constexpr int kPlacedWords = 4;
+
 enum WalkStatus : int { kInside = 0, kLost = 1, kCapped = 2 };
 
 // Most steps one multi-step launch takes (its 2048-slot LDS hash holds ~40 keys per step and

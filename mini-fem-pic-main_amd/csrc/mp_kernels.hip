@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(1024) k_tally_finish(const int32_t* __restrict
 // weight, 2 = LDS hash per workgroup.  Each thread takes ppt particles, one per round, a round
 // being blockDim.x consecutive particles; MULTI == false fixes ppt = 1 at compile time (the
 // round loop's live counters cost 8 VGPRs, one wave per SIMD of occupancy).  The placing push
-// runs fixed workgroups whatever the block size option: DET 3 256 threads, DET 4 512.
+// runs fixed workgroups whatever the block size option: DET 3 256 threads writing PlacedRec
+// records into keys (pid slabs), DET 4 512 threads writing pids and weights into keys and vals.
 template <int WALL, int DET, int AGG, bool MULTI>
 __global__ void __launch_bounds__(DET == 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(MP_PUSH_WAVES, 8)))
 k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, double dt,
@@ -108,7 +109,8 @@ k_push_soa(const TetBlock* __restrict__ tets, Soa s, int64_t n, double qm, doubl
             }
         }
         if constexpr (DET >= 3)
-            place_records<DET == 4 ? 512 : 256>(tets, ok, fc, pid, l0, l1, l2, l3, pl, keys, vals, MULTI && k > 0);
+            place_records<DET == 4 ? 512 : 256, DET == 3>(tets, ok, fc, pid, l0, l1, l2, l3, pl, keys, vals,
+                                                          MULTI && k > 0);
         else
             scatter_any<DET, AGG>(tets, den, ok, fc, l0, l1, l2, l3);
     }

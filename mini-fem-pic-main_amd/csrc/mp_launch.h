@@ -34,8 +34,8 @@ struct PushArgs {
     int det;  // 0 atomic scatter; 1 records only; 2 no records: each node's count of particles
               // ending in its tets added to den (unit weights), the records being rebuilt from
               // the pushed state afterwards (launch_seg_fold); 3 and 4 the records placed into
-              // the slab segments of `place` (keys: pids, vals: weights) by 256- and 512-thread
-              // workgroups
+              // the slab segments of `place`: 3 by 256-thread workgroups as PlacedRec records
+              // in keys (vals unused), 4 by 512-thread ones as pids in keys, weights in vals
     int agg;
     int block;
     int ppt;  // particles per thread (rounds of blockDim.x consecutive particles)
@@ -120,14 +120,16 @@ hipError_t launch_place_check(const int32_t* tab, int32_t nn, int64_t smax, cons
 hipError_t launch_place_to_raw(const int32_t* tab, int32_t nn, int64_t smax, const int32_t* cur, int32_t* scur,
                                int32_t* scan_tmp, double* raw, hipStream_t st);
 // One wave per node: the placed records of each slab in pid order, folded into out; nothing
-// when flags[0] or flags[1] is set.  A cold node over 512 records is sorted in place by a
-// workgroup (tmp_pid / tmp_val: scratch at the same offsets) and folded by one wave.
+// when flags[0] or flags[1] is set.  A cold node over 512 records is sorted in (spid, sval) by a
+// workgroup (tmp_pid / tmp_val: scratch at the same offsets; PlacedRec records are unpacked
+// there first) and folded by one wave.
 struct PlaceFoldArgs {
     const int32_t* tab;  // launch_seg_fold's table of the layout's exact step
     int32_t nn;
-    uint32_t* spid;      // the placed records
+    const uint32_t* rec;  // the placed records as PlacedRec, or nullptr: placed in (spid, sval)
+    uint32_t* spid;       // as many entries as the layout has records
     double* sval;
-    uint32_t* tmp_pid;   // as many entries as spid
+    uint32_t* tmp_pid;
     double* tmp_val;
     const int32_t* base;  // the step's segment starts
     const int32_t* cur;   // the slab counts

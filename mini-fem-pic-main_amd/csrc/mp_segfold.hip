@@ -569,6 +569,19 @@ __device__ __forceinline__ void slab_load(const uint32_t* __restrict__ spid, con
     }
 }
 
+// The same from placed 12-byte records (PlacedRec).
+template <int R>
+__device__ __forceinline__ void slab_load_rec(const uint32_t* __restrict__ rec, int64_t s, int m, int lane,
+                                              uint32_t (&key)[R], double (&val)[R]) {
+#pragma unroll
+    for (int e = 0; e < R; e++) {
+        const int g = e * kWave + lane;
+        key[e] = kNoKey;
+        val[e] = 0.0;
+        if (g < m) get_placed(rec, s + g, key[e], val[e]);
+    }
+}
+
 // Adds the slab (already in registers) to acc: the smallest register count that holds it, by
 // rank, or by the bitonic network where the rank sort declines.
 template <int E, int R>
@@ -580,25 +593,25 @@ __device__ __forceinline__ double slab_fold_e(const uint32_t (&key)[R], const do
 
 template <int R>
 __device__ __forceinline__ double slab_fold(const uint32_t (&key)[R], const double (&val)[R], int m, int lane,
-                                           RankLds& rl, double* lv, double acc, const double* __restrict__ sval,
-                                           int64_t s) {
+                                           RankLds& rl, double* lv, double acc) {
     if (R > 4 && m > 4 * kWave) return slab_fold_e<(R > 4 ? 8 : 4)>(key, val, m, lane, rl, lv, acc);
     if (m > 2 * kWave) return slab_fold_e<4>(key, val, m, lane, rl, lv, acc);
     if (m > kWave) return slab_fold_e<2>(key, val, m, lane, rl, lv, acc);
     if (m > 1) return slab_fold_e<1>(key, val, m, lane, rl, lv, acc);
-    if (m == 1 && lane == 0) acc = acc + sval[s];
+    if (m == 1 && lane == 0) acc = acc + val[0];  // record 0 is lane 0's first register
     return acc;
 }
 
 // One wave per node: the big hot nodes first, then the other hot nodes, then the cold ones (or,
 // without a hot node, wave q takes node q).  The node index is a scalar, so every branch on a
 // slab's length is wave-uniform.  Slab j + 1 is loaded into registers while slab j is sorted
-// and folded.  Slab j's records: [sstart[j], sstart[j + 1]) grouped by the tile pass, or, PLACED,
-// [sstart[j], sstart[j] + scnt[j]) placed by the push (nothing is done when flags report a record
+// and folded.  Slab j's records: [sstart[j], sstart[j + 1]) of (spid, sval) grouped by the tile
+// pass, or, PLACED, [sstart[j], sstart[j] + scnt[j]) placed by the push, as (spid, sval) or, AOS,
+// PlacedRec records in spid (nothing is done when flags report a record
 // that did not fit or a hot slab over its limit: the step is rebuilt).  A hot node with a slab
 // over 256 records (never PLACED: the check before it flags such a slab), or a placed cold node
 // over 512, is listed for the fallback with its record range instead.
-template <bool PLACED>
+template <bool PLACED, bool AOS>
 __global__ void __launch_bounds__(256)
 k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval, const int32_t* __restrict__ sbase,
                const int32_t* __restrict__ sstart, const int32_t* __restrict__ scnt,
@@ -613,6 +626,13 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int32_t q = (int32_t)((int64_t)blockIdx.x * (256 / kWave) + w);
     if (q >= nn) return;
+    // a slab's records into registers (AOS: spid holds PlacedRec records, sval is unused)
+    auto load = [&](int64_t st, int m, auto& k, auto& v) {
+        if constexpr (AOS)
+            slab_load_rec(spid, st, m, lane, k, v);
+        else
+            slab_load(spid, sval, st, m, lane, k, v);
+    };
     const int32_t n_hot = __builtin_amdgcn_readfirstlane(ctrs[0]);
     int32_t n = q;
     if (n_hot) {
@@ -647,8 +667,8 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
         }
         uint32_t key[8];
         double val[8];
-        slab_load(spid, sval, s, m, lane, key, val);
-        acc = slab_fold(key, val, m, lane, rl, lv, acc, sval, s);
+        load(s, m, key, val);
+        acc = slab_fold(key, val, m, lane, rl, lv, acc);
     } else if (ns > 1) {
         // a hot node's slabs hold 64..128 records on average; one over 256 (pids bunched within
         // the node's range) sends the node to the fallback
@@ -686,7 +706,7 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
         int64_t ps;
         int pm;
         range_at(0, ps, pm);
-        slab_load(spid, sval, ps, pm, lane, pk, pv);
+        load(ps, pm, pk, pv);
         for (int32_t j = 0; j < ns; j++) {
             uint32_t ck[4];
             double cv[4];
@@ -699,9 +719,9 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
             const int m = pm;
             if (j + 1 < ns) {
                 range_at(j + 1, ps, pm);
-                slab_load(spid, sval, ps, pm, lane, pk, pv);
+                load(ps, pm, pk, pv);
             }
-            acc = slab_fold(ck, cv, m, lane, rl, lv, acc, sval, s);
+            acc = slab_fold(ck, cv, m, lane, rl, lv, acc);
         }
     }
     if (lane == 0) out[n] = acc * (spwt / nvol[n]);
@@ -795,6 +815,20 @@ k_seg_hot_sort(uint32_t* __restrict__ pa, double* __restrict__ va, uint32_t* __r
 }
 
 // ---- records placed by the push (det_algo 2) -------------------------------------------------
+
+// The listed nodes' placed ranges [start[n], end[n]) copied from PlacedRec records into (pid,
+// weight) arrays at the same offsets, for the workgroup sort; each workgroup takes listed nodes
+// in turn.
+__global__ void __launch_bounds__(256)
+k_place_unpack(const uint32_t* __restrict__ rec, const int32_t* __restrict__ start, const int32_t* __restrict__ end,
+               const int32_t* __restrict__ list, const int32_t* __restrict__ count, uint32_t* __restrict__ pid,
+               double* __restrict__ val) {
+    const int32_t nl = *count;
+    for (int32_t h = blockIdx.x; h < nl; h += gridDim.x) {
+        const int32_t n = list[h];
+        for (int64_t i = start[n] + threadIdx.x; i < end[n]; i += blockDim.x) get_placed(rec, i, pid[i], val[i]);
+    }
+}
 
 // A slab's segment for the next step: its count now and some slack (the counts change by a few
 // per cent from one step to the next).  A cold node's one slab (lim 0) gets a quarter more and
@@ -892,7 +926,7 @@ hipError_t launch_seg_fold(const SegFoldArgs& a, hipStream_t st) {
     if (a.n > 0)
         hipLaunchKernelGGL(k_seg_tile<false>, dim3(gt), dim3(256), 0, st, a.tets, a.soa, a.n, t.sinfo, a.mul, t.ctrs,
                            t.slab_cnt, t.slab_start, t.cur, a.seg_pid, a.seg_val);
-    hipLaunchKernelGGL(k_seg_sortfold<false>, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid,
+    hipLaunchKernelGGL((k_seg_sortfold<false, false>), dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.seg_pid,
                        a.seg_val, t.sbase, t.slab_start, (const int32_t*)nullptr, t.order, t.big, t.ctrs, nn, a.nvol,
                        a.spwt, a.out, t.ovf_list, t.nstart, t.nend, (const int32_t*)nullptr);
     e = hipGetLastError();
@@ -952,14 +986,23 @@ hipError_t launch_place_to_raw(const int32_t* tab, int32_t nn, int64_t smax, con
 hipError_t launch_place_fold(const PlaceFoldArgs& a, hipStream_t st) {
     const int32_t nn = a.nn;
     const SegTab t = carve(const_cast<int32_t*>(a.tab), nn, 0);
-    hipLaunchKernelGGL(k_seg_sortfold<true>, dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.spid, a.sval,
-                       t.sbase, a.base, a.cur, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list, t.nstart,
-                       t.nend, a.flags);
+    if (a.rec)
+        hipLaunchKernelGGL((k_seg_sortfold<true, true>), dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.rec,
+                           (const double*)nullptr, t.sbase, a.base, a.cur, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt,
+                           a.out, t.ovf_list, t.nstart, t.nend, a.flags);
+    else
+        hipLaunchKernelGGL((k_seg_sortfold<true, false>), dim3(grid_for(nn, 256 / kWave)), dim3(256), 0, st, a.spid,
+                           a.sval, t.sbase, a.base, a.cur, t.order, t.big, t.ctrs, nn, a.nvol, a.spwt, a.out,
+                           t.ovf_list, t.nstart, t.nend, a.flags);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // the cold nodes listed over 512 records (usually none: the workgroups return at once)
+    // the cold nodes listed over 512 records (usually none: the workgroups return at once),
+    // sorted in (spid, sval) (PlacedRec records unpacked there first)
     const int passes = std::max(2, ((a.pid_bits + 7) / 8 + 1) / 2 * 2);
     const unsigned gh = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nn));
+    if (a.rec)
+        hipLaunchKernelGGL(k_place_unpack, dim3(gh), dim3(256), 0, st, a.rec, t.nstart, t.nend, t.ovf_list,
+                           t.ctrs + 2, a.spid, a.sval);
     hipLaunchKernelGGL(k_seg_hot_sort, dim3(gh), dim3(256), 0, st, a.spid, a.sval, a.tmp_pid, a.tmp_val, t.nstart,
                        t.nend, t.ovf_list, t.ctrs + 2, passes);
     e = hipGetLastError();
