@@ -91,7 +91,7 @@ typedef struct mp_options {
                                       step's counts (a node's one slab, or a dense node's pid
                                       slabs as 1 cuts them), and each slab is put in pid order
                                       on chip and folded (a step whose records do not fit, or
-                                      with a dense node's slab over 256 records, is done as
+                                      with a dense node's slab over 512 records, is done as
                                       with 1); 1 the push
                                       counts each node's records, which are rebuilt from the
                                       pushed state and grouped into node segments (dense nodes
@@ -192,7 +192,7 @@ int  mp_reset_counters(mp_ctx* ctx);
 /* Deterministic steps since mp_set_particles / mp_reset_counters by how their records were
  * grouped: *placed by the push into slab segments sized from the last step (det_algo 2),
  * *rebuilt from the pushed state (det_algo 1, and det_algo 2 steps whose records did not fit
- * or that had a dense node's slab over 256 records).  Either pointer may be NULL. */
+ * or that had a dense node's slab over 512 records).  Either pointer may be NULL. */
 int  mp_get_det_paths(mp_ctx* ctx, int64_t* placed, int64_t* rebuilt);
 /* Sort the resident particles by cell (or, with keep_pid_order, compact them in pid order)
  * and drop the lost ones now. */

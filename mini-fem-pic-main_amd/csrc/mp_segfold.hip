@@ -610,7 +610,8 @@ __device__ __forceinline__ double slab_fold(const uint32_t (&key)[R], const doub
 // PlacedRec records in spid (nothing is done when flags report a record
 // that did not fit or a hot slab over its limit: the step is rebuilt).  A hot node with a slab
 // over 256 records (never PLACED: the check before it flags such a slab), or a placed cold node
-// over 512, is listed for the fallback with its record range instead.
+// over 512, is listed for the fallback with its record range instead; a placed hot node's
+// chain stops at a slab over 256 and k_seg_slab_cont finishes it.
 template <bool PLACED, bool AOS>
 __global__ void __launch_bounds__(256)
 k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sval, const int32_t* __restrict__ sbase,
@@ -717,6 +718,18 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
             }
             const int64_t s = ps;
             const int m = pm;
+            (void)s;
+            if (PLACED && m > 4 * kWave) {
+                // a placed slab grown past 256 records since the layout's exact step: the node's
+                // chain goes on from slab j in k_seg_slab_cont (listed from the back of
+                // ovf_list, its slab index in nstart[n], the sum so far in out[n])
+                if (lane == 0) {
+                    ovf_list[nn - 1 - atomicAdd(ctrs + 6, 1)] = n;
+                    nstart[n] = j;
+                    out[n] = acc;
+                }
+                return;
+            }
             if (j + 1 < ns) {
                 range_at(j + 1, ps, pm);
                 load(ps, pm, pk, pv);
@@ -725,6 +738,44 @@ k_seg_sortfold(const uint32_t* __restrict__ spid, const double* __restrict__ sva
         }
     }
     if (lane == 0) out[n] = acc * (spwt / nvol[n]);
+}
+
+// The placed fold's continuation for the nodes k_seg_sortfold listed with a slab over 256
+// records (at most 512: the check flags more): one wave per node, from its recorded slab on,
+// each slab loaded whole into 8 registers per lane (a kernel of its own, so the common slab
+// loop keeps its smaller register budget); the waves stride over the list (count ctrs[6]).
+template <bool AOS>
+__global__ void __launch_bounds__(256)
+k_seg_slab_cont(const uint32_t* __restrict__ spid, const double* __restrict__ sval, const int32_t* __restrict__ sbase,
+                const int32_t* __restrict__ sstart, const int32_t* __restrict__ scnt, const int32_t* __restrict__ ctrs,
+                int32_t nn, const double* __restrict__ nvol, double spwt, double* __restrict__ out,
+                const int32_t* __restrict__ list, const int32_t* __restrict__ nstart,
+                const int32_t* __restrict__ flags) {
+    __shared__ double lval[256 / kWave][kSegWaveMax];
+    __shared__ RankLds lrank[256 / kWave];
+    if (flags[0] | flags[1]) return;
+    const int lane = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int32_t count = __builtin_amdgcn_readfirstlane(ctrs[6]);
+    const int32_t waves = (int32_t)(gridDim.x * (256 / kWave));
+    for (int32_t i = (int32_t)blockIdx.x * (256 / kWave) + w; i < count; i += waves) {
+        const int32_t n = __builtin_amdgcn_readfirstlane(list[nn - 1 - i]);
+        const int32_t b0 = __builtin_amdgcn_readfirstlane(sbase[n]);
+        const int32_t ns = __builtin_amdgcn_readfirstlane(sbase[n + 1]) - b0;
+        double acc = out[n];
+        for (int32_t j = __builtin_amdgcn_readfirstlane(nstart[n]); j < ns; j++) {
+            const int64_t st = __builtin_amdgcn_readfirstlane(sstart[b0 + j]);
+            const int m = __builtin_amdgcn_readfirstlane(scnt[b0 + j]);
+            uint32_t key[8];
+            double val[8];
+            if constexpr (AOS)
+                slab_load_rec(spid, st, m, lane, key, val);
+            else
+                slab_load(spid, sval, st, m, lane, key, val);
+            acc = slab_fold(key, val, m, lane, lrank[w], lval[w], acc);
+        }
+        if (lane == 0) out[n] = acc * (spwt / nvol[n]);
+    }
 }
 
 // The fallback for a hot node with a slab over 256 records (pids bunched within its range):
@@ -838,9 +889,10 @@ __device__ __forceinline__ int32_t place_capacity(int32_t c, int32_t lim) {
     return lim == 0 ? c + (c >> 2) + 32 : c + (c >> 3) + 16;
 }
 
-// The most records a slab may hold for the placed fold: a hot node's slabs 256 (the prefetching
-// slab loop); a cold node's one slab has no limit (lim 0: over 512 records its node is sorted
-// by the workgroup fallback, its range being contiguous).  One thread per node.
+// The most records a slab may hold for the placed fold: a hot node's slabs 512 (the prefetching
+// slab loop takes up to 256, k_seg_slab_cont the node's rest from a larger one on); a cold
+// node's one slab has no limit (lim 0: over 512 records its node is sorted by the workgroup
+// fallback, its range being contiguous).  One thread per node.
 __global__ void __launch_bounds__(256)
 k_place_lim(const int32_t* __restrict__ sbase, const int32_t* __restrict__ bits, int32_t nn,
             int32_t* __restrict__ lim) {
@@ -851,7 +903,7 @@ k_place_lim(const int32_t* __restrict__ sbase, const int32_t* __restrict__ bits,
         lim[s0] = 0;
         return;
     }
-    for (int32_t j = 0; j < (1 << b); j++) lim[s0 + j] = 4 * kWave;
+    for (int32_t j = 0; j < (1 << b); j++) lim[s0 + j] = kSegWaveMax;
 }
 
 // Capacities (smax + 1 entries, 0 past the *ns slabs) from the exact path's slab counts.
@@ -865,13 +917,13 @@ k_place_cap(const int32_t* __restrict__ cnt, const int32_t* __restrict__ lim, co
 
 // After a placing push: the cursors are the slab counts; a hot node's slab over its limit raises
 // flags[1], and the next step's capacities are written (0 past the *ns slabs, for the scan's
-// total).  The fallback list's count (ctrs[2]) is cleared for the placed fold.
+// total).  The fallback lists' counts (ctrs[2], ctrs[6]) are cleared for the placed fold.
 __global__ void __launch_bounds__(256)
 k_place_check(const int32_t* __restrict__ cur, const int32_t* __restrict__ lim, const int32_t* __restrict__ ns,
               int64_t smax, int32_t* __restrict__ flags, int32_t* __restrict__ cap, int32_t* __restrict__ ctrs) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i > smax) return;
-    if (i == 0) ctrs[2] = 0;
+    if (i == 0) ctrs[2] = ctrs[6] = 0;
     if (i < *ns) {
         const int32_t c = cur[i], l = lim[i];
         if (l && c > l) flags[1] = 1;
@@ -1000,9 +1052,16 @@ hipError_t launch_place_fold(const PlaceFoldArgs& a, hipStream_t st) {
     // sorted in (spid, sval) (PlacedRec records unpacked there first)
     const int passes = std::max(2, ((a.pid_bits + 7) / 8 + 1) / 2 * 2);
     const unsigned gh = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nn));
-    if (a.rec)
+    if (a.rec) {
+        // the nodes whose chain stopped at a slab over 256 records (usually none)
+        hipLaunchKernelGGL(k_seg_slab_cont<true>, dim3(64), dim3(256), 0, st, a.rec, (const double*)nullptr, t.sbase,
+                           a.base, a.cur, t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list, t.nstart, a.flags);
         hipLaunchKernelGGL(k_place_unpack, dim3(gh), dim3(256), 0, st, a.rec, t.nstart, t.nend, t.ovf_list,
                            t.ctrs + 2, a.spid, a.sval);
+    } else {
+        hipLaunchKernelGGL(k_seg_slab_cont<false>, dim3(64), dim3(256), 0, st, a.spid, a.sval, t.sbase, a.base, a.cur,
+                           t.ctrs, nn, a.nvol, a.spwt, a.out, t.ovf_list, t.nstart, a.flags);
+    }
     hipLaunchKernelGGL(k_seg_hot_sort, dim3(gh), dim3(256), 0, st, a.spid, a.sval, a.tmp_pid, a.tmp_val, t.nstart,
                        t.nend, t.ovf_list, t.ctrs + 2, passes);
     e = hipGetLastError();

@@ -225,3 +225,52 @@ def test_placed_hot_slabs_match_oracle(mp, orc, wall, pid_bits):
     assert hot_steps == 10
     # rebuilt: the first step (no layout yet) and the injection's
     assert placed >= 7 and rebuilt >= 2 and placed + rebuilt == 10, (placed, rebuilt)
+
+
+def test_placed_slab_over_256_continues(mp, orc):
+    """A motionless blob of 350 particles with consecutive new pids dropped into one tet of a
+    uniform run: the injection step overflows the segments and is rebuilt, which cuts the
+    blob's nodes into 8 pid slabs each, the top one holding the blob (~380 records, over the
+    slab loop's 256); the placed steps after it stop each such node's chain at that slab and
+    finish it in the continuation kernel.  Every step bit-exact against the serial fold."""
+    side = 0.1
+    mesh = mp.Mesh.box_mesh(10, side)
+    ef = mesh.field_sinusoid(10.0)
+    pos, vel = mp.synth_uniform(31, 100000, [0, 0, 0], [side] * 3, 2000.0, [0.0, 0, 0])
+    cell = mesh.locate_box(pos)
+    k_blob = 350
+    bpos = np.tile([0.0552, 0.0517, 0.0473], (k_blob, 1))
+    bvel = np.zeros((k_blob, 3))
+    bcell = mesh.locate_box(bpos)
+    run = orc.Run(mesh.tets, ef, mesh.node_volume, pos, vel, cell)
+    n0 = len(pos)
+    with mp.Mover.from_mesh(mesh) as mv:
+        mv.set_options(sort_interval=4, det_algo=2)
+        mv.upload_field(ef)
+        mv.set_particles(pos, vel, cell)
+        top = 0
+        for k in range(8):
+            if k == 3:
+                mv.add_particles(bpos, bvel, bcell)
+                run.pos = np.ascontiguousarray(np.concatenate([run.pos, bpos]))
+                run.vel = np.ascontiguousarray(np.concatenate([run.vel, bvel]))
+                run.cell = np.ascontiguousarray(np.concatenate([run.cell, bcell]).astype(np.int32))
+                run.pid = np.concatenate([run.pid, np.arange(n0, n0 + k_blob)])
+            mv.step(QM, 4e-7, SPWT, DET)
+            ref = run.step(QM, 4e-7, SPWT)
+            assert mv.density().tobytes() == ref.tobytes(), f"step {k}: density not bit-exact"
+            if k >= 4:
+                # records of the blob's first node in the top eighth of the pid digit (its top
+                # slab when it has 8): over 256
+                nodes = mesh.tets["con"][run.cell]
+                p_all = n0 + k_blob
+                mul = (1 << 48) // p_all
+                live = run.cell >= 0
+                node = int(mesh.tets["con"][bcell[0]][0])
+                hit = (nodes == node).any(axis=1) & live
+                d = (run.pid[hit].astype(np.uint64) * np.uint64(mul)) >> np.uint64(32)
+                top = max(top, int((d >> np.uint64(13) == 7).sum()))
+        check_state(mv, run, 8)
+        placed, rebuilt = mv.det_paths()
+    assert 256 < top <= 512, top
+    assert rebuilt >= 2 and placed >= 5 and placed + rebuilt == 8, (placed, rebuilt)
