@@ -426,6 +426,7 @@ constexpr int kSortThreads = 512;
 constexpr int kSortItems = 2;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kSortHash = 2048;
+static_assert(kSortHash % kSortThreads == 0, "whole hash slots per thread");
 static_assert(kSortHash >= 2 * kSortTile, "the LDS hash of a tile stays at most half full");
 
 struct SortSlots {
@@ -594,12 +595,31 @@ k_cell_scatter(Soa src, const int32_t* __restrict__ trank, int64_t n, int shift,
         sort_take(t, key, slot[j], rank[j]);
     }
     __syncthreads();
-    // each bucket of the tile claims its global range and a tile-local one
-    for (int q = threadIdx.x; q < kSortHash; q += kSortThreads) {
-        if (t.key[q] >= 0) {
-            const int32_t k = t.cnt[q];
-            t.base[q] = atomicAdd(cursor + t.key[q], k);
-            t.cnt[q] = atomicAdd(&n_live, k);
+    // each bucket of the tile claims its global range and a tile-local one, a thread's claims
+    // all issued before any is waited for
+    {
+        constexpr int kClaims = kSortHash / kSortThreads;
+        int32_t key[kClaims], k[kClaims], gb[kClaims], lb[kClaims];
+#pragma unroll
+        for (int u = 0; u < kClaims; u++) {
+            const int q = threadIdx.x + u * kSortThreads;
+            key[u] = t.key[q];
+            k[u] = key[u] >= 0 ? t.cnt[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kClaims; u++) {
+            if (key[u] >= 0) {
+                gb[u] = atomicAdd(cursor + key[u], k[u]);
+                lb[u] = atomicAdd(&n_live, k[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kClaims; u++) {
+            if (key[u] >= 0) {
+                const int q = threadIdx.x + u * kSortThreads;
+                t.base[q] = gb[u];
+                t.cnt[q] = lb[u];
+            }
         }
     }
     __syncthreads();
@@ -653,8 +673,21 @@ k_cell_scatter(Soa src, const int32_t* __restrict__ trank, int64_t n, int shift,
 // Claims each of the tile's buckets a range of its global cursor (one returning atomic per
 // bucket per tile).
 __device__ __forceinline__ void sort_claim(SortSlots& t, int32_t* __restrict__ cursor) {
-    for (int q = threadIdx.x; q < kSortHash; q += kSortThreads)
-        if (t.key[q] >= 0) t.base[q] = atomicAdd(cursor + t.key[q], t.cnt[q]);
+    constexpr int kClaims = kSortHash / kSortThreads;
+    int32_t key[kClaims], got[kClaims];
+#pragma unroll
+    for (int u = 0; u < kClaims; u++) {
+        const int q = threadIdx.x + u * kSortThreads;
+        key[u] = t.key[q];
+        got[u] = key[u] >= 0 ? t.cnt[q] : 0;
+    }
+    // all of a thread's claims in flight before any is waited for
+#pragma unroll
+    for (int u = 0; u < kClaims; u++)
+        if (key[u] >= 0) got[u] = atomicAdd(cursor + key[u], got[u]);
+#pragma unroll
+    for (int u = 0; u < kClaims; u++)
+        if (key[u] >= 0) t.base[threadIdx.x + u * kSortThreads] = got[u];
 }
 
 // The rank pass of a gather sort: k_cell_scatter's bucket ranges, but only the particle's

@@ -259,13 +259,32 @@ k_seg_tile(const TetBlock* __restrict__ tets, Soa st, int64_t n, const int2* __r
 #pragma unroll
     for (int r = 0; r < 4 * kTileItems; r++) tile_take(hk, hc, k[r], slot[r], rank[r]);
     __syncthreads();
-    for (int q = threadIdx.x; q < kSlots; q += blockDim.x) {
-        const uint32_t key = hk[q];
-        if (key == kNoKey) continue;
-        if (COUNT)
-            atomicAdd(slab_cnt + key, hc[q]);
-        else
-            hc[q] = slab_start[key] + atomicAdd(&cur[key], hc[q]);
+    // one atomic per occupied slot; a thread's returning claims go four at a time, all four
+    // in flight before any is waited for
+    constexpr int kClaims = 4;
+    static_assert(kSlots % (kClaims * 256) == 0, "whole claim groups per thread");
+    for (int q0 = threadIdx.x; q0 < kSlots; q0 += kClaims * 256) {
+        uint32_t key[kClaims];
+        int32_t got[kClaims];
+#pragma unroll
+        for (int u = 0; u < kClaims; u++) {
+            const int q = q0 + u * 256;
+            key[u] = hk[q];
+            got[u] = key[u] != kNoKey ? hc[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kClaims; u++) {
+            if (key[u] == kNoKey) continue;
+            if (COUNT)
+                atomicAdd(slab_cnt + key[u], got[u]);
+            else
+                got[u] = slab_start[key[u]] + atomicAdd(&cur[key[u]], got[u]);
+        }
+        if (!COUNT) {
+#pragma unroll
+            for (int u = 0; u < kClaims; u++)
+                if (key[u] != kNoKey) hc[q0 + u * 256] = got[u];
+        }
     }
     if (COUNT) return;
     __syncthreads();
