@@ -26,6 +26,12 @@
 // 64..128 records on average; where they are bunched and a slab exceeds 256, the node goes to
 // a slower path (a workgroup LSD radix sort of its whole range, then a wave's serial fold).
 // No pass waits on another workgroup's progress.
+// det_algo 2 keeps the slab cut of an exact step and has the push place the records of the next
+// steps (mp_common.h, place_records): k_place_lim / k_place_cap lay out each slab's segment
+// from the exact step's slab counts, k_place_check turns the placed counts into the next
+// layout, k_seg_sortfold<true, ...> folds the placed slabs (k_seg_slab_cont finishing a node
+// whose slab outgrew 256 records, k_place_unpack feeding a cold node over 512 to the workgroup
+// sort), and k_place_to_raw hands the counts to the exact path when a step is rebuilt.
 #include <algorithm>
 
 #include "mp_common.h"
