@@ -906,12 +906,12 @@ k_place_unpack(const uint32_t* __restrict__ rec, const int32_t* __restrict__ sta
     }
 }
 
-// A slab's segment for the next step: its count now and some slack (the counts change by a few
-// per cent from one step to the next).  A cold node's one slab (lim 0) gets a quarter more and
-// 32; a hot node's slab (lim 256, 64..128 records when cut) an eighth more and 16, which keeps
-// the placed arrays near 1.25 x the records where most records fall on hot nodes.
+// A slab's segment for the next step: its count now, a quarter more and 32 (the counts change
+// by a few per cent from one step to the next; a tighter 1/8 + 16 for hot slabs rebuilt one
+// step in 50 of the 200M clustered run).  Capacity costs memory only: the fold reads counts.
 __device__ __forceinline__ int32_t place_capacity(int32_t c, int32_t lim) {
-    return lim == 0 ? c + (c >> 2) + 32 : c + (c >> 3) + 16;
+    (void)lim;
+    return c + (c >> 2) + 32;
 }
 
 // The most records a slab may hold for the placed fold: a hot node's slabs 512 (the prefetching
