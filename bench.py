@@ -219,7 +219,8 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
                         "registers"}
 
 
-def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order=0, pid_bits=8):
+def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order=0, pid_bits=8, det_shift=5,
+                   det_order=1):
     """Contention stress test: a 100^3 six-tet box (side 1 m, 6,000,000 tets, 1,030,301 nodes),
     n particles from a 3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2, seeded centres in
     [0.3, 0.7]^3) so a few thousand nodes take most of the scatter, Maxwellian velocities
@@ -227,8 +228,10 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
     steps.  The same initial state is run in deterministic mode (the configuration's mode, its
     cell sort keyed by bucket and the top pid_bits of a pid digit, so that the dense nodes' pid
     slabs stay few per tile; det_algo 2: after an exact first step the push places each record
-    into its pid slab's segment, sized from the last step's counts) and in atomic mode; the
-    difference of the two step times is the cost of the sorted reduce.  A third run keeps the particles in pid order (compaction
+    into its pid slab's segment, sized from the last step's counts; its cell sort on buckets of
+    2^det_shift tets along det_order, wider buckets cheapening the sort more than they slow the
+    push) and in atomic mode (buckets of 2^shift tets along order); the difference of the two
+    step times is the cost of the sorted reduce.  A third run keeps the particles in pid order (compaction
     instead of the cell sort) with the device radix sort of pid-ordered records (det_algo 0)."""
     t = time.perf_counter()
     mesh = mp.Mesh.box_mesh(100, 1.0)
@@ -246,8 +249,10 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
         mv.upload_field(ef)
         for name, mode, pid_order in (("deterministic", mp.SCATTER_DETERMINISTIC, 0), ("atomic", mp.SCATTER_ATOMIC, 0),
                                       ("deterministic_pid_order", mp.SCATTER_DETERMINISTIC, 1)):
+            cell_det = mode == mp.SCATTER_DETERMINISTIC and not pid_order
             mv.set_options(keep_pid_order=pid_order, det_algo=0 if pid_order else 2,
-                           sort_pid_bits=pid_bits if mode == mp.SCATTER_DETERMINISTIC and not pid_order else 0)
+                           sort_pid_bits=pid_bits if cell_det else 0, sort_shift=det_shift if cell_det else shift,
+                           sort_order=det_order if cell_det else order)
             mv.set_particles(pos, vel, cell)
             for _ in range(warmup):
                 mv.step(QM, dt, SPWT, mode)
@@ -287,8 +292,9 @@ def run_contention(steps, n=200_000_000, warmup=2, seed=SEED + 5, shift=3, order
             "deterministic_pid_order": res["deterministic_pid_order"],
             "workload": f"100^3 six-tet box (6,000,000 tets, 1,030,301 nodes), {n} particles from a "
                         "3-Gaussian mixture (sigma 0.05 m, weights 0.5/0.3/0.2), Maxwellian sigma 3000 m/s, "
-                        f"no drift, reflecting walls, sort every 8 steps (shift {shift}; deterministic: "
-                        f"sort_pid_bits {pid_bits}), dt {dt:g} s"}
+                        f"no drift, reflecting walls, sort every 8 steps (shift {shift}, order {order}; "
+                        f"deterministic: shift {det_shift}, order {det_order}, sort_pid_bits {pid_bits}), "
+                        f"dt {dt:g} s"}
 
 
 ION_TEMPERATURE_1EV = 11604.518121745585  # kelvin: 1 eV / k_B
@@ -530,7 +536,11 @@ def main():
     ap.add_argument("--contention-pid-bits", type=int, default=8,
                     help="contention line, deterministic run: cell sort refined by this many pid-digit bits")
     ap.add_argument("--contention-sort-order", type=int, default=0,
-                    help="contention line: cell sort by cell id (0) or along the Morton curve (1)")
+                    help="contention line, atomic run: cell sort by cell id (0) or along the Morton curve (1)")
+    ap.add_argument("--contention-det-shift", type=int, default=5,
+                    help="contention line, deterministic run: cell sort buckets of 2^shift tets")
+    ap.add_argument("--contention-det-order", type=int, default=1,
+                    help="contention line, deterministic run: cell sort by cell id (0) or along the Morton curve (1)")
     ap.add_argument("--coupled-steps", type=int, default=200, help="steps of the coupled PIC loop (0: skip)")
     args = ap.parse_args()
     if args.many_steps is None:
@@ -611,7 +621,9 @@ def main():
         extra["adjoint"] = run_adjoint(args.adjoint_steps)
     if args.contention_steps > 0:
         extra["contention"] = run_contention(args.contention_steps, args.contention_particles,
-                                             order=args.contention_sort_order, pid_bits=args.contention_pid_bits)
+                                             order=args.contention_sort_order, pid_bits=args.contention_pid_bits,
+                                             det_shift=args.contention_det_shift,
+                                             det_order=args.contention_det_order)
     if args.coupled_steps > 0:
         extra["coupled"] = run_coupled(args.coupled_steps)
     cpu = None
