@@ -515,6 +515,8 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=10)
     ap.add_argument("--many-steps", type=int, default=None,
                     help="timed steps of the multi-step (mp_step_many) line (default: --steps; 0: skip)")
+    ap.add_argument("--many-sort-interval", type=int, default=16,
+                    help="multi-step line: steps between two sorts (one launch each), and the tuner's start")
     ap.add_argument("--headline", choices=["multi", "per-step"], default="multi",
                     help="which atomic-mode line is value/roofline (the other is reported beside it)")
     ap.add_argument("--det-steps", type=int, default=20, help="timed steps of the deterministic-mode line (0: skip)")
@@ -579,7 +581,7 @@ def main():
     extra = {}
     if args.many_steps > 0 and mode == mp.SCATTER_ATOMIC:
         m = run_gpu_many(mesh, ef, pos, vel, cell, args.many_steps, args.warmup,
-                         dict(sort_interval=args.sort_interval, sort_shift=args.sort_shift, scatter_agg=args.agg,
+                         dict(sort_interval=args.many_sort_interval, sort_shift=args.sort_shift, scatter_agg=args.agg,
                               gather_sort=args.gather_sort),
                          tune=not args.no_tune)
         log(f"gpu atomic multi-step: {m['value'] / 1e9:.3f} G particle-steps/s, push {m['push_ms_per_step']:.3f} "
