@@ -175,9 +175,11 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
         runs.append((pos, vel, mesh.locate_box(pos)))
     out = {}
     with mp.Mover.from_mesh(mesh) as mv:
-        # buckets along the Morton curve of the tet centroids: at a few particles per tet a
-        # workgroup's particles then share nodes (33-34 against 29 G/s for the multi-step line)
-        mv.set_options(sort_interval=10, sort_shift=3, sort_order=1)
+        # buckets of 16 tets along the Morton curve of the tet centroids: at a few particles per
+        # tet a workgroup's particles then share nodes (31.4 against 29 G/s in cell-id order for
+        # the multi-step line on 8-tet buckets every 10 steps); sorting every 12 steps on 16-tet
+        # buckets: 32.4-32.6 G/s over 100 steps, the batched and sequential lines unchanged
+        mv.set_options(sort_interval=12, sort_shift=4, sort_order=1)
         mv.upload_field(ef)
         for how in ("batched", "sequential", "multi_step"):
             with mp.Batch(mv, n_runs) as b:
@@ -213,9 +215,9 @@ def run_ensemble(steps, warmup, n_runs=64, seed=SEED + 3):
             "steps": steps, "runs": n_runs, "particles": int(counts.sum()),
             "workload": "64 runs on a 40^3 six-tet box (384,000 tets), counts log-uniform 1e5..4e6, "
                         "drift U[2,8] km/s, sigma U[500,3000] m/s, frozen sinusoidal field, dt 4e-7 s, "
-                        "cell sort every 10 steps on the Morton rank of the tet >> 3; "
+                        "cell sort every 12 steps on the Morton rank of the tet >> 4; "
                         "one hipGraph launch per step vs 64 per-run launch sequences; multi_step: "
-                        "mp_batch_step_many, one launch per sort interval (10 steps) with the particles in "
+                        "mp_batch_step_many, one launch per sort interval (12 steps) with the particles in "
                         "registers"}
 
 
